@@ -1,0 +1,631 @@
+// extern "C" entry points (include/of_hip.h). Each wraps the device launches of one
+// reference interface, translating C++ exceptions into status codes plus a thread-local
+// message that carries the reference's exception text.
+
+#include "stepper.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+namespace ofh {
+
+void* Context::scratch_get(int slot, size_t bytes) {
+  if((int)scratch.size() <= slot) scratch.resize(slot + 1, {nullptr, 0});
+  auto& s = scratch[slot];
+  if(s.second < bytes) {
+    if(s.first) OFH_CHECK(hipStreamSynchronize(stream));
+    if(s.first) OFH_CHECK(hipFree(s.first));
+    OFH_CHECK(hipMalloc(&s.first, bytes));
+    s.second = bytes;
+  }
+  return s.first;
+}
+
+void Context::prof_begin(const char*, hipEvent_t* end) {
+  if(prof_used == prof_events.size()) {
+    hipEvent_t a, b;
+    OFH_CHECK(hipEventCreate(&a));
+    OFH_CHECK(hipEventCreate(&b));
+    prof_events.emplace_back(a, b);
+  }
+  auto& e = prof_events[prof_used++];
+  OFH_CHECK(hipEventRecord(e.first, stream));
+  *end = e.second;
+}
+
+Context::~Context() {
+  for(auto& e : prof_events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for(auto& s : scratch)
+    if(s.first) (void)hipFree(s.first);
+  if(partials) (void)hipFree(partials);
+  if(counters) (void)hipFree(counters);
+  if(scal) (void)hipFree(scal);
+  if(host_scal) (void)hipHostFree(host_scal);
+  if(own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+bool axis_periodic(const of_bc_set& b, int axis);
+
+}  // namespace ofh
+
+using namespace ofh;
+
+struct of_context {
+  Context ctx;
+};
+struct of_stepper {
+  Stepper* s;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return OF_OK;
+  } catch(const InvalidArgument& e) {
+    g_error = e.what();
+    return OF_INVALID_ARGUMENT;
+  } catch(const std::invalid_argument& e) {
+    g_error = e.what();
+    return OF_INVALID_ARGUMENT;
+  } catch(const HipError& e) {
+    g_error = e.what();
+    return OF_HIP_ERROR;
+  } catch(const std::exception& e) {
+    g_error = e.what();
+    return OF_RUNTIME_ERROR;
+  } catch(...) {
+    g_error = "unknown error";
+    return OF_RUNTIME_ERROR;
+  }
+}
+
+Context& C(of_context* c) {
+  if(c == nullptr) throw InvalidArgument("context must not be null");
+  OFH_CHECK(hipSetDevice(c->ctx.device));
+  return c->ctx;
+}
+
+void need(const void* p, const char* what) {
+  if(p == nullptr) throw InvalidArgument(std::string(what) + " must not be null");
+}
+
+of_poisson_diagnostics to_diag(const PoissonResult& r, double* hist, int cap) {
+  of_poisson_diagnostics d = r.diag;
+  if(hist != nullptr)
+    for(int i = 0; i < std::min(cap, (int)r.history.size()); ++i) hist[i] = r.history[i];
+  return d;
+}
+
+void check_projection_options(const of_projection_options* o, const char* fn) {
+  need(o, "options");
+  if(!(o->dt > 0.0)) throw InvalidArgument(std::string(fn) + " expects dt > 0");
+  if(!(o->density > 0.0)) throw InvalidArgument(std::string(fn) + " expects density > 0");
+}
+
+size_t fsize(const of_grid& g, int loc) { return make_lay(g, loc).size() * sizeof(double); }
+
+}  // namespace
+
+extern "C" {
+
+const char* of_last_error(void) { return g_error.c_str(); }
+
+const char* of_build_info(void) {
+  return "solver_amd gfx950 projection step (HIP " __DATE__ ")";
+}
+
+int of_context_create(int device, void* stream, of_context** out) {
+  return guarded([&] {
+    need(out, "out");
+    auto* c = new of_context();
+    try {
+      Context& x = c->ctx;
+      x.device = device;
+      OFH_CHECK(hipSetDevice(device));
+      if(stream) {
+        x.stream = static_cast<hipStream_t>(stream);
+      } else {
+        OFH_CHECK(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+        x.own_stream = true;
+      }
+      OFH_CHECK(hipMalloc(&x.partials, sizeof(double) * kMaxRed * kMaxPartials));
+      OFH_CHECK(hipMalloc(&x.counters, sizeof(unsigned) * kCounterCount));
+      OFH_CHECK(hipMemset(x.counters, 0, sizeof(unsigned) * kCounterCount));
+      OFH_CHECK(hipMalloc(&x.scal, sizeof(double) * kScalCount));
+      OFH_CHECK(hipMemset(x.scal, 0, sizeof(double) * kScalCount));
+      OFH_CHECK(hipHostMalloc(&x.host_scal, sizeof(double) * kScalCount, hipHostMallocDefault));
+    } catch(...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int of_context_destroy(of_context* c) {
+  return guarded([&] {
+    if(c == nullptr) return;
+    (void)hipSetDevice(c->ctx.device);
+    (void)hipStreamSynchronize(c->ctx.stream);
+    delete c;
+  });
+}
+
+int of_profile_kernel(of_context* c, const char* name) {
+  return guarded([&] {
+    Context& x = C(c);
+    OFH_CHECK(hipStreamSynchronize(x.stream));
+    x.prof_name = name ? name : "";
+    x.prof_used = 0;
+  });
+}
+
+int of_profile_read(of_context* c, int* launches, double* total_ms, double* min_ms, double* max_ms) {
+  return guarded([&] {
+    Context& x = C(c);
+    OFH_CHECK(hipStreamSynchronize(x.stream));
+    double tot = 0.0, mn = 1e300, mx = 0.0;
+    for(size_t i = 0; i < x.prof_used; ++i) {
+      float ms = 0.0f;
+      OFH_CHECK(hipEventElapsedTime(&ms, x.prof_events[i].first, x.prof_events[i].second));
+      tot += ms;
+      mn = std::min(mn, (double)ms);
+      mx = std::max(mx, (double)ms);
+    }
+    if(launches) *launches = (int)x.prof_used;
+    if(total_ms) *total_ms = tot;
+    if(min_ms) *min_ms = x.prof_used ? mn : 0.0;
+    if(max_ms) *max_ms = mx;
+    x.prof_used = 0;
+  });
+}
+
+int of_context_synchronize(of_context* c) {
+  return guarded([&] { OFH_CHECK(hipStreamSynchronize(C(c).stream)); });
+}
+
+int of_apply_velocity_boundary_conditions(of_context* c, const of_grid* g, const of_bc_set* b, double* u, double* v,
+                                          double* w) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    need(u, "u"), need(v, "v"), need(w, "w");
+    launch_velocity_bc(C(c), *g, *b, u, v, w);
+  });
+}
+
+int of_apply_pressure_boundary_conditions(of_context* c, const of_grid* g, const of_pressure_bc_set* b, double* p) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    need(p, "p");
+    launch_pressure_bc(C(c), *g, *b, p);
+  });
+}
+
+int of_apply_total_pressure_boundary_conditions(of_context* c, const of_grid* g, const of_bc_set* b, const double* su,
+                                                const double* sv, const double* sw, double* p) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    need(p, "p_total");
+    launch_total_pressure_bc(C(c), *g, *b, su, sv, sw, p);
+  });
+}
+
+int of_derive_pressure_correction_boundary_conditions(const of_bc_set* b, of_pressure_bc_set* out) {
+  return guarded([&] {
+    need(b, "bcs");
+    need(out, "out");
+    for(int a = 0; a < 3; ++a) (void)axis_periodic(*b, a);
+    *out = derive_pressure_bcs(*b);
+  });
+}
+
+int of_compute_gradient(of_context* c, const of_grid* g, const double* p, double* gx, double* gy, double* gz) {
+  return guarded([&] {
+    validate_grid(g);
+    launch_gradient(C(c), *g, p, gx, gy, gz);
+  });
+}
+
+int of_compute_divergence(of_context* c, const of_grid* g, const double* u, const double* v, const double* w,
+                          double* d) {
+  return guarded([&] {
+    validate_grid(g);
+    launch_divergence(C(c), *g, u, v, w, d);
+  });
+}
+
+int of_compute_laplacian(of_context* c, const of_grid* g, int loc, const double* in, double* out) {
+  return guarded([&] {
+    validate_grid(g);
+    if(loc < OF_CELL || loc > OF_FACE_Z) throw InvalidArgument("compute_laplacian: unknown field location");
+    launch_laplacian(C(c), *g, loc, 1.0, in, out);
+  });
+}
+
+int of_compute_advection_term(of_context* c, const of_grid* g, int scheme, const double* u, const double* v,
+                              const double* w, double* au, double* av, double* aw) {
+  return guarded([&] {
+    validate_grid(g);
+    if(scheme < 0 || scheme > 2) throw InvalidArgument("compute_advection_term: unknown advection scheme");
+    launch_advection(C(c), *g, scheme, u, v, w, au, av, aw);
+  });
+}
+
+int of_compute_diffusion_term(of_context* c, const of_grid* g, double nu, const double* u, const double* v,
+                              const double* w, double* du, double* dv, double* dw) {
+  return guarded([&] {
+    validate_grid(g);
+    Context& x = C(c);
+    const double* in[3] = {u, v, w};
+    double* out[3] = {du, dv, dw};
+    for(int a = 0; a < 3; ++a) {
+      // compute_laplacian then scale_active(viscosity): lap * nu
+      launch_laplacian(x, *g, OF_FACE_X + a, 1.0, in[a], out[a]);
+      launch_scale_active(x, make_lay(*g, OF_FACE_X + a), nu, out[a], out[a]);
+    }
+  });
+}
+
+int of_compute_advective_cfl(of_context* c, const of_grid* g, const double* u, const double* v, const double* w,
+                             double dt, of_cfl_diagnostics* out) {
+  return guarded([&] {
+    validate_grid(g);
+    need(out, "out");
+    if(dt < 0.0) throw InvalidArgument("compute_advective_cfl expects a non-negative dt");
+    Context& x = C(c);
+    launch_cfl(x, *g, u, v, w, dt, x.scal + kScalCfl);
+    OFH_CHECK(hipMemcpyAsync(x.host_scal, x.scal + kScalCfl, 5 * sizeof(double), hipMemcpyDeviceToHost, x.stream));
+    OFH_CHECK(hipStreamSynchronize(x.stream));
+    out->dt = x.host_scal[0];
+    out->max_cfl = x.host_scal[1];
+    out->max_u = x.host_scal[2];
+    out->max_v = x.host_scal[3];
+    out->max_w = x.host_scal[4];
+  });
+}
+
+int of_compute_factorized_correction(of_context* c, const of_grid* g, int variant, double alpha, const double* u,
+                                     const double* v, const double* w, double* fu, double* fv, double* fw) {
+  return guarded([&] {
+    validate_grid(g);
+    launch_factorized_correction(C(c), *g, variant, alpha, u, v, w, fu, fv, fw);
+  });
+}
+
+int of_assemble_predictor_rhs(of_context* c, const of_grid* g, int variant, int scheme, double nu, double dt,
+                              const double* u, const double* v, const double* w, const double* p, const double* pu,
+                              const double* pv, const double* pw, double* au, double* av, double* aw, double* ru,
+                              double* rv, double* rw) {
+  return guarded([&] {
+    validate_grid(g);
+    Context& x = C(c);
+    const bool prev = pu != nullptr;
+    if(prev && (pv == nullptr || pw == nullptr)) throw InvalidArgument("previous advection must be all or none");
+    // predictor_rhs = current_velocity (whole storage), then active-range updates
+    const double* in[3] = {u, v, w};
+    double* out[3] = {ru, rv, rw};
+    for(int a = 0; a < 3; ++a)
+      OFH_CHECK(hipMemcpyAsync(out[a], in[a], fsize(*g, OF_FACE_X + a), hipMemcpyDeviceToDevice, x.stream));
+    launch_predictor_rhs(x, *g, variant, scheme, nu, dt, u, v, w, p, pu, pv, pw, au, av, aw, ru, rv, rw);
+  });
+}
+
+int of_solve_predictor_adi(of_context* c, const of_grid* g, const of_bc_set* b, double alpha, const double* ru,
+                           const double* rv, const double* rw, double* ou, double* ov, double* ow, int* line_solves) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    const int n = predictor_adi(C(c), *g, *b, alpha, ru, rv, rw, ou, ov, ow);
+    if(line_solves) *line_solves = n;
+  });
+}
+
+int of_build_pressure_rhs(of_context* c, const of_grid* g, const of_bc_set* b, const of_projection_options* o,
+                          const double* u, const double* v, const double* w, double* rhs) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    check_projection_options(o, "build_pressure_rhs");
+    Context& x = C(c);
+    // working = predicted + BC ghosts (solver/projection.cpp:1060-1063)
+    double* wk[3];
+    const double* in[3] = {u, v, w};
+    for(int a = 0; a < 3; ++a) {
+      wk[a] = static_cast<double*>(x.scratch_get(a, fsize(*g, OF_FACE_X + a)));
+      OFH_CHECK(hipMemcpyAsync(wk[a], in[a], fsize(*g, OF_FACE_X + a), hipMemcpyDeviceToDevice, x.stream));
+    }
+    launch_velocity_bc(x, *g, *b, wk[0], wk[1], wk[2]);
+    launch_divergence_scaled(x, *g, wk[0], wk[1], wk[2], o->density / o->dt, rhs);
+  });
+}
+
+int of_correct_velocity(of_context* c, const of_grid* g, const of_bc_set* b, const of_projection_options* o,
+                        const double* u, const double* v, const double* w, const double* phi, double* ou, double* ov,
+                        double* ow) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    check_projection_options(o, "correct_velocity");
+    Context& x = C(c);
+    const of_pressure_bc_set pb = derive_pressure_bcs(*b);
+    double* pg = static_cast<double*>(x.scratch_get(3, fsize(*g, OF_CELL)));
+    OFH_CHECK(hipMemcpyAsync(pg, phi, fsize(*g, OF_CELL), hipMemcpyDeviceToDevice, x.stream));
+    launch_pressure_bc(x, *g, pb, pg);
+    const double* in[3] = {u, v, w};
+    double* out[3] = {ou, ov, ow};
+    for(int a = 0; a < 3; ++a)
+      OFH_CHECK(hipMemcpyAsync(out[a], in[a], fsize(*g, OF_FACE_X + a), hipMemcpyDeviceToDevice, x.stream));
+    launch_correct(x, *g, u, v, w, pg, -o->dt / o->density, out[0], out[1], out[2]);
+    launch_velocity_bc(x, *g, *b, ou, ov, ow);
+  });
+}
+
+int of_solve_pressure_poisson(of_context* c, const of_grid* g, const of_pressure_bc_set* b,
+                              const of_projection_options* o, int precision, const double* rhs, double* p,
+                              of_poisson_diagnostics* diag, double* hist, int cap) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    need(o, "options");
+    if(o->poisson_max_iterations <= 0) throw InvalidArgument("solve_pressure_poisson expects iterations > 0");
+    if(!(o->poisson_tolerance > 0.0)) throw InvalidArgument("solve_pressure_poisson expects tolerance > 0");
+    Context& x = C(c);
+    PoissonSolver* s = poisson_create(x, *g, *b, precision);
+    PoissonResult r;
+    try {
+      r = poisson_solve(s, rhs, p, o->poisson_max_iterations, o->poisson_tolerance, hist != nullptr);
+    } catch(...) {
+      poisson_destroy(s);
+      throw;
+    }
+    poisson_destroy(s);
+    const of_poisson_diagnostics d = to_diag(r, hist, cap);
+    if(diag) *diag = d;
+  });
+}
+
+int of_build_poisson_rhs_from_pressure(of_context* c, const of_grid* g, const of_pressure_bc_set* b, const double* p,
+                                       double* rhs) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    Context& x = C(c);
+    double* tmp = static_cast<double*>(x.scratch_get(4, fsize(*g, OF_CELL)));
+    launch_poisson_apply(x, *g, *b, false, p, tmp);
+    launch_scale_active(x, make_lay(*g, OF_CELL), -1.0, tmp, rhs);
+  });
+}
+
+int of_apply_poisson_operator(of_context* c, const of_grid* g, const of_pressure_bc_set* b, const double* in,
+                              double* out) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    launch_poisson_apply(C(c), *g, *b, false, in, out);
+  });
+}
+
+int of_project_velocity(of_context* c, const of_grid* g, const of_bc_set* b, const of_projection_options* o,
+                        int precision, const double* u, const double* v, const double* w, double* phi, double* ou,
+                        double* ov, double* ow, double* prhs_out, of_projection_diagnostics* diag, double* hist,
+                        int cap) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    check_projection_options(o, "project_velocity");
+    Context& x = C(c);
+    const of_pressure_bc_set pb = derive_pressure_bcs(*b);
+    const Lay Lc = make_lay(*g, OF_CELL);
+    double* wk[3];
+    const double* in[3] = {u, v, w};
+    for(int a = 0; a < 3; ++a) {
+      wk[a] = static_cast<double*>(x.scratch_get(10 + a, fsize(*g, OF_FACE_X + a)));
+      OFH_CHECK(hipMemcpyAsync(wk[a], in[a], fsize(*g, OF_FACE_X + a), hipMemcpyDeviceToDevice, x.stream));
+    }
+    launch_velocity_bc(x, *g, *b, wk[0], wk[1], wk[2]);
+    double* div = static_cast<double*>(x.scratch_get(13, fsize(*g, OF_CELL)));
+    double* rhs = static_cast<double*>(x.scratch_get(14, fsize(*g, OF_CELL)));
+    OFH_CHECK(hipMemsetAsync(div, 0, fsize(*g, OF_CELL), x.stream));
+    OFH_CHECK(hipMemsetAsync(rhs, 0, fsize(*g, OF_CELL), x.stream));
+    of_projection_diagnostics d{};
+    launch_divergence(x, *g, wk[0], wk[1], wk[2], div);
+    launch_norms(x, Lc, div, kScalRed0);
+    launch_divergence_scaled(x, *g, wk[0], wk[1], wk[2], o->density / o->dt, rhs);
+    launch_norms(x, Lc, rhs, kScalRed0 + 2);
+    OFH_CHECK(hipMemcpyAsync(x.host_scal, x.scal + kScalRed0, 4 * sizeof(double), hipMemcpyDeviceToHost, x.stream));
+    OFH_CHECK(hipStreamSynchronize(x.stream));
+    const double ncell = (double)g->nx * g->ny * g->nz;
+    d.divergence_l2_before = std::sqrt(x.host_scal[0] / ncell);
+    d.rhs_l2 = std::sqrt(x.host_scal[2] / ncell);
+    PoissonSolver* s = poisson_create(x, *g, pb, precision);
+    PoissonResult r;
+    try {
+      r = poisson_solve(s, rhs, phi, o->poisson_max_iterations, o->poisson_tolerance, hist != nullptr);
+    } catch(...) {
+      poisson_destroy(s);
+      throw;
+    }
+    poisson_destroy(s);
+    d.pressure_solve = to_diag(r, hist, cap);
+    // correct_velocity(working, phi, ...)
+    double* pg = static_cast<double*>(x.scratch_get(3, fsize(*g, OF_CELL)));
+    OFH_CHECK(hipMemcpyAsync(pg, phi, fsize(*g, OF_CELL), hipMemcpyDeviceToDevice, x.stream));
+    launch_pressure_bc(x, *g, pb, pg);
+    double* out[3] = {ou, ov, ow};
+    for(int a = 0; a < 3; ++a)
+      OFH_CHECK(hipMemcpyAsync(out[a], wk[a], fsize(*g, OF_FACE_X + a), hipMemcpyDeviceToDevice, x.stream));
+    launch_correct(x, *g, wk[0], wk[1], wk[2], pg, -o->dt / o->density, out[0], out[1], out[2]);
+    launch_velocity_bc(x, *g, *b, ou, ov, ow);
+    launch_divergence(x, *g, ou, ov, ow, div);
+    launch_norms(x, Lc, div, kScalRed0);
+    OFH_CHECK(hipMemcpyAsync(x.host_scal, x.scal + kScalRed0, 2 * sizeof(double), hipMemcpyDeviceToHost, x.stream));
+    OFH_CHECK(hipStreamSynchronize(x.stream));
+    d.divergence_l2_after = std::sqrt(x.host_scal[0] / ncell);
+    d.divergence_linf_after = x.host_scal[1];
+    {
+      // mean of phi over the active range
+      std::vector<double> hp(make_lay(*g, OF_CELL).size());
+      OFH_CHECK(hipMemcpy(hp.data(), phi, fsize(*g, OF_CELL), hipMemcpyDeviceToHost));
+      double s2 = 0.0;
+      for(int k = 1; k <= g->nz; ++k)
+        for(int j = 1; j <= g->ny; ++j)
+          for(int i = 1; i <= g->nx; ++i) s2 += hp[Lc.at(i, j, k)];
+      d.pressure_mean = s2 / ncell;
+    }
+    if(prhs_out) {
+      launch_scale_active(x, Lc, 1.0, rhs, prhs_out);
+      OFH_CHECK(hipStreamSynchronize(x.stream));
+    }
+    if(diag) *diag = d;
+  });
+}
+
+int of_taylor_green_dt(int nx, int ny, int nz, double cfl, double final_time, double* dt, int* steps) {
+  return guarded([&] {
+    if(nx <= 0 || ny <= 0 || nz <= 0) throw InvalidArgument("Taylor-Green config expects positive nx, ny, and nz");
+    if(!(cfl > 0.0)) throw InvalidArgument("Taylor-Green config expects cfl_limit > 0");
+    if(!(final_time > 0.0)) throw InvalidArgument("Taylor-Green config expects final_time > 0");
+    const double L = 2.0 * std::acos(-1.0);
+    const double dx = L / (double)nx, dy = L / (double)ny, dz = nz > 1 ? L / (double)nz : 1.0;
+    const double inv = 1.0 / dx + 1.0 / dy + (nz == 1 ? 0.0 : 1.0 / dz);
+    const double upper = cfl / inv;
+    const int n = std::max(1, (int)std::ceil(final_time / upper));
+    if(dt) *dt = final_time / (double)n;
+    if(steps) *steps = n;
+  });
+}
+
+int of_lid_driven_cavity_dt(int nx, int ny, double lid, double cfl, double* dt) {
+  return guarded([&] {
+    if(nx <= 0 || ny <= 0) throw InvalidArgument("lid-driven cavity config expects positive nx and ny");
+    const double dx = 1.0 / (double)nx, dy = 1.0 / (double)ny;
+    *dt = cfl / (lid / dx + lid / dy);
+  });
+}
+
+int of_stepper_create(of_context* c, const of_stepper_config* cfg, of_stepper** out) {
+  return guarded([&] {
+    need(cfg, "config");
+    need(out, "out");
+    auto* s = new of_stepper{nullptr};
+    try {
+      s->s = new Stepper(C(c), *cfg);
+    } catch(...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+int of_stepper_destroy(of_stepper* s) {
+  return guarded([&] {
+    if(!s) return;
+    (void)hipSetDevice(s->s->c.device);
+    delete s->s;
+    delete s;
+  });
+}
+
+int of_stepper_initialize(of_stepper* s, int convention) {
+  return guarded([&] {
+    need(s, "stepper");
+    OFH_CHECK(hipSetDevice(s->s->c.device));
+    s->s->initialize(convention);
+  });
+}
+
+int of_stepper_upload(of_stepper* s, const double* u, const double* v, const double* w, const double* p,
+                      const double* au, const double* av, const double* aw, int has_prev, const of_step_metrics* m) {
+  return guarded([&] {
+    need(s, "stepper");
+    Stepper& S = *s->s;
+    OFH_CHECK(hipSetDevice(S.c.device));
+    const double* vin[3] = {u, v, w};
+    const double* ain[3] = {au, av, aw};
+    for(int a = 0; a < 3; ++a) {
+      need(vin[a], "velocity");
+      OFH_CHECK(hipMemcpy(S.vel[a], vin[a], fsize(S.g, OF_FACE_X + a), hipMemcpyHostToDevice));
+      if(ain[a]) OFH_CHECK(hipMemcpy(S.adv[a], ain[a], fsize(S.g, OF_FACE_X + a), hipMemcpyHostToDevice));
+      else OFH_CHECK(hipMemset(S.adv[a], 0, fsize(S.g, OF_FACE_X + a)));
+    }
+    need(p, "p_total");
+    OFH_CHECK(hipMemcpy(S.p_total, p, fsize(S.g, OF_CELL), hipMemcpyHostToDevice));
+    S.has_prev = has_prev ? 1 : 0;
+    if(m) S.metrics = *m;
+    else {
+      std::memset(&S.metrics, 0, sizeof S.metrics);
+      S.metrics.dt = S.cfg.dt;
+    }
+  });
+}
+
+int of_stepper_download(of_stepper* s, double* u, double* v, double* w, double* p, double* au, double* av, double* aw,
+                        int* has_prev, of_step_metrics* m) {
+  return guarded([&] {
+    need(s, "stepper");
+    Stepper& S = *s->s;
+    OFH_CHECK(hipSetDevice(S.c.device));
+    OFH_CHECK(hipStreamSynchronize(S.c.stream));
+    double* vo[3] = {u, v, w};
+    double* ao[3] = {au, av, aw};
+    for(int a = 0; a < 3; ++a) {
+      if(vo[a]) OFH_CHECK(hipMemcpy(vo[a], S.vel[a], fsize(S.g, OF_FACE_X + a), hipMemcpyDeviceToHost));
+      if(ao[a]) OFH_CHECK(hipMemcpy(ao[a], S.adv[a], fsize(S.g, OF_FACE_X + a), hipMemcpyDeviceToHost));
+    }
+    if(p) OFH_CHECK(hipMemcpy(p, S.p_total, fsize(S.g, OF_CELL), hipMemcpyDeviceToHost));
+    if(has_prev) *has_prev = S.has_prev;
+    if(m) *m = S.metrics;
+  });
+}
+
+int of_stepper_device_fields(of_stepper* s, double** f) {
+  return guarded([&] {
+    need(s, "stepper");
+    need(f, "fields");
+    Stepper& S = *s->s;
+    f[0] = S.vel[0];
+    f[1] = S.vel[1];
+    f[2] = S.vel[2];
+    f[3] = S.p_total;
+    f[4] = S.adv[0];
+    f[5] = S.adv[1];
+    f[6] = S.adv[2];
+  });
+}
+
+int of_stepper_run(of_stepper* s, int steps, of_step_metrics* m) {
+  return guarded([&] {
+    need(s, "stepper");
+    if(steps < 0) throw InvalidArgument("run_taylor_green_steps expects a non-negative step count");
+    Stepper& S = *s->s;
+    OFH_CHECK(hipSetDevice(S.c.device));
+    for(int i = 0; i < steps; ++i) S.step();
+    if(m) *m = S.metrics;
+  });
+}
+
+int of_stepper_run_batched(of_stepper* s, int steps, of_step_metrics* m) { return of_stepper_run(s, steps, m); }
+
+int of_stepper_kinetic_energy(of_stepper* s, double* out) {
+  return guarded([&] {
+    need(s, "stepper");
+    need(out, "out");
+    OFH_CHECK(hipSetDevice(s->s->c.device));
+    *out = s->s->kinetic_energy();
+  });
+}
+
+}  // extern "C"

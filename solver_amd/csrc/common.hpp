@@ -1,0 +1,181 @@
+// Shared device/host helpers of the gfx950 projection-step library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "of_hip.h"
+
+namespace ofh {
+
+// ------------------------------------------------------------------ errors
+struct InvalidArgument : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+struct RuntimeError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define OFH_CHECK(call)                                                                                   \
+  do {                                                                                                    \
+    hipError_t e_ = (call);                                                                               \
+    if(e_ != hipSuccess)                                                                                  \
+      throw ::ofh::HipError(std::string(#call) + " failed: " + hipGetErrorString(e_));                    \
+  } while(0)
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// ------------------------------------------------------------------ layout
+// Ghosted storage of core/field_layout.hpp:69-290 with one ghost layer.
+struct Lay {
+  int ax, ay, az;  // active extents
+  int sx, sy, sz;  // storage extents
+  __host__ __device__ inline size_t at(int i, int j, int k) const {
+    return (size_t)i + (size_t)sx * ((size_t)j + (size_t)sy * (size_t)k);
+  }
+  __host__ __device__ inline size_t size() const { return (size_t)sx * sy * sz; }
+  __host__ __device__ inline int a(int d) const { return d == 0 ? ax : d == 1 ? ay : az; }
+  __host__ __device__ inline int s(int d) const { return d == 0 ? sx : d == 1 ? sy : sz; }
+};
+
+__host__ __device__ inline Lay make_lay(int nx, int ny, int nz, int loc) {
+  Lay L;
+  L.ax = nx + (loc == OF_FACE_X);
+  L.ay = ny + (loc == OF_FACE_Y);
+  L.az = nz + (loc == OF_FACE_Z);
+  L.sx = L.ax + 2;
+  L.sy = L.ay + 2;
+  L.sz = L.az + 2;
+  return L;
+}
+__host__ __device__ inline Lay make_lay(const of_grid& g, int loc) { return make_lay(g.nx, g.ny, g.nz, loc); }
+
+__host__ __device__ inline double spacing(const of_grid& g, int axis) {
+  return axis == 0 ? g.dx : axis == 1 ? g.dy : g.dz;
+}
+
+inline void validate_grid(const of_grid* g) {
+  if(g == nullptr) throw InvalidArgument("grid must not be null");
+  if(g->nx <= 0 || g->ny <= 0 || g->nz <= 0) throw InvalidArgument("grid dimensions must be positive");
+  if(!(g->dx > 0.0) || !(g->dy > 0.0) || !(g->dz > 0.0)) throw InvalidArgument("grid spacing must be positive");
+}
+
+// ------------------------------------------------------------------ launch shapes
+// 2D thread blocks 64 (i) x 4 (j) over one k-plane per grid z: a wave reads 64 consecutive
+// doubles of a row (coalesced 512-byte segments).
+constexpr int kBX = 64;
+constexpr int kBY = 4;
+constexpr int kThreads = kBX * kBY;
+
+inline dim3 grid_for(int ex, int ey, int ez) {
+  return dim3((unsigned)((ex + kBX - 1) / kBX), (unsigned)((ey + kBY - 1) / kBY), (unsigned)ez);
+}
+
+inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ deterministic reductions
+// A reduction kernel writes one partial per block (fixed order in memory); the last block to
+// finish (detected with a device-scope counter) combines the partials in a fixed order, so
+// the result is bitwise reproducible for a given launch shape. The counter is reset by the
+// last block, so it needs no clearing between launches.
+constexpr int kMaxRed = 8;           // values per reduction
+constexpr int kMaxPartials = 65536;  // partial slots per reduction buffer
+
+enum RedOp : int { kSum = 0, kMax = 1 };
+
+__device__ inline double wave_sum(double v) {
+  for(int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for(int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Block-wide reduction of NV values, result valid in thread 0. ops[k] selects sum or max.
+template <int NV>
+__device__ inline void block_reduce(double (&v)[NV], const int (&ops)[NV]) {
+  __shared__ double red[16][NV];
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int nwaves = (nthreads + 63) >> 6;
+#pragma unroll
+  for(int k = 0; k < NV; ++k) v[k] = ops[k] == kSum ? wave_sum(v[k]) : wave_max(v[k]);
+  if(lane == 0) {
+#pragma unroll
+    for(int k = 0; k < NV; ++k) red[wave][k] = v[k];
+  }
+  __syncthreads();
+  if(tid == 0) {
+#pragma unroll
+    for(int k = 0; k < NV; ++k) {
+      double acc = red[0][k];
+      for(int w = 1; w < nwaves; ++w) acc = ops[k] == kSum ? acc + red[w][k] : fmax(acc, red[w][k]);
+      v[k] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// Writes this block's partials; returns true (in every thread of the block) if this block is
+// the last one to finish, in which case `out` holds the combined values (valid in thread 0)
+// and the counter has been reset.
+template <int NV>
+__device__ inline bool reduce_partials(double (&v)[NV], const int (&ops)[NV], double* partials, unsigned* counter,
+                                       double (&out)[NV]) {
+  block_reduce<NV>(v, ops);
+  const unsigned nblocks = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+  __shared__ int last;
+  if(tid == 0) {
+#pragma unroll
+    for(int k = 0; k < NV; ++k) partials[(size_t)k * kMaxPartials + bid] = v[k];
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    last = prev == nblocks - 1;
+  }
+  __syncthreads();
+  if(!last) return false;
+  __threadfence();
+  double acc[NV];
+#pragma unroll
+  for(int k = 0; k < NV; ++k) acc[k] = ops[k] == kSum ? 0.0 : -INFINITY;
+  for(unsigned b = tid; b < nblocks; b += nthreads) {
+#pragma unroll
+    for(int k = 0; k < NV; ++k) {
+      const double x = partials[(size_t)k * kMaxPartials + b];
+      acc[k] = ops[k] == kSum ? acc[k] + x : fmax(acc[k], x);
+    }
+  }
+  block_reduce<NV>(acc, ops);
+  if(tid == 0) {
+#pragma unroll
+    for(int k = 0; k < NV; ++k) out[k] = acc[k];
+    atomicExch(counter, 0u);
+  }
+  return true;
+}
+
+// ------------------------------------------------------------------ XCD-aware block order
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each with a private
+// L2. Remapping the linear block id makes each XCD own a contiguous run of tiles, so the
+// halo planes shared by neighbouring tiles mostly hit the same L2.
+__device__ inline unsigned xcd_remap(unsigned bid, unsigned nblocks) {
+  constexpr unsigned kXcd = 8;
+  const unsigned per = nblocks / kXcd;
+  if(per == 0) return bid;
+  const unsigned full = per * kXcd;
+  if(bid >= full) return bid;
+  return (bid % kXcd) * per + bid / kXcd;
+}
+
+}  // namespace ofh

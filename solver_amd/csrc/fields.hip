@@ -1,0 +1,710 @@
+// Field kernels: boundary-condition fills, MAC operators, TVD advection, the factorisation
+// correction and the fused predictor right-hand side, all on the reference's ghosted storage.
+//
+// Floating-point expressions keep the reference's association order (this file is built with
+// -ffp-contract=off), so per-point results match the reference to the last bit where the
+// inputs do.
+
+#include "ofh.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace ofh {
+
+ColumnLaunch column_launch(int ex, int ey, int ez, int min_kz) {
+  ColumnLaunch l;
+  l.block = dim3(kBX, kBY, 1);
+  const unsigned gx = (unsigned)((ex + kBX - 1) / kBX), gy = (unsigned)((ey + kBY - 1) / kBY);
+  const size_t plane = (size_t)gx * gy;
+  int kz = std::max(1, min_kz);
+  // keep the block count below the partial-slot capacity and near 8 blocks per CU x 256 CUs
+  while(plane * (size_t)((ez + kz - 1) / kz) > (size_t)std::min(kMaxPartials, 16384) && kz < ez) kz *= 2;
+  kz = std::min(kz, std::max(ez, 1));
+  l.kz = kz;
+  l.grid = dim3(gx, gy, (unsigned)((ez + kz - 1) / kz));
+  return l;
+}
+
+namespace {
+
+// ------------------------------------------------------------------ boundary conditions
+struct Slab {
+  int axis, o0, o1;
+};
+__device__ inline Slab slab_of(int axis) {
+  Slab s;
+  s.axis = axis;
+  s.o0 = axis == 0 ? 1 : 0;
+  s.o1 = axis == 2 ? 1 : 2;
+  return s;
+}
+__device__ inline size_t slab_at(const Lay& L, const Slab& s, int along, int t0, int t1) {
+  int c[3];
+  c[s.axis] = along;
+  c[s.o0] = 1 + t0;
+  c[s.o1] = 1 + t1;
+  return L.at(c[0], c[1], c[2]);
+}
+
+// solver/projection.cpp:233-300 (periodic) and :384-449 (walls) for one component along one axis
+__device__ inline void component_axis_bc(double* f, const Lay& L, int comp, int axis, int periodic, const of_bc& lo,
+                                         const of_bc& hi, int t0, int t1) {
+  const Slab s = slab_of(axis);
+  if(t0 >= L.a(s.o0) || t1 >= L.a(s.o1)) return;
+  const int A = L.a(axis);
+  auto F = [&](int along) -> double& { return f[slab_at(L, s, along, t0, t1)]; };
+  if(periodic) {
+    if(comp == axis) {
+      const double shared = 0.5 * (F(1) + F(A));
+      F(1) = shared;
+      F(A) = shared;
+      F(0) = F(A - 1);
+      F(A + 1) = F(2);
+    } else {
+      F(0) = F(A);
+      F(A + 1) = F(1);
+    }
+    return;
+  }
+  for(int side = 0; side < 2; ++side) {
+    const of_bc& c = side == 0 ? lo : hi;
+    const bool lower = side == 0;
+    if(c.type == OF_PERIODIC) continue;
+    const int b = lower ? 1 : A;
+    const int gh = lower ? 0 : A + 1;
+    if(comp == axis) {
+      const int ii = lower ? b + 1 : b - 1;
+      if(c.type == OF_FIXED_PRESSURE) {
+        F(b) = F(ii);
+        F(gh) = F(b);
+        continue;
+      }
+      const double bv = c.type == OF_SYMMETRY ? 0.0 : c.velocity[comp];
+      F(b) = bv;
+      F(gh) = 2.0 * bv - F(ii);
+      continue;
+    }
+    if(c.type == OF_NO_SLIP_WALL || c.type == OF_PRESCRIBED_VELOCITY) {
+      F(gh) = 2.0 * c.velocity[comp] - F(b);
+    } else if(c.type == OF_SYMMETRY || c.type == OF_FIXED_PRESSURE) {
+      F(gh) = F(b);
+    }
+  }
+}
+
+__global__ void k_velocity_bc_axis(double* u, double* v, double* w, Lay Lu, Lay Lv, Lay Lw, int axis, int periodic,
+                                   of_bc lo, of_bc hi, int comp_mask) {
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t1 = blockIdx.y * blockDim.y + threadIdx.y;
+  if(comp_mask & 1) component_axis_bc(u, Lu, 0, axis, periodic, lo, hi, t0, t1);
+  if(comp_mask & 2) component_axis_bc(v, Lv, 1, axis, periodic, lo, hi, t0, t1);
+  if(comp_mask & 4) component_axis_bc(w, Lw, 2, axis, periodic, lo, hi, t0, t1);
+}
+
+// solver/projection.cpp:207-231,302-337,973-984
+__global__ void k_pressure_bc_axis(double* p, Lay L, int axis, int periodic, of_pressure_bc lo, of_pressure_bc hi,
+                                   double h) {
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t1 = blockIdx.y * blockDim.y + threadIdx.y;
+  const Slab s = slab_of(axis);
+  if(t0 >= L.a(s.o0) || t1 >= L.a(s.o1)) return;
+  const int A = L.a(axis);
+  auto F = [&](int along) -> double& { return p[slab_at(L, s, along, t0, t1)]; };
+  if(periodic) {
+    F(0) = F(A);
+    F(A + 1) = F(1);
+    return;
+  }
+  for(int side = 0; side < 2; ++side) {
+    const of_pressure_bc& c = side == 0 ? lo : hi;
+    const bool lower = side == 0;
+    const double av = F(lower ? 1 : A);
+    double& g = F(lower ? 0 : A + 1);
+    if(c.type == OF_PRESSURE_NEUMANN) g = lower ? av - c.gradient * h : av + c.gradient * h;
+    else if(c.type == OF_PRESSURE_DIRICHLET) g = 2.0 * c.value - av;
+  }
+}
+
+// solver/projection.cpp:339-382,986-1026
+__global__ void k_total_pressure_bc_axis(double* p, Lay L, const double* src, Lay Ls, int axis, int periodic,
+                                         of_bc lo, of_bc hi, double h) {
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t1 = blockIdx.y * blockDim.y + threadIdx.y;
+  const Slab s = slab_of(axis);
+  if(t0 >= L.a(s.o0) || t1 >= L.a(s.o1)) return;
+  const int A = L.a(axis);
+  auto F = [&](int along) -> double& { return p[slab_at(L, s, along, t0, t1)]; };
+  if(periodic) {
+    F(0) = F(A);
+    F(A + 1) = F(1);
+    return;
+  }
+  for(int side = 0; side < 2; ++side) {
+    const of_bc& c = side == 0 ? lo : hi;
+    const bool lower = side == 0;
+    const double av = F(lower ? 1 : A);
+    double& g = F(lower ? 0 : A + 1);
+    if(c.type == OF_NO_SLIP_WALL || c.type == OF_PRESCRIBED_VELOCITY || c.type == OF_SYMMETRY) {
+      const double grad = src[slab_at(Ls, s, lower ? 1 : Ls.a(axis), t0, t1)];
+      g = lower ? av - grad * h : av + grad * h;
+    } else if(c.type == OF_FIXED_PRESSURE) {
+      g = 2.0 * c.pressure - av;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ MAC operators
+// operators/discrete_operators.cpp:77-109 for one component
+__global__ void k_gradient(const double* __restrict__ p, Lay Lp, double* __restrict__ g, Lay Lg, int axis, double h,
+                           int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > Lg.ax || j > Lg.ay) return;
+  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + Lg.az);
+  const int di = axis == 0, dj = axis == 1, dk = axis == 2;
+  for(int k = kb; k < ke; ++k) g[Lg.at(i, j, k)] = (p[Lp.at(i, j, k)] - p[Lp.at(i - di, j - dj, k - dk)]) / h;
+}
+
+// operators/discrete_operators.cpp:111-126 (times `scale`, as build_pressure_rhs does when
+// scale != 1: solver/projection.cpp:1062-1063)
+__global__ void k_divergence(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
+                             Lay Lu, Lay Lv, Lay Lw, Lay Lc, of_grid g, double scale, int apply_scale,
+                             double* __restrict__ d, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > Lc.ax || j > Lc.ay) return;
+  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + Lc.az);
+  for(int k = kb; k < ke; ++k) {
+    const double a = (u[Lu.at(i + 1, j, k)] - u[Lu.at(i, j, k)]) / g.dx;
+    const double b = (v[Lv.at(i, j + 1, k)] - v[Lv.at(i, j, k)]) / g.dy;
+    const double c = (w[Lw.at(i, j, k + 1)] - w[Lw.at(i, j, k)]) / g.dz;
+    const double dv = a + b + c;
+    d[Lc.at(i, j, k)] = apply_scale ? dv * scale : dv;
+  }
+}
+
+// operators/discrete_operators.cpp:37-73 (+ the viscosity scaling of compute_diffusion_term,
+// solver/momentum_terms.cpp:362-371, when apply_scale)
+__device__ inline double lap7(const double* f, const Lay& L, int i, int j, int k, double ix, double iy, double iz) {
+  const double c = f[L.at(i, j, k)];
+  const double a = (f[L.at(i + 1, j, k)] - 2.0 * c + f[L.at(i - 1, j, k)]) * ix;
+  const double b = (f[L.at(i, j + 1, k)] - 2.0 * c + f[L.at(i, j - 1, k)]) * iy;
+  const double e = (f[L.at(i, j, k + 1)] - 2.0 * c + f[L.at(i, j, k - 1)]) * iz;
+  return a + b + e;
+}
+
+__global__ void k_laplacian(const double* __restrict__ in, double* __restrict__ out, Lay L, double ix, double iy,
+                            double iz, double scale, int apply_scale, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > L.ax || j > L.ay) return;
+  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + L.az);
+  for(int k = kb; k < ke; ++k) {
+    const double l = lap7(in, L, i, j, k, ix, iy, iz);
+    out[L.at(i, j, k)] = apply_scale ? l * scale : l;
+  }
+}
+
+// ------------------------------------------------------------------ TVD advection
+constexpr double kSmall = 1.0e-14;  // solver/momentum_terms.cpp:62
+
+__device__ inline double van_leer(double r) { return (r + fabs(r)) / (1.0 + fabs(r)); }
+__device__ inline double safe_ratio(double n, double d) { return fabs(d) < kSmall ? 0.0 : n / d; }
+
+// solver/momentum_terms.cpp:88-197. `f` points at the value at storage index `r` along the
+// axis; `stride` is the storage stride of that axis; `max_index` its last storage index.
+__device__ inline double reconstruct(const double* f, ptrdiff_t stride, int r, int max_index, double vel,
+                                     int scheme) {
+  const double left = f[-stride], right = f[0];
+  if(scheme == 2) return 0.5 * (left + right);
+  if(scheme == 1 || fabs(vel) < kSmall) return vel >= 0.0 ? left : right;
+  if(vel >= 0.0) {
+    if(r - 2 < 0) return left;
+    const double dd = right - left;
+    const double ratio = safe_ratio(left - f[-2 * stride], dd);
+    return left + 0.5 * van_leer(ratio) * dd;
+  }
+  if(r + 1 > max_index) return right;
+  const double d = right - left;
+  const double ratio = safe_ratio(f[stride] - right, d);
+  return right - 0.5 * van_leer(ratio) * d;
+}
+
+struct VelPtrs {
+  const double* u;
+  const double* v;
+  const double* w;
+  Lay Lu, Lv, Lw;
+};
+
+// solver/momentum_terms.cpp:199-312: conservative flux divergence of component `c` at
+// storage point (i,j,k) of that component's layout.
+__device__ inline double advect_point(const VelPtrs& V, int c, int i, int j, int k, const of_grid& g, int scheme) {
+  double wv, ev, sv, nv, bv, fv;
+  const double* u = V.u;
+  const double* v = V.v;
+  const double* w = V.w;
+  const Lay &Lu = V.Lu, &Lv = V.Lv, &Lw = V.Lw;
+  if(c == 0) {
+    wv = 0.5 * (u[Lu.at(i - 1, j, k)] + u[Lu.at(i, j, k)]);
+    ev = 0.5 * (u[Lu.at(i, j, k)] + u[Lu.at(i + 1, j, k)]);
+    sv = 0.5 * (v[Lv.at(i - 1, j, k)] + v[Lv.at(i, j, k)]);
+    nv = 0.5 * (v[Lv.at(i - 1, j + 1, k)] + v[Lv.at(i, j + 1, k)]);
+    bv = 0.5 * (w[Lw.at(i - 1, j, k)] + w[Lw.at(i, j, k)]);
+    fv = 0.5 * (w[Lw.at(i - 1, j, k + 1)] + w[Lw.at(i, j, k + 1)]);
+  } else if(c == 1) {
+    wv = 0.5 * (u[Lu.at(i, j - 1, k)] + u[Lu.at(i, j, k)]);
+    ev = 0.5 * (u[Lu.at(i + 1, j - 1, k)] + u[Lu.at(i + 1, j, k)]);
+    sv = 0.5 * (v[Lv.at(i, j - 1, k)] + v[Lv.at(i, j, k)]);
+    nv = 0.5 * (v[Lv.at(i, j, k)] + v[Lv.at(i, j + 1, k)]);
+    bv = 0.5 * (w[Lw.at(i, j - 1, k)] + w[Lw.at(i, j, k)]);
+    fv = 0.5 * (w[Lw.at(i, j - 1, k + 1)] + w[Lw.at(i, j, k + 1)]);
+  } else {
+    wv = 0.5 * (u[Lu.at(i, j, k - 1)] + u[Lu.at(i, j, k)]);
+    ev = 0.5 * (u[Lu.at(i + 1, j, k - 1)] + u[Lu.at(i + 1, j, k)]);
+    sv = 0.5 * (v[Lv.at(i, j, k - 1)] + v[Lv.at(i, j, k)]);
+    nv = 0.5 * (v[Lv.at(i, j + 1, k - 1)] + v[Lv.at(i, j + 1, k)]);
+    bv = 0.5 * (w[Lw.at(i, j, k - 1)] + w[Lw.at(i, j, k)]);
+    fv = 0.5 * (w[Lw.at(i, j, k)] + w[Lw.at(i, j, k + 1)]);
+  }
+  const double* q = c == 0 ? u : c == 1 ? v : w;
+  const Lay& L = c == 0 ? Lu : c == 1 ? Lv : Lw;
+  const ptrdiff_t sj = L.sx, sk = (ptrdiff_t)L.sx * L.sy;
+  const double* p = q + L.at(i, j, k);
+  const double wf = wv * reconstruct(p, 1, i, L.sx - 1, wv, scheme);
+  const double ef = ev * reconstruct(p + 1, 1, i + 1, L.sx - 1, ev, scheme);
+  const double sf = sv * reconstruct(p, sj, j, L.sy - 1, sv, scheme);
+  const double nf = nv * reconstruct(p + sj, sj, j + 1, L.sy - 1, nv, scheme);
+  const double bf = bv * reconstruct(p, sk, k, L.sz - 1, bv, scheme);
+  const double ff = fv * reconstruct(p + sk, sk, k + 1, L.sz - 1, fv, scheme);
+  return (ef - wf) / g.dx + (nf - sf) / g.dy + (ff - bf) / g.dz;
+}
+
+__global__ void k_advection(VelPtrs V, int c, of_grid g, int scheme, double* __restrict__ out, int kz) {
+  const Lay& L = c == 0 ? V.Lu : c == 1 ? V.Lv : V.Lw;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > L.ax || j > L.ay) return;
+  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + L.az);
+  for(int k = kb; k < ke; ++k) out[L.at(i, j, k)] = advect_point(V, c, i, j, k, g, scheme);
+}
+
+// ------------------------------------------------------------------ factorisation correction
+// solver/taylor_green.cpp:132-225 (3D/2D) at storage point (i,j,k)
+__device__ inline double sd_axis(const double* f, const Lay& L, int i, int j, int k, int axis, double inv) {
+  const int di = axis == 0, dj = axis == 1, dk = axis == 2;
+  return (f[L.at(i + di, j + dj, k + dk)] - 2.0 * f[L.at(i, j, k)] + f[L.at(i - di, j - dj, k - dk)]) * inv;
+}
+__device__ inline double mixed_sd(const double* f, const Lay& L, int i, int j, int k, int outer, int inner,
+                                  const double* inv) {
+  const int di = outer == 0, dj = outer == 1, dk = outer == 2;
+  return (sd_axis(f, L, i + di, j + dj, k + dk, inner, inv[inner]) - 2.0 * sd_axis(f, L, i, j, k, inner, inv[inner]) +
+          sd_axis(f, L, i - di, j - dj, k - dk, inner, inv[inner])) *
+         inv[outer];
+}
+__device__ inline double fc_point_tg(const double* f, const Lay& L, int i, int j, int k, const double* inv,
+                                     bool three_d, double alpha) {
+  double corr = mixed_sd(f, L, i, j, k, 0, 1, inv);
+  if(three_d) {
+    corr += mixed_sd(f, L, i, j, k, 0, 2, inv);
+    corr += mixed_sd(f, L, i, j, k, 1, 2, inv);
+    const double t = (mixed_sd(f, L, i + 1, j, k, 1, 2, inv) - 2.0 * mixed_sd(f, L, i, j, k, 1, 2, inv) +
+                      mixed_sd(f, L, i - 1, j, k, 1, 2, inv)) *
+                     inv[0];
+    corr -= alpha * t;
+  }
+  return corr;
+}
+// solver/lid_driven_cavity.cpp:219-252
+__device__ inline double fc_point_cavity(const double* f, const Lay& L, int i, int j, int k, double dy2,
+                                         double idx2) {
+  const double a = (f[L.at(i - 1, j + 1, k)] - 2.0 * f[L.at(i - 1, j, k)] + f[L.at(i - 1, j - 1, k)]) / dy2;
+  const double b = (f[L.at(i, j + 1, k)] - 2.0 * f[L.at(i, j, k)] + f[L.at(i, j - 1, k)]) / dy2;
+  const double c = (f[L.at(i + 1, j + 1, k)] - 2.0 * f[L.at(i + 1, j, k)] + f[L.at(i + 1, j - 1, k)]) / dy2;
+  return (c - 2.0 * b + a) * idx2;
+}
+
+struct FcParams {
+  double inv[3];
+  double alpha, dy2, idx2;
+  int variant, three_d;
+};
+
+__device__ inline double fc_point(const double* f, const Lay& L, int i, int j, int k, const FcParams& P) {
+  return P.variant == 0 ? fc_point_tg(f, L, i, j, k, P.inv, P.three_d != 0, P.alpha)
+                        : fc_point_cavity(f, L, i, j, k, P.dy2, P.idx2);
+}
+
+__global__ void k_fc(const double* __restrict__ f, Lay L, FcParams P, double* __restrict__ out, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > L.ax || j > L.ay) return;
+  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + L.az);
+  for(int k = kb; k < ke; ++k) out[L.at(i, j, k)] = fc_point(f, L, i, j, k, P);
+}
+
+// ------------------------------------------------------------------ fused predictor RHS
+// solver/taylor_green.cpp:227-256 / solver/lid_driven_cavity.cpp:489-518 for component c:
+//   rhs = q - dt*grad p + (dt/2)*nu*lap q - dt*(3/2 A - 1/2 A_prev) + alpha^2 * FC(q)
+// evaluated as the reference's chain of axpy updates; A is written to `adv` after A_prev has
+// been read from `prev` (they may alias).
+struct RhsParams {
+  of_grid g;
+  int c, scheme, has_prev;
+  double ix, iy, iz, nu;
+  double s_grad, s_dif, s_adv1, s_prev, s_adv0, s_fc;
+  FcParams fc;
+};
+
+__global__ void k_predictor_rhs(VelPtrs V, const double* __restrict__ p, Lay Lp, const double* prev, double* adv,
+                                double* __restrict__ rhs, RhsParams P, int kz) {
+  const int c = P.c;
+  const Lay& L = c == 0 ? V.Lu : c == 1 ? V.Lv : V.Lw;
+  const double* q = c == 0 ? V.u : c == 1 ? V.v : V.w;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > L.ax || j > L.ay) return;
+  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + L.az);
+  const int di = c == 0, dj = c == 1, dk = c == 2;
+  const double h = c == 0 ? P.g.dx : c == 1 ? P.g.dy : P.g.dz;
+  for(int k = kb; k < ke; ++k) {
+    const size_t at = L.at(i, j, k);
+    const double a = advect_point(V, c, i, j, k, P.g, P.scheme);
+    const double dif = lap7(q, L, i, j, k, P.ix, P.iy, P.iz) * P.nu;
+    const double grad = (p[Lp.at(i, j, k)] - p[Lp.at(i - di, j - dj, k - dk)]) / h;
+    const double fc = fc_point(q, L, i, j, k, P.fc);
+    double r = q[at];
+    r += P.s_grad * grad;
+    r += P.s_dif * dif;
+    if(P.has_prev) {
+      const double pa = prev[at];
+      r += P.s_adv1 * a;
+      r += P.s_prev * pa;
+    } else {
+      r += P.s_adv0 * a;
+    }
+    r += P.s_fc * fc;
+    adv[at] = a;
+    rhs[at] = r;
+  }
+}
+
+// ------------------------------------------------------------------ CFL diagnostic
+// solver/momentum_terms.cpp:373-401
+__global__ void k_cfl(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w, Lay Lu,
+                      Lay Lv, Lay Lw, of_grid g, double dt, double* partials, unsigned* counter, double* out, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  double vals[4] = {0.0, 0.0, 0.0, 0.0};
+  if(i <= g.nx && j <= g.ny) {
+    const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + g.nz);
+    for(int k = kb; k < ke; ++k) {
+      const double uc = 0.5 * (u[Lu.at(i, j, k)] + u[Lu.at(i + 1, j, k)]);
+      const double vc = 0.5 * (v[Lv.at(i, j, k)] + v[Lv.at(i, j + 1, k)]);
+      const double wc = 0.5 * (w[Lw.at(i, j, k)] + w[Lw.at(i, j, k + 1)]);
+      vals[1] = fmax(vals[1], fabs(uc));
+      vals[2] = fmax(vals[2], fabs(vc));
+      vals[3] = fmax(vals[3], fabs(wc));
+      const double local = dt * (fabs(uc) / g.dx + fabs(vc) / g.dy + fabs(wc) / g.dz);
+      vals[0] = fmax(vals[0], local);
+    }
+  }
+  const int ops[4] = {kMax, kMax, kMax, kMax};
+  double res[4];
+  if(reduce_partials<4>(vals, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
+    out[0] = dt;
+    out[1] = res[0];
+    out[2] = res[1];
+    out[3] = res[2];
+    out[4] = res[3];
+  }
+}
+
+// sum of squares and max |x| over the active box
+__global__ void k_norms(const double* __restrict__ x, Lay L, double* partials, unsigned* counter, double* out, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  double vals[2] = {0.0, 0.0};
+  if(i <= L.ax && j <= L.ay) {
+    const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + L.az);
+    for(int k = kb; k < ke; ++k) {
+      const double a = x[L.at(i, j, k)];
+      vals[0] += a * a;
+      vals[1] = fmax(vals[1], fabs(a));
+    }
+  }
+  const int ops[2] = {kSum, kMax};
+  double res[2];
+  if(reduce_partials<2>(vals, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
+    out[0] = res[0];
+    out[1] = res[1];
+  }
+}
+
+__global__ void k_scale_active(const double* in, double* out, Lay L, double s, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > L.ax || j > L.ay) return;
+  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + L.az);
+  for(int k = kb; k < ke; ++k) out[L.at(i, j, k)] = in[L.at(i, j, k)] * s;
+}
+
+// solver/projection.cpp:1082-1093: out = pred + scale * (phi(i) - phi(i-1))/h on every
+// active face (scale = -dt/density); phi carries ghosts.
+__global__ void k_correct(const double* __restrict__ pred, Lay L, const double* __restrict__ phi, Lay Lp, int axis,
+                          double h, double scale, double* __restrict__ out, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > L.ax || j > L.ay) return;
+  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + L.az);
+  const int di = axis == 0, dj = axis == 1, dk = axis == 2;
+  for(int k = kb; k < ke; ++k) {
+    const double gr = (phi[Lp.at(i, j, k)] - phi[Lp.at(i - di, j - dj, k - dk)]) / h;
+    out[L.at(i, j, k)] = pred[L.at(i, j, k)] + scale * gr;
+  }
+}
+
+dim3 slab_grid(int e0, int e1, dim3 block) {
+  return dim3((unsigned)((e0 + block.x - 1) / block.x), (unsigned)((e1 + block.y - 1) / block.y), 1);
+}
+
+void slab_extents(const of_grid& g, int axis, int& e0, int& e1) {
+  // largest other-axis extents over the three face layouts (each is at most n+1)
+  const int n[3] = {g.nx, g.ny, g.nz};
+  const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
+  e0 = n[o0] + 1;
+  e1 = n[o1] + 1;
+}
+
+VelPtrs vel_ptrs(const of_grid& g, const double* u, const double* v, const double* w) {
+  VelPtrs V;
+  V.u = u;
+  V.v = v;
+  V.w = w;
+  V.Lu = make_lay(g, OF_FACE_X);
+  V.Lv = make_lay(g, OF_FACE_Y);
+  V.Lw = make_lay(g, OF_FACE_Z);
+  return V;
+}
+
+FcParams fc_params(const of_grid& g, int variant, double alpha) {
+  FcParams P;
+  P.inv[0] = 1.0 / (g.dx * g.dx);
+  P.inv[1] = 1.0 / (g.dy * g.dy);
+  P.inv[2] = 1.0 / (g.dz * g.dz);
+  P.alpha = alpha;
+  P.dy2 = g.dy * g.dy;
+  P.idx2 = 1.0 / (g.dx * g.dx);
+  P.variant = variant;
+  P.three_d = g.nz != 1;
+  return P;
+}
+
+}  // namespace
+
+bool axis_periodic(const of_bc_set& b, int axis) {
+  const bool lo = b.faces[2 * axis].type == OF_PERIODIC, hi = b.faces[2 * axis + 1].type == OF_PERIODIC;
+  if(lo != hi) throw InvalidArgument("periodic velocity boundaries must be paired on both sides");
+  return lo;
+}
+
+bool pressure_axis_periodic(const of_pressure_bc_set& b, int axis) {
+  const bool lo = b.faces[2 * axis].type == OF_PRESSURE_PERIODIC, hi = b.faces[2 * axis + 1].type == OF_PRESSURE_PERIODIC;
+  if(lo != hi) throw InvalidArgument("periodic pressure boundaries must be paired on both sides");
+  return lo;
+}
+
+void launch_component_bc_mask(Context& c, const of_grid& g, const of_bc_set& b, double* u, double* v, double* w,
+                              int mask) {
+  const Lay Lu = make_lay(g, OF_FACE_X), Lv = make_lay(g, OF_FACE_Y), Lw = make_lay(g, OF_FACE_Z);
+  const dim3 block(64, 4, 1);
+  for(int axis = 0; axis < 3; ++axis) {
+    int e0, e1;
+    slab_extents(g, axis, e0, e1);
+    const int per = axis_periodic(b, axis) ? 1 : 0;
+    hipLaunchKernelGGL(k_velocity_bc_axis, slab_grid(e0, e1, block), block, 0, c.stream, u, v, w, Lu, Lv, Lw, axis,
+                       per, b.faces[2 * axis], b.faces[2 * axis + 1], mask);
+  }
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_velocity_bc(Context& c, const of_grid& g, const of_bc_set& b, double* u, double* v, double* w) {
+  launch_component_bc_mask(c, g, b, u, v, w, 7);
+}
+
+void launch_component_bc(Context& c, const of_grid& g, const of_bc_set& b, int comp, double* f) {
+  launch_component_bc_mask(c, g, b, comp == 0 ? f : nullptr, comp == 1 ? f : nullptr, comp == 2 ? f : nullptr,
+                           1 << comp);
+}
+
+void launch_pressure_bc(Context& c, const of_grid& g, const of_pressure_bc_set& b, double* p) {
+  const Lay L = make_lay(g, OF_CELL);
+  const dim3 block(64, 4, 1);
+  for(int axis = 0; axis < 3; ++axis) {
+    const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
+    const int per = pressure_axis_periodic(b, axis) ? 1 : 0;
+    hipLaunchKernelGGL(k_pressure_bc_axis, slab_grid(L.a(o0), L.a(o1), block), block, 0, c.stream, p, L, axis, per,
+                       b.faces[2 * axis], b.faces[2 * axis + 1], spacing(g, axis));
+  }
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_total_pressure_bc(Context& c, const of_grid& g, const of_bc_set& b, const double* su, const double* sv,
+                              const double* sw, double* p) {
+  const Lay L = make_lay(g, OF_CELL);
+  const dim3 block(64, 4, 1);
+  for(int axis = 0; axis < 3; ++axis) {
+    const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
+    const int per = axis_periodic(b, axis) ? 1 : 0;
+    const double* src = axis == 0 ? su : axis == 1 ? sv : sw;
+    hipLaunchKernelGGL(k_total_pressure_bc_axis, slab_grid(L.a(o0), L.a(o1), block), block, 0, c.stream, p, L, src,
+                       make_lay(g, OF_FACE_X + axis), axis, per, b.faces[2 * axis], b.faces[2 * axis + 1],
+                       spacing(g, axis));
+  }
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_gradient(Context& c, const of_grid& g, const double* p, double* gx, double* gy, double* gz) {
+  const Lay Lp = make_lay(g, OF_CELL);
+  double* outs[3] = {gx, gy, gz};
+  for(int a = 0; a < 3; ++a) {
+    const Lay L = make_lay(g, OF_FACE_X + a);
+    const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
+    hipLaunchKernelGGL(k_gradient, cl.grid, cl.block, 0, c.stream, p, Lp, outs[a], L, a, spacing(g, a), cl.kz);
+  }
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_divergence(Context& c, const of_grid& g, const double* u, const double* v, const double* w, double* d) {
+  const Lay Lc = make_lay(g, OF_CELL);
+  const ColumnLaunch cl = column_launch(Lc.ax, Lc.ay, Lc.az);
+  {
+    ProfScope prof_(c, "k_divergence", true);
+    hipLaunchKernelGGL(k_divergence, cl.grid, cl.block, 0, c.stream, u, v, w, make_lay(g, OF_FACE_X),
+                       make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), Lc, g, 1.0, 0, d, cl.kz);
+  }
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_divergence_scaled(Context& c, const of_grid& g, const double* u, const double* v, const double* w,
+                              double scale, double* d) {
+  const Lay Lc = make_lay(g, OF_CELL);
+  const ColumnLaunch cl = column_launch(Lc.ax, Lc.ay, Lc.az);
+  {
+    ProfScope prof_(c, "k_divergence", true);
+    hipLaunchKernelGGL(k_divergence, cl.grid, cl.block, 0, c.stream, u, v, w, make_lay(g, OF_FACE_X),
+                       make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), Lc, g, scale, 1, d, cl.kz);
+  }
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_laplacian(Context& c, const of_grid& g, int loc, double scale, const double* in, double* out) {
+  const Lay L = make_lay(g, loc);
+  const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
+  hipLaunchKernelGGL(k_laplacian, cl.grid, cl.block, 0, c.stream, in, out, L, 1.0 / (g.dx * g.dx),
+                     1.0 / (g.dy * g.dy), 1.0 / (g.dz * g.dz), scale, scale != 1.0 ? 1 : 0, cl.kz);
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_advection(Context& c, const of_grid& g, int scheme, const double* u, const double* v, const double* w,
+                      double* au, double* av, double* aw) {
+  const VelPtrs V = vel_ptrs(g, u, v, w);
+  double* outs[3] = {au, av, aw};
+  for(int comp = 0; comp < 3; ++comp) {
+    const Lay L = make_lay(g, OF_FACE_X + comp);
+    const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
+    hipLaunchKernelGGL(k_advection, cl.grid, cl.block, 0, c.stream, V, comp, g, scheme, outs[comp], cl.kz);
+  }
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_cfl(Context& c, const of_grid& g, const double* u, const double* v, const double* w, double dt,
+                double* out5) {
+  const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz);
+  hipLaunchKernelGGL(k_cfl, cl.grid, cl.block, 0, c.stream, u, v, w, make_lay(g, OF_FACE_X), make_lay(g, OF_FACE_Y),
+                     make_lay(g, OF_FACE_Z), g, dt, c.partials, c.counters + 0, out5, cl.kz);
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_norms(Context& c, const Lay& L, const double* x, int slot) {
+  const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
+  hipLaunchKernelGGL(k_norms, cl.grid, cl.block, 0, c.stream, x, L, c.partials, c.counters + 1, c.scal + slot, cl.kz);
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_scale_active(Context& c, const Lay& L, double scale, const double* in, double* out) {
+  const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
+  hipLaunchKernelGGL(k_scale_active, cl.grid, cl.block, 0, c.stream, in, out, L, scale, cl.kz);
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_factorized_correction(Context& c, const of_grid& g, int variant, double alpha, const double* u,
+                                  const double* v, const double* w, double* fu, double* fv, double* fw) {
+  const double* ins[3] = {u, v, w};
+  double* outs[3] = {fu, fv, fw};
+  const FcParams P = fc_params(g, variant, alpha);
+  for(int comp = 0; comp < 3; ++comp) {
+    const Lay L = make_lay(g, OF_FACE_X + comp);
+    OFH_CHECK(hipMemsetAsync(outs[comp], 0, L.size() * sizeof(double), c.stream));
+    const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
+    hipLaunchKernelGGL(k_fc, cl.grid, cl.block, 0, c.stream, ins[comp], L, P, outs[comp], cl.kz);
+  }
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_predictor_rhs(Context& c, const of_grid& g, int variant, int scheme, double nu, double dt,
+                          const double* u, const double* v, const double* w, const double* p, const double* pu,
+                          const double* pv, const double* pw, double* au, double* av, double* aw, double* ru,
+                          double* rv, double* rw) {
+  const VelPtrs V = vel_ptrs(g, u, v, w);
+  const double alpha = 0.5 * nu * dt;
+  RhsParams P;
+  P.g = g;
+  P.scheme = scheme;
+  P.has_prev = pu != nullptr;
+  P.ix = 1.0 / (g.dx * g.dx);
+  P.iy = 1.0 / (g.dy * g.dy);
+  P.iz = 1.0 / (g.dz * g.dz);
+  P.nu = nu;
+  P.s_grad = -dt;
+  P.s_dif = 0.5 * dt;
+  P.s_adv1 = -1.5 * dt;
+  P.s_prev = 0.5 * dt;
+  P.s_adv0 = -dt;
+  P.s_fc = alpha * alpha;
+  P.fc = fc_params(g, variant, alpha);
+  const double* prev[3] = {pu, pv, pw};
+  double* adv[3] = {au, av, aw};
+  double* rhs[3] = {ru, rv, rw};
+  for(int comp = 0; comp < 3; ++comp) {
+    const Lay L = make_lay(g, OF_FACE_X + comp);
+    const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
+    P.c = comp;
+    {
+      ProfScope prof_(c, "k_predictor_rhs", true);
+      hipLaunchKernelGGL(k_predictor_rhs, cl.grid, cl.block, 0, c.stream, V, p, make_lay(g, OF_CELL), prev[comp],
+                         adv[comp], rhs[comp], P, cl.kz);
+    }
+  }
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_correct(Context& c, const of_grid& g, const double* u, const double* v, const double* w,
+                    const double* phi, double scale, double* ou, double* ov, double* ow) {
+  const Lay Lp = make_lay(g, OF_CELL);
+  const double* ins[3] = {u, v, w};
+  double* outs[3] = {ou, ov, ow};
+  for(int a = 0; a < 3; ++a) {
+    const Lay L = make_lay(g, OF_FACE_X + a);
+    const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
+    {
+      ProfScope prof_(c, "k_correct", true);
+      hipLaunchKernelGGL(k_correct, cl.grid, cl.block, 0, c.stream, ins[a], L, phi, Lp, a, spacing(g, a), scale,
+                         outs[a], cl.kz);
+    }
+  }
+  OFH_CHECK(hipGetLastError());
+}
+
+}  // namespace ofh

@@ -1,0 +1,284 @@
+// Helmholtz predictor: (1 - a dxx)(1 - a dyy)(1 - a dzz) u* = rhs per velocity component,
+// solved exactly as three batched sweeps of tridiagonal line solves
+// (solver/projection.cpp:464-888,1028-1047).
+//
+// The line matrices have constant coefficients (-b, 1+2b, -b), b = a/h^2, with at most the
+// first and last diagonal entries modified by the boundary relation, so the Thomas pivots,
+// multipliers and the Sherman-Morrison vector of the periodic (cyclic) case depend only on
+// (component, axis). They are computed once per sweep on the host with the reference's own
+// recurrences (solve_tridiagonal :522-559, solve_cyclic_tridiagonal :561-617) and shared by
+// every line; each device thread then runs the forward/backward substitution of one line.
+
+#include "ofh.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace ofh {
+
+void launch_component_bc(Context& c, const of_grid& g, const of_bc_set& b, int comp, double* f);
+bool axis_periodic(const of_bc_set& b, int axis);
+
+namespace {
+
+constexpr double kSmallNumber = 1.0e-30;
+
+struct LineSys {
+  int b, e, n;           // unknowns are storage indices [b, e) along the axis
+  int cyclic;            // Sherman-Morrison correction
+  int copy_dup;          // copy index b to index e (periodic normal component)
+  double lo;             // sub-diagonal (-beta)
+  double add_lo, add_hi; // constants added to rhs[0], rhs[n-1]
+  double uc, gamma;      // cyclic: upper corner and gamma
+  double den;            // cyclic: Sherman-Morrison denominator
+  double single_pivot;   // cyclic n == 1
+};
+
+// Thomas factors of one line matrix (solver/projection.cpp:522-559).
+void thomas_factors(const std::vector<double>& lo, const std::vector<double>& di, const std::vector<double>& up,
+                    std::vector<double>& piv, std::vector<double>& mu) {
+  const size_t n = di.size();
+  piv.assign(n, 0.0);
+  mu.assign(n, 0.0);
+  double p = di[0];
+  if(std::abs(p) < kSmallNumber) throw RuntimeError("tridiagonal solve encountered a zero pivot");
+  piv[0] = p;
+  mu[0] = n > 1 ? up[0] / p : 0.0;
+  for(size_t i = 1; i < n; ++i) {
+    p = di[i] - lo[i] * mu[i - 1];
+    if(std::abs(p) < kSmallNumber) throw RuntimeError("tridiagonal solve encountered a zero pivot");
+    piv[i] = p;
+    mu[i] = i + 1 < n ? up[i] / p : 0.0;
+  }
+}
+
+std::vector<double> thomas_apply(const std::vector<double>& lo, const std::vector<double>& piv,
+                                 const std::vector<double>& mu, const std::vector<double>& rhs) {
+  const size_t n = piv.size();
+  std::vector<double> mr(n), x(n);
+  mr[0] = rhs[0] / piv[0];
+  for(size_t i = 1; i < n; ++i) mr[i] = (rhs[i] - lo[i] * mr[i - 1]) / piv[i];
+  x[n - 1] = mr[n - 1];
+  for(size_t i = n - 1; i-- > 0;) x[i] = mr[i] - mu[i] * x[i + 1];
+  return x;
+}
+
+// One thread per line. Forward elimination stores the modified right-hand side in place,
+// back substitution overwrites it with the solution; the cyclic case then applies the
+// rank-one correction sol = x - fac*z with fac from x[0], x[n-1].
+__global__ void k_line_solve(double* f, Lay L, int axis, LineSys S, const double* __restrict__ piv,
+                             const double* __restrict__ mu, const double* __restrict__ z) {
+  const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t1 = blockIdx.y * blockDim.y + threadIdx.y;
+  if(t0 >= L.a(o0) || t1 >= L.a(o1)) return;
+  int c[3];
+  c[axis] = 0;
+  c[o0] = 1 + t0;
+  c[o1] = 1 + t1;
+  double* base = f + L.at(c[0], c[1], c[2]);
+  const ptrdiff_t st = axis == 0 ? 1 : axis == 1 ? (ptrdiff_t)L.sx : (ptrdiff_t)L.sx * L.sy;
+  const int n = S.n;
+  if(S.cyclic && n == 1) {
+    base[S.b * st] = base[S.b * st] / S.single_pivot;
+  } else {
+    // forward elimination
+    double r0 = base[S.b * st];
+    if(n == 1) {
+      r0 = r0 + S.add_lo;
+      r0 = r0 + S.add_hi;
+    } else {
+      r0 = r0 + S.add_lo;
+    }
+    double mr = r0 / piv[0];
+    base[S.b * st] = mr;
+    for(int t = 1; t < n; ++t) {
+      double r = base[(S.b + t) * st];
+      if(t == n - 1) r = r + S.add_hi;
+      mr = (r - S.lo * mr) / piv[t];
+      base[(S.b + t) * st] = mr;
+    }
+    // back substitution
+    double x = mr;
+    const double x_last = x;
+    for(int t = n - 2; t >= 0; --t) {
+      x = base[(S.b + t) * st] - mu[t] * x;
+      base[(S.b + t) * st] = x;
+    }
+    if(S.cyclic) {
+      const double fac = (x + S.uc * x_last / S.gamma) / S.den;
+      for(int t = 0; t < n; ++t) base[(S.b + t) * st] = base[(S.b + t) * st] - fac * z[t];
+    }
+  }
+  if(S.copy_dup) base[S.e * st] = base[S.b * st];
+}
+
+inline bool dirichlet_velocity(const of_bc& c) {  // solver/projection.cpp:45-49
+  return c.type == OF_NO_SLIP_WALL || c.type == OF_PRESCRIBED_VELOCITY || c.type == OF_SYMMETRY;
+}
+
+// solver/projection.cpp:60-80
+void tangential_relation(const of_bc& c, int comp, double& coef, double& cst) {
+  switch(c.type) {
+    case OF_NO_SLIP_WALL:
+    case OF_PRESCRIBED_VELOCITY:
+      coef = -1.0;
+      cst = 2.0 * c.velocity[comp];
+      return;
+    case OF_SYMMETRY:
+    case OF_FIXED_PRESSURE:
+      coef = 1.0;
+      cst = 0.0;
+      return;
+    default:
+      coef = 0.0;
+      cst = 0.0;
+  }
+}
+
+}  // namespace
+
+// Returns the reference's line_solves count.
+int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* ru, const double* rv,
+                  const double* rw, double* ou, double* ov, double* ow) {
+  if(alpha < 0.0) throw InvalidArgument("solve_predictor_adi expects a non-negative alpha");
+  const double* rhs[3] = {ru, rv, rw};
+  double* out[3] = {ou, ov, ow};
+  int line_solves = 0;
+  const int nmax = std::max(g.nx, std::max(g.ny, g.nz)) + 2;
+  double* dev = static_cast<double*>(c.scratch_get(7, sizeof(double) * 3 * nmax * 9));
+  std::vector<double> host(3 * (size_t)nmax * 9, 0.0);
+  struct Pending {
+    int comp, axis;
+    LineSys S;
+    size_t off;
+  };
+  std::vector<Pending> sweeps;
+  for(int comp = 0; comp < 3; ++comp) {
+    const Lay L = make_lay(g, OF_FACE_X + comp);
+    OFH_CHECK(hipMemcpyAsync(out[comp], rhs[comp], L.size() * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  }
+  for(int comp = 0; comp < 3; ++comp) {
+    const Lay L = make_lay(g, OF_FACE_X + comp);
+    for(int axis = 0; axis < 3; ++axis) {
+      const double h = spacing(g, axis);
+      const double beta = alpha / (h * h);
+      const int A = L.a(axis);
+      LineSys S{};
+      S.lo = -beta;
+      std::vector<double> lo, di, up, piv, mu, zv;
+      if(axis_periodic(b, axis)) {  // solve_component_sweep_periodic :619-720
+        S.b = 1;
+        S.e = comp == axis ? A : 1 + A;
+        S.n = S.e - S.b;
+        S.copy_dup = comp == axis;
+        S.cyclic = 1;
+        if(S.n <= 0) continue;
+        const int n = S.n;
+        lo.assign(n, -beta);
+        di.assign(n, 1.0 + 2.0 * beta);
+        up.assign(n, -beta);
+        lo[0] = 0.0;
+        up[n - 1] = 0.0;
+        const double lc = -beta, uc = -beta;
+        if(n == 1) {
+          S.single_pivot = di[0] + lc + uc;
+          if(std::abs(S.single_pivot) < kSmallNumber)
+            throw RuntimeError("cyclic tridiagonal solve encountered a zero pivot");
+          piv.assign(1, 1.0);
+          mu.assign(1, 0.0);
+          zv.assign(1, 0.0);
+        } else {
+          std::vector<double> dm = di;
+          const double gamma = -di[0];
+          if(std::abs(gamma) < kSmallNumber) throw RuntimeError("cyclic tridiagonal solve encountered a zero gamma");
+          dm[0] = di[0] - gamma;
+          dm[n - 1] = di[n - 1] - lc * uc / gamma;
+          std::vector<double> lm = lo, um = up;
+          lm[0] = 0.0;
+          um[n - 1] = 0.0;
+          thomas_factors(lm, dm, um, piv, mu);
+          std::vector<double> uvec(n, 0.0);
+          uvec[0] = gamma;
+          uvec[n - 1] = lc;
+          zv = thomas_apply(lm, piv, mu, uvec);
+          S.den = 1.0 + zv[0] + uc * zv[n - 1] / gamma;
+          if(std::abs(S.den) < kSmallNumber)
+            throw RuntimeError("cyclic tridiagonal solve encountered a singular Sherman-Morrison update");
+          S.uc = uc;
+          S.gamma = gamma;
+        }
+      } else {  // solve_component_sweep :722-866
+        int bb = 1, ee = 1 + A;
+        if(comp == axis) {
+          if(dirichlet_velocity(b.faces[2 * axis])) bb += 1;
+          if(dirichlet_velocity(b.faces[2 * axis + 1])) ee -= 1;
+        }
+        S.b = bb;
+        S.e = ee;
+        S.n = ee - bb;
+        S.cyclic = 0;
+        S.copy_dup = 0;
+        if(S.n <= 0) continue;
+        const int n = S.n;
+        lo.assign(n, -beta);
+        di.assign(n, 1.0 + 2.0 * beta);
+        up.assign(n, -beta);
+        lo[0] = 0.0;
+        up[n - 1] = 0.0;
+        const of_bc& cl = b.faces[2 * axis];
+        const of_bc& cu = b.faces[2 * axis + 1];
+        if(comp == axis && dirichlet_velocity(cl)) {
+          S.add_lo = beta * cl.velocity[axis];
+        } else {
+          double coef, cst;
+          tangential_relation(cl, comp, coef, cst);
+          di[0] -= beta * coef;
+          S.add_lo = beta * cst;
+        }
+        if(comp == axis && dirichlet_velocity(cu)) {
+          S.add_hi = beta * cu.velocity[axis];
+        } else {
+          double coef, cst;
+          tangential_relation(cu, comp, coef, cst);
+          di[n - 1] -= beta * coef;
+          S.add_hi = beta * cst;
+        }
+        thomas_factors(lo, di, up, piv, mu);
+        zv.assign(n, 0.0);
+      }
+      const size_t off = (size_t)(comp * 3 + axis) * 3 * nmax;
+      std::copy(piv.begin(), piv.end(), host.begin() + off);
+      std::copy(mu.begin(), mu.end(), host.begin() + off + nmax);
+      std::copy(zv.begin(), zv.end(), host.begin() + off + 2 * nmax);
+      const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
+      line_solves += L.a(o0) * L.a(o1);
+      sweeps.push_back(Pending{comp, axis, S, off});
+    }
+  }
+  OFH_CHECK(hipMemcpyAsync(dev, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  // hipMemcpyAsync from pageable memory returns after the copy has been staged, so `host`
+  // may be released when this function returns.
+  for(int comp = 0; comp < 3; ++comp) {
+    launch_component_bc(c, g, b, comp, out[comp]);
+    if(alpha == 0.0) continue;
+    const Lay L = make_lay(g, OF_FACE_X + comp);
+    for(const Pending& p : sweeps) {
+      if(p.comp != comp) continue;
+      const int o0 = p.axis == 0 ? 1 : 0, o1 = p.axis == 2 ? 1 : 2;
+      const dim3 block(64, 4, 1);
+      const dim3 grid((unsigned)((L.a(o0) + 63) / 64), (unsigned)((L.a(o1) + 3) / 4), 1);
+      {
+        ProfScope prof_(c, "k_line_solve", true);
+        hipLaunchKernelGGL(k_line_solve, grid, block, 0, c.stream, out[comp], L, p.axis, p.S, dev + p.off,
+                           dev + p.off + nmax, dev + p.off + 2 * nmax);
+      }
+      launch_component_bc(c, g, b, comp, out[comp]);
+    }
+  }
+  OFH_CHECK(hipGetLastError());
+  OFH_CHECK(hipStreamSynchronize(c.stream));  // `host` staging buffer lifetime
+  return alpha == 0.0 ? 0 : line_solves;
+}
+
+}  // namespace ofh

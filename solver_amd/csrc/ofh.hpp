@@ -1,0 +1,114 @@
+// Host-side declarations shared by the translation units of the gfx950 library.
+#pragma once
+
+#include "common.hpp"
+
+#include <string>
+#include <vector>
+
+namespace ofh {
+
+// ------------------------------------------------------------------ device context
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  double* partials = nullptr;   // kMaxRed x kMaxPartials reduction partials
+  unsigned* counters = nullptr; // reduction completion counters
+  double* scal = nullptr;       // device scalars
+  double* host_scal = nullptr;  // pinned mirror for read-back
+  // grow-only scratch pool for the per-operator entry points
+  std::vector<std::pair<void*, size_t>> scratch;
+  void* scratch_get(int slot, size_t bytes);
+  // live timing of one kernel: launches of the kernel named `prof_name` are bracketed with
+  // events on the context stream (finest-level launches only for the multigrid kernels)
+  std::string prof_name;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
+  size_t prof_used = 0;
+  bool prof_on(const char* name) const { return !prof_name.empty() && prof_name == name; }
+  void prof_begin(const char* name, hipEvent_t* end);
+  ~Context();
+};
+
+// Brackets one launch with timing events when `name` is the profiled kernel.
+struct ProfScope {
+  Context& c;
+  hipEvent_t end = nullptr;
+  ProfScope(Context& ctx, const char* name, bool eligible = true) : c(ctx) {
+    if(eligible && c.prof_on(name)) c.prof_begin(name, &end);
+  }
+  ~ProfScope() {
+    if(end) (void)hipEventRecord(end, c.stream);
+  }
+};
+
+// Fixed slots in Context::scal
+enum ScalarSlot : int {
+  kScalRed0 = 0,      // 8 values: generic reduction results
+  kScalCfl = 8,       // 5 values: cfl dt, max cfl, max u, max v, max w
+  kScalStep = 16,     // step diagnostics (32 values)
+  kScalPoisson = 64,  // MGPCG scalars (see poisson.hip)
+  kScalCount = 256
+};
+enum CounterSlot : int { kCounterCount = 64 };
+
+// Launch shape for a "k-column" kernel over an active box ex*ey*ez: blocks of 64x4 threads,
+// each thread walking up to kz consecutive k so that the grid stays within kMaxPartials.
+struct ColumnLaunch {
+  dim3 grid, block;
+  int kz;
+};
+ColumnLaunch column_launch(int ex, int ey, int ez, int min_kz = 1);
+
+// ------------------------------------------------------------------ fields.hip
+void launch_velocity_bc(Context& c, const of_grid& g, const of_bc_set& b, double* u, double* v, double* w);
+void launch_component_bc(Context& c, const of_grid& g, const of_bc_set& b, int comp, double* f);
+void launch_pressure_bc(Context& c, const of_grid& g, const of_pressure_bc_set& b, double* p);
+void launch_total_pressure_bc(Context& c, const of_grid& g, const of_bc_set& b, const double* su, const double* sv,
+                              const double* sw, double* p);
+void launch_gradient(Context& c, const of_grid& g, const double* p, double* gx, double* gy, double* gz);
+void launch_divergence(Context& c, const of_grid& g, const double* u, const double* v, const double* w, double* d);
+void launch_laplacian(Context& c, const of_grid& g, int loc, double scale, const double* in, double* out);
+void launch_advection(Context& c, const of_grid& g, int scheme, const double* u, const double* v, const double* w,
+                      double* au, double* av, double* aw);
+void launch_cfl(Context& c, const of_grid& g, const double* u, const double* v, const double* w, double dt,
+                double* out5_device);
+void launch_factorized_correction(Context& c, const of_grid& g, int variant, double alpha, const double* u,
+                                  const double* v, const double* w, double* fu, double* fv, double* fw);
+void launch_predictor_rhs(Context& c, const of_grid& g, int variant, int scheme, double nu, double dt,
+                          const double* u, const double* v, const double* w, const double* p, const double* pu,
+                          const double* pv, const double* pw, double* au, double* av, double* aw, double* ru,
+                          double* rv, double* rw);
+// sum(x^2) and max|x| over an active box, into scal[slot], scal[slot+1]
+void launch_norms(Context& c, const Lay& L, const double* x, int slot);
+// scaled copy of the active range: out = scale * in
+void launch_scale_active(Context& c, const Lay& L, double scale, const double* in, double* out);
+
+// ------------------------------------------------------------------ helmholtz.hip
+int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* ru,
+                  const double* rv, const double* rw, double* ou, double* ov, double* ow);
+
+// ------------------------------------------------------------------ poisson.hip
+struct PoissonResult {
+  of_poisson_diagnostics diag;
+  std::vector<double> history;
+};
+class PoissonSolver;  // MGPCG on the device, reused across solves of one grid/BC set
+PoissonSolver* poisson_create(Context& c, const of_grid& g, const of_pressure_bc_set& b, int precision);
+void poisson_destroy(PoissonSolver* s);
+// Solves -lap(p) = rhs (reference sign conventions); rhs and p in ghosted cell storage.
+// p holds the warm start on entry. Only the active range of p is written.
+PoissonResult poisson_solve(PoissonSolver* s, const double* rhs, double* p, int max_iterations, double tol,
+                            bool want_history);
+void launch_poisson_apply(Context& c, const of_grid& g, const of_pressure_bc_set& b, bool homogeneous,
+                          const double* in, double* out);
+
+// ------------------------------------------------------------------ projection (fields.hip)
+// div(u) * scale into a ghosted cell field (active range)
+void launch_divergence_scaled(Context& c, const of_grid& g, const double* u, const double* v, const double* w,
+                              double scale, double* out);
+// out = pred - scale * grad(phi_with_ghosts) on every active face
+void launch_correct(Context& c, const of_grid& g, const double* u, const double* v, const double* w,
+                    const double* phi, double scale, double* ou, double* ov, double* ow);
+
+}  // namespace ofh

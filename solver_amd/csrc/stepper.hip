@@ -1,0 +1,322 @@
+// Device-resident fractional-step driver: the step loop of run_taylor_green_steps
+// (solver/taylor_green.cpp:540-634) and advance_lid_driven_cavity_impl
+// (solver/lid_driven_cavity.cpp:524-624) with every field kept in HBM.
+
+#include "stepper.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace ofh {
+
+bool axis_periodic(const of_bc_set& b, int axis);
+
+namespace {
+
+// Step diagnostics over the new velocity (solver/taylor_green.cpp:607-616): divergence RMS and
+// max (metric), CFL (solver/momentum_terms.cpp:373-401), max |u_new - u_old| over every active
+// face (solver/taylor_green.cpp:100-118) and a non-finite count. Cell (i,j,k) owns the lower
+// face of each component, and the last cell along an axis also owns the upper face.
+__global__ void k_step_diag(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
+                            const double* __restrict__ uo, const double* __restrict__ vo,
+                            const double* __restrict__ wo, Lay Lu, Lay Lv, Lay Lw, of_grid g, double dt,
+                            double* partials, unsigned* counter, double* out, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  // 0 sum div^2, 1 max|div|, 2 max cfl, 3 max|uc|, 4 max|vc|, 5 max|wc|, 6 max|du|, 7 non-finite
+  double vals[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if(i <= g.nx && j <= g.ny) {
+    const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + g.nz);
+    for(int k = kb; k < ke; ++k) {
+      const double uw = u[Lu.at(i, j, k)], ue = u[Lu.at(i + 1, j, k)];
+      const double vs = v[Lv.at(i, j, k)], vn = v[Lv.at(i, j + 1, k)];
+      const double wb = w[Lw.at(i, j, k)], wf = w[Lw.at(i, j, k + 1)];
+      const double a = (ue - uw) / g.dx;
+      const double b = (vn - vs) / g.dy;
+      const double c = (wf - wb) / g.dz;
+      const double dv = a + b + c;
+      vals[0] += dv * dv;
+      vals[1] = fmax(vals[1], fabs(dv));
+      const double uc = 0.5 * (uw + ue), vc = 0.5 * (vs + vn), wc = 0.5 * (wb + wf);
+      vals[3] = fmax(vals[3], fabs(uc));
+      vals[4] = fmax(vals[4], fabs(vc));
+      vals[5] = fmax(vals[5], fabs(wc));
+      vals[2] = fmax(vals[2], dt * (fabs(uc) / g.dx + fabs(vc) / g.dy + fabs(wc) / g.dz));
+      double ch = fabs(uw - uo[Lu.at(i, j, k)]);
+      ch = fmax(ch, fabs(vs - vo[Lv.at(i, j, k)]));
+      ch = fmax(ch, fabs(wb - wo[Lw.at(i, j, k)]));
+      if(i == g.nx) ch = fmax(ch, fabs(ue - uo[Lu.at(i + 1, j, k)]));
+      if(j == g.ny) ch = fmax(ch, fabs(vn - vo[Lv.at(i, j + 1, k)]));
+      if(k == g.nz) ch = fmax(ch, fabs(wf - wo[Lw.at(i, j, k + 1)]));
+      vals[6] = fmax(vals[6], ch);
+      if(!isfinite(dv) || !isfinite(ch) || !isfinite(uc) || !isfinite(vc) || !isfinite(wc)) vals[7] += 1.0;
+    }
+  }
+  const int ops[8] = {kSum, kMax, kMax, kMax, kMax, kMax, kMax, kSum};
+  double res[8];
+  if(reduce_partials<8>(vals, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
+    for(int q = 0; q < 8; ++q) out[q] = res[q];
+  }
+}
+
+// p_total += phi; p_total += (-nu dt / 2) rhs   (solver/taylor_green.cpp:607-608)
+__global__ void k_pressure_update(double* __restrict__ pt, const double* __restrict__ phi,
+                                  const double* __restrict__ rhs, Lay L, double s, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > L.ax || j > L.ay) return;
+  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + L.az);
+  for(int k = kb; k < ke; ++k) {
+    const size_t at = L.at(i, j, k);
+    double p = pt[at] + 1.0 * phi[at];
+    p = p + s * rhs[at];
+    pt[at] = p;
+  }
+}
+
+// Taylor-Green initial field (solver/taylor_green.cpp:335-389) on the device.
+__global__ void k_tg_init(double* f, Lay L, int loc, of_grid g, int three_d, int convention) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > L.ax || j > L.ay) return;
+  for(int k = 1; k <= L.az; ++k) {
+    // core/field_layout.hpp:366-377
+    const double x = loc == OF_FACE_X ? (double)(i - 1) * g.dx : ((double)(i - 1) + 0.5) * g.dx;
+    const double y = loc == OF_FACE_Y ? (double)(j - 1) * g.dy : ((double)(j - 1) + 0.5) * g.dy;
+    const double z = loc == OF_FACE_Z ? (double)(k - 1) * g.dz : ((double)(k - 1) + 0.5) * g.dz;
+    double val;
+    if(loc == OF_FACE_X) {
+      const double zf = three_d ? cos(z) : 1.0;
+      val = convention == 0 ? -cos(x) * sin(y) * zf * 1.0 : sin(x) * cos(y) * zf * 1.0;
+    } else if(loc == OF_FACE_Y) {
+      const double zf = three_d ? cos(z) : 1.0;
+      val = convention == 0 ? sin(x) * cos(y) * zf * 1.0 : -cos(x) * sin(y) * zf * 1.0;
+    } else if(loc == OF_FACE_Z) {
+      val = 0.0;
+    } else {
+      const double sg = convention == 0 ? 1.0 : -1.0;
+      if(three_d)
+        val = -sg * 0.0625 * (cos(2.0 * x) + cos(2.0 * y)) * (cos(2.0 * z) + 2.0) * 1.0;
+      else
+        val = -sg * 0.25 * (cos(2.0 * x) + cos(2.0 * y)) * 1.0;
+    }
+    f[L.at(i, j, k)] = val;
+  }
+}
+
+// kinetic energy sum (solver/taylor_green.cpp:269-288)
+__global__ void k_kinetic(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
+                          Lay Lu, Lay Lv, Lay Lw, of_grid g, double* partials, unsigned* counter, double* out, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  double vals[1] = {0.0};
+  if(i <= g.nx && j <= g.ny) {
+    const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + g.nz);
+    for(int k = kb; k < ke; ++k) {
+      const double uc = 0.5 * (u[Lu.at(i, j, k)] + u[Lu.at(i + 1, j, k)]);
+      const double vc = 0.5 * (v[Lv.at(i, j, k)] + v[Lv.at(i, j + 1, k)]);
+      const double wc = 0.5 * (w[Lw.at(i, j, k)] + w[Lw.at(i, j, k + 1)]);
+      vals[0] += 0.5 * (uc * uc + vc * vc + wc * wc);
+    }
+  }
+  const int ops[1] = {kSum};
+  double res[1];
+  if(reduce_partials<1>(vals, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) out[0] = res[0];
+}
+
+}  // namespace
+
+of_bc_set stepper_bcs(const of_stepper_config& c) {
+  of_bc_set b{};
+  if(c.flow_case == OF_CASE_TAYLOR_GREEN) {  // solver/taylor_green.cpp:510-517
+    for(int f = 0; f < 6; ++f) b.faces[f].type = OF_PERIODIC;
+    if(c.nz <= 1) b.faces[4].type = b.faces[5].type = OF_SYMMETRY;
+  } else {  // solver/lid_driven_cavity.cpp:362-370
+    for(int f = 0; f < 6; ++f) b.faces[f].type = OF_NO_SLIP_WALL;
+    b.faces[3].type = OF_PRESCRIBED_VELOCITY;
+    b.faces[3].velocity[0] = c.lid_velocity;
+    b.faces[4].type = b.faces[5].type = OF_SYMMETRY;
+  }
+  return b;
+}
+
+of_pressure_bc_set derive_pressure_bcs(const of_bc_set& b) {  // solver/projection.cpp:934-964
+  of_pressure_bc_set m{};
+  for(int f = 0; f < 6; ++f) {
+    switch(b.faces[f].type) {
+      case OF_FIXED_PRESSURE: m.faces[f].type = OF_PRESSURE_DIRICHLET; break;
+      case OF_PERIODIC: m.faces[f].type = OF_PRESSURE_PERIODIC; break;
+      default: m.faces[f].type = OF_PRESSURE_NEUMANN; break;
+    }
+    m.faces[f].value = 0.0;
+    m.faces[f].gradient = 0.0;
+  }
+  for(int a = 0; a < 3; ++a) {
+    const bool lo = m.faces[2 * a].type == OF_PRESSURE_PERIODIC, hi = m.faces[2 * a + 1].type == OF_PRESSURE_PERIODIC;
+    if(lo != hi) throw InvalidArgument("periodic pressure boundaries must be paired on both sides");
+  }
+  return m;
+}
+
+Stepper::Stepper(Context& ctx, const of_stepper_config& cfg) : c(ctx), cfg(cfg) {
+  if(cfg.nx <= 0 || cfg.ny <= 0 || cfg.nz <= 0) throw InvalidArgument("stepper expects positive nx, ny, nz");
+  if(!(cfg.viscosity > 0.0)) throw InvalidArgument("stepper expects viscosity > 0");
+  if(!(cfg.dt > 0.0)) throw InvalidArgument("stepper expects dt > 0");
+  if(cfg.poisson_max_iterations <= 0 || !(cfg.poisson_tolerance > 0.0))
+    throw InvalidArgument("stepper expects positive Poisson controls");
+  const double L = 2.0 * std::acos(-1.0);
+  if(cfg.flow_case == OF_CASE_TAYLOR_GREEN) {  // solver/taylor_green.cpp:497-508
+    g = of_grid{cfg.nx, cfg.ny, cfg.nz, L / (double)cfg.nx, L / (double)cfg.ny, cfg.nz > 1 ? L / (double)cfg.nz : 1.0};
+  } else if(cfg.flow_case == OF_CASE_LID_DRIVEN_CAVITY) {  // solver/lid_driven_cavity.cpp:372-380
+    if(cfg.nz != 1) throw InvalidArgument("the lid-driven cavity is two-dimensional (nz = 1)");
+    g = of_grid{cfg.nx, cfg.ny, 1, 1.0 / (double)cfg.nx, 1.0 / (double)cfg.ny, 1.0};
+  } else {
+    throw InvalidArgument("unknown flow case");
+  }
+  bcs = stepper_bcs(cfg);
+  pbcs = derive_pressure_bcs(bcs);
+  all_periodic = axis_periodic(bcs, 0) && axis_periodic(bcs, 1) && axis_periodic(bcs, 2);
+  const size_t fu = make_lay(g, OF_FACE_X).size(), fv = make_lay(g, OF_FACE_Y).size(),
+               fw = make_lay(g, OF_FACE_Z).size(), fc = make_lay(g, OF_CELL).size();
+  const size_t sizes[3] = {fu, fv, fw};
+  auto alloc = [&](double** p, size_t n) {
+    OFH_CHECK(hipMalloc(p, n * sizeof(double)));
+    OFH_CHECK(hipMemsetAsync(*p, 0, n * sizeof(double), c.stream));
+    owned.push_back(*p);
+  };
+  for(int a = 0; a < 3; ++a) {
+    alloc(&vel[a], sizes[a]);
+    alloc(&adv[a], sizes[a]);
+    alloc(&rhs[a], sizes[a]);
+    alloc(&pred[a], sizes[a]);
+    alloc(&corr[a], sizes[a]);
+    if(!all_periodic) alloc(&dif[a], sizes[a]);
+  }
+  alloc(&p_total, fc);
+  alloc(&phi, fc);
+  alloc(&prhs, fc);
+  OFH_CHECK(hipMalloc(&diag, 32 * sizeof(double)));
+  owned.push_back(diag);
+  poisson = poisson_create(c, g, pbcs, cfg.precision);
+  std::memset(&metrics, 0, sizeof metrics);
+  metrics.dt = cfg.dt;
+  OFH_CHECK(hipStreamSynchronize(c.stream));
+}
+
+Stepper::~Stepper() {
+  if(poisson) poisson_destroy(poisson);
+  for(double* p : owned) (void)hipFree(p);
+}
+
+void Stepper::total_pressure_bc(const double* const* v) {
+  if(all_periodic) {
+    launch_total_pressure_bc(c, g, bcs, v[0], v[1], v[2], p_total);  // periodic copies only
+    return;
+  }
+  for(int a = 0; a < 3; ++a) launch_laplacian(c, g, OF_FACE_X + a, cfg.viscosity, v[a], dif[a]);
+  launch_total_pressure_bc(c, g, bcs, dif[0], dif[1], dif[2], p_total);
+}
+
+void Stepper::initialize(int convention) {
+  const int three_d = g.nz > 1;
+  for(int a = 0; a < 3; ++a) OFH_CHECK(hipMemsetAsync(adv[a], 0, make_lay(g, OF_FACE_X + a).size() * 8, c.stream));
+  for(int a = 0; a < 3; ++a) OFH_CHECK(hipMemsetAsync(vel[a], 0, make_lay(g, OF_FACE_X + a).size() * 8, c.stream));
+  OFH_CHECK(hipMemsetAsync(p_total, 0, make_lay(g, OF_CELL).size() * 8, c.stream));
+  if(cfg.flow_case == OF_CASE_TAYLOR_GREEN) {
+    for(int loc = 0; loc < 4; ++loc) {
+      const Lay L = make_lay(g, loc);
+      double* f = loc == OF_CELL ? p_total : vel[loc - 1];
+      const dim3 block(64, 4, 1);
+      const dim3 grid((unsigned)((L.ax + 63) / 64), (unsigned)((L.ay + 3) / 4), 1);
+      hipLaunchKernelGGL(k_tg_init, grid, block, 0, c.stream, f, L, loc, g, three_d, convention);
+    }
+  }
+  launch_velocity_bc(c, g, bcs, vel[0], vel[1], vel[2]);
+  total_pressure_bc(vel);
+  OFH_CHECK(hipGetLastError());
+  has_prev = 0;
+  std::memset(&metrics, 0, sizeof metrics);
+  metrics.dt = cfg.dt;
+  OFH_CHECK(hipStreamSynchronize(c.stream));
+}
+
+void Stepper::step() {
+  const double dt = cfg.dt, nu = cfg.viscosity;
+  const char* name = cfg.flow_case == OF_CASE_TAYLOR_GREEN ? "Taylor-Green" : "lid-driven cavity";
+  // current velocity with BC ghosts (idempotent on a state that already carries them)
+  launch_velocity_bc(c, g, bcs, vel[0], vel[1], vel[2]);
+  // predictor RHS; A overwrites A_prev in place after it has been read
+  launch_predictor_rhs(c, g, cfg.flow_case == OF_CASE_TAYLOR_GREEN ? 0 : 1, cfg.scheme, nu, dt, vel[0], vel[1], vel[2],
+                       p_total, has_prev ? adv[0] : nullptr, has_prev ? adv[1] : nullptr,
+                       has_prev ? adv[2] : nullptr, adv[0], adv[1], adv[2], rhs[0], rhs[1], rhs[2]);
+  // Helmholtz predictor (exact line solves); output carries BC ghosts
+  predictor_adi(c, g, bcs, 0.5 * nu * dt, rhs[0], rhs[1], rhs[2], pred[0], pred[1], pred[2]);
+  // projection (solver/projection.cpp:1097-1131); pred already carries its BC ghosts
+  launch_divergence_scaled(c, g, pred[0], pred[1], pred[2], 1.0 / dt, prhs);
+  OFH_CHECK(hipMemsetAsync(phi, 0, make_lay(g, OF_CELL).size() * sizeof(double), c.stream));
+  const PoissonResult pr = poisson_solve(poisson, prhs, phi, cfg.poisson_max_iterations, cfg.poisson_tolerance, false);
+  if(!pr.diag.converged) {  // solver/projection.cpp:1133-1147
+    char buf[512];
+    std::snprintf(buf, sizeof buf,
+                  "%s pressure solve did not converge at step %d (iterations=%d, relative_residual=%g, tolerance=%g)",
+                  name, metrics.step + 1, pr.diag.iterations, pr.diag.relative_residual, cfg.poisson_tolerance);
+    throw RuntimeError(buf);
+  }
+  launch_pressure_bc(c, g, pbcs, phi);
+  launch_correct(c, g, pred[0], pred[1], pred[2], phi, -dt / 1.0, corr[0], corr[1], corr[2]);
+  launch_velocity_bc(c, g, bcs, corr[0], corr[1], corr[2]);
+  {
+    const Lay L = make_lay(g, OF_CELL);
+    const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
+    {
+      ProfScope prof_(c, "k_pressure_update", true);
+      hipLaunchKernelGGL(k_pressure_update, cl.grid, cl.block, 0, c.stream, p_total, phi, prhs, L, -0.5 * nu * dt, cl.kz);
+    }
+  }
+  total_pressure_bc(corr);
+  {
+    const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz);
+    {
+      ProfScope prof_(c, "k_step_diag", true);
+      hipLaunchKernelGGL(k_step_diag, cl.grid, cl.block, 0, c.stream, corr[0], corr[1], corr[2], vel[0], vel[1], vel[2],
+                         make_lay(g, OF_FACE_X), make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), g, dt, c.partials,
+                         c.counters + 2, diag, cl.kz);
+    }
+  }
+  OFH_CHECK(hipGetLastError());
+  double h[8];
+  OFH_CHECK(hipMemcpyAsync(h, diag, 8 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  OFH_CHECK(hipStreamSynchronize(c.stream));
+  const double ncell = (double)g.nx * (double)g.ny * (double)g.nz;
+  const double div_l2 = std::sqrt(h[0] / ncell);
+  if(h[7] != 0.0 || !std::isfinite(h[6]) || !std::isfinite(div_l2))
+    throw RuntimeError(std::string(name) + " solve produced a non-finite state");
+  for(int a = 0; a < 3; ++a) std::swap(vel[a], corr[a]);
+  has_prev = 1;
+  of_step_metrics n{};
+  n.step = metrics.step + 1;
+  n.time = metrics.time + dt;
+  n.dt = dt;
+  n.max_cfl = h[2];
+  n.max_velocity_change = h[6];
+  n.divergence_l2 = div_l2;
+  n.max_divergence_l2 = std::max(metrics.max_divergence_l2, div_l2);
+  n.pressure_iterations = pr.diag.iterations;
+  n.pressure_relative_residual = pr.diag.relative_residual;
+  n.divergence_linf = h[1];
+  n.max_divergence_linf = std::max(metrics.max_divergence_linf, h[1]);
+  metrics = n;
+}
+
+double Stepper::kinetic_energy() {
+  const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz);
+  hipLaunchKernelGGL(k_kinetic, cl.grid, cl.block, 0, c.stream, vel[0], vel[1], vel[2], make_lay(g, OF_FACE_X),
+                     make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), g, c.partials, c.counters + 3, diag + 16, cl.kz);
+  double e = 0.0;
+  OFH_CHECK(hipMemcpyAsync(&e, diag + 16, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  OFH_CHECK(hipStreamSynchronize(c.stream));
+  return e / ((double)g.nx * (double)g.ny * (double)g.nz);
+}
+
+}  // namespace ofh

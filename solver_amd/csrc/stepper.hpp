@@ -1,0 +1,41 @@
+// Device-resident stepper state (see stepper.hip).
+#pragma once
+
+#include "ofh.hpp"
+
+namespace ofh {
+
+of_bc_set stepper_bcs(const of_stepper_config& c);
+of_pressure_bc_set derive_pressure_bcs(const of_bc_set& b);
+
+struct Stepper {
+  Context& c;
+  of_stepper_config cfg;
+  of_grid g{};
+  of_bc_set bcs{};
+  of_pressure_bc_set pbcs{};
+  bool all_periodic = false;
+  double* vel[3]{};
+  double* adv[3]{};
+  double* rhs[3]{};
+  double* pred[3]{};
+  double* corr[3]{};
+  double* dif[3]{};
+  double* p_total = nullptr;
+  double* phi = nullptr;
+  double* prhs = nullptr;
+  double* diag = nullptr;
+  PoissonSolver* poisson = nullptr;
+  int has_prev = 0;
+  of_step_metrics metrics{};
+  std::vector<double*> owned;
+
+  Stepper(Context& ctx, const of_stepper_config& cfg);
+  ~Stepper();
+  void initialize(int convention);
+  void step();
+  double kinetic_energy();
+  void total_pressure_bc(const double* const* v);
+};
+
+}  // namespace ofh
