@@ -42,7 +42,6 @@ Context::~Context() {
   for(auto& s : scratch)
     if(s.first) (void)hipFree(s.first);
   if(partials) (void)hipFree(partials);
-  if(counters) (void)hipFree(counters);
   if(scal) (void)hipFree(scal);
   if(host_scal) (void)hipHostFree(host_scal);
   if(own_stream && stream) (void)hipStreamDestroy(stream);
@@ -138,8 +137,6 @@ int of_context_create(int device, void* stream, of_context** out) {
         x.own_stream = true;
       }
       OFH_CHECK(hipMalloc(&x.partials, sizeof(double) * kMaxRed * kMaxPartials));
-      OFH_CHECK(hipMalloc(&x.counters, sizeof(unsigned) * kCounterCount));
-      OFH_CHECK(hipMemset(x.counters, 0, sizeof(unsigned) * kCounterCount));
       OFH_CHECK(hipMalloc(&x.scal, sizeof(double) * kScalCount));
       OFH_CHECK(hipMemset(x.scal, 0, sizeof(double) * kScalCount));
       OFH_CHECK(hipHostMalloc(&x.host_scal, sizeof(double) * kScalCount, hipHostMallocDefault));

@@ -80,12 +80,15 @@ inline dim3 grid_for(int ex, int ey, int ez) {
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 // ------------------------------------------------------------------ deterministic reductions
-// A reduction kernel writes one partial per block (fixed order in memory); the last block to
-// finish (detected with a device-scope counter) combines the partials in a fixed order, so
-// the result is bitwise reproducible for a given launch shape. The counter is reset by the
-// last block, so it needs no clearing between launches.
+// Each block of a reducing kernel stores one partial per value (store_partials); a one-block
+// kernel launched next on the same stream combines them in a fixed order (k_finalize) and runs
+// the reduction's epilogue. The kernel boundary orders the partial stores before the combine,
+// so no device-scope fence or completion counter is needed: a device-scope release in every
+// block writes back the XCD-private L2 and made streaming kernels 4x slower on this part.
+// Results are bitwise reproducible for a given launch shape.
 constexpr int kMaxRed = 8;           // values per reduction
-constexpr int kMaxPartials = 65536;  // partial slots per reduction buffer
+constexpr int kMaxPartials = 65536;  // partial slots per value
+constexpr int kFinalizeThreads = 1024;
 
 enum RedOp : int { kSum = 0, kMax = 1 };
 
@@ -99,6 +102,7 @@ __device__ inline double wave_max(double v) {
 }
 
 // Block-wide reduction of NV values, result valid in thread 0. ops[k] selects sum or max.
+// Blocks of up to 1024 threads.
 template <int NV>
 __device__ inline void block_reduce(double (&v)[NV], const int (&ops)[NV]) {
   __shared__ double red[16][NV];
@@ -124,46 +128,66 @@ __device__ inline void block_reduce(double (&v)[NV], const int (&ops)[NV]) {
   __syncthreads();
 }
 
-// Writes this block's partials; returns true (in every thread of the block) if this block is
-// the last one to finish, in which case `out` holds the combined values (valid in thread 0)
-// and the counter has been reset.
+// Reduces v over the block and stores the block's partials at partials[k * kMaxPartials + bid].
 template <int NV>
-__device__ inline bool reduce_partials(double (&v)[NV], const int (&ops)[NV], double* partials, unsigned* counter,
-                                       double (&out)[NV]) {
+__device__ inline void store_partials(double (&v)[NV], const int (&ops)[NV], double* __restrict__ partials) {
   block_reduce<NV>(v, ops);
-  const unsigned nblocks = gridDim.x * gridDim.y * gridDim.z;
-  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-  const int nthreads = blockDim.x * blockDim.y * blockDim.z;
-  __shared__ int last;
-  if(tid == 0) {
+  if(threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) {
+    const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 #pragma unroll
     for(int k = 0; k < NV; ++k) partials[(size_t)k * kMaxPartials + bid] = v[k];
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    last = prev == nblocks - 1;
   }
-  __syncthreads();
-  if(!last) return false;
-  __threadfence();
-  double acc[NV];
+}
+
+// Combines nb partials per value in a fixed order (one block); result valid in thread 0.
+template <int NV>
+__device__ inline void fold_partials(const double* __restrict__ partials, unsigned nb, const int (&ops)[NV],
+                                     double (&out)[NV]) {
 #pragma unroll
-  for(int k = 0; k < NV; ++k) acc[k] = ops[k] == kSum ? 0.0 : -INFINITY;
-  for(unsigned b = tid; b < nblocks; b += nthreads) {
+  for(int k = 0; k < NV; ++k) out[k] = ops[k] == kSum ? 0.0 : -INFINITY;
+#pragma unroll 4
+  for(unsigned b = threadIdx.x; b < nb; b += blockDim.x) {
 #pragma unroll
     for(int k = 0; k < NV; ++k) {
       const double x = partials[(size_t)k * kMaxPartials + b];
-      acc[k] = ops[k] == kSum ? acc[k] + x : fmax(acc[k], x);
+      out[k] = ops[k] == kSum ? out[k] + x : fmax(out[k], x);
     }
   }
-  block_reduce<NV>(acc, ops);
-  if(tid == 0) {
-#pragma unroll
-    for(int k = 0; k < NV; ++k) out[k] = acc[k];
-    atomicExch(counter, 0u);
-  }
-  return true;
+  block_reduce<NV>(out, ops);
 }
+
+// Finalize kernel: Epi provides NV, op(k), active() and operator()(res) (run by thread 0).
+template <class Epi>
+__global__ void __launch_bounds__(kFinalizeThreads) k_finalize(const double* __restrict__ partials, unsigned nb,
+                                                               Epi epi) {
+  if(!epi.active()) return;
+  int ops[Epi::NV];
+#pragma unroll
+  for(int k = 0; k < Epi::NV; ++k) ops[k] = epi.op(k);
+  double res[Epi::NV];
+  fold_partials<Epi::NV>(partials, nb, ops, res);
+  if(threadIdx.x == 0) epi(res);
+}
+
+template <class Epi>
+inline void launch_finalize(hipStream_t s, const double* partials, dim3 grid, const Epi& epi) {
+  hipLaunchKernelGGL(k_finalize<Epi>, dim3(1), dim3(kFinalizeThreads), 0, s, partials, grid.x * grid.y * grid.z,
+                     epi);
+}
+
+// Epilogue that stores the NV results at out[0..NV-1]; bit k of max_mask selects max for value k.
+template <int N>
+struct EpiStore {
+  static constexpr int NV = N;
+  double* out;
+  unsigned max_mask;
+  __device__ int op(int k) const { return (max_mask >> k) & 1u ? kMax : kSum; }
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double (&r)[N]) const {
+#pragma unroll
+    for(int k = 0; k < N; ++k) out[k] = r[k];
+  }
+};
 
 // ------------------------------------------------------------------ XCD-aware block order
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each with a private

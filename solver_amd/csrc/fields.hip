@@ -393,7 +393,7 @@ __global__ void k_predictor_rhs(VelPtrs V, const double* __restrict__ p, Lay Lp,
 // ------------------------------------------------------------------ CFL diagnostic
 // solver/momentum_terms.cpp:373-401
 __global__ void k_cfl(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w, Lay Lu,
-                      Lay Lv, Lay Lw, of_grid g, double dt, double* partials, unsigned* counter, double* out, int kz) {
+                      Lay Lv, Lay Lw, of_grid g, double dt, double* partials, int kz) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   double vals[4] = {0.0, 0.0, 0.0, 0.0};
@@ -411,18 +411,24 @@ __global__ void k_cfl(const double* __restrict__ u, const double* __restrict__ v
     }
   }
   const int ops[4] = {kMax, kMax, kMax, kMax};
-  double res[4];
-  if(reduce_partials<4>(vals, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
-    out[0] = dt;
-    out[1] = res[0];
-    out[2] = res[1];
-    out[3] = res[2];
-    out[4] = res[3];
-  }
+  store_partials<4>(vals, ops, partials);
 }
 
+// out = {dt, max cfl, max |uc|, max |vc|, max |wc|}
+struct EpiCfl {
+  static constexpr int NV = 4;
+  double* out;
+  double dt;
+  __device__ int op(int) const { return kMax; }
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double (&r)[4]) const {
+    out[0] = dt;
+    for(int k = 0; k < 4; ++k) out[1 + k] = r[k];
+  }
+};
+
 // sum of squares and max |x| over the active box
-__global__ void k_norms(const double* __restrict__ x, Lay L, double* partials, unsigned* counter, double* out, int kz) {
+__global__ void k_norms(const double* __restrict__ x, Lay L, double* partials, int kz) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   double vals[2] = {0.0, 0.0};
@@ -435,11 +441,7 @@ __global__ void k_norms(const double* __restrict__ x, Lay L, double* partials, u
     }
   }
   const int ops[2] = {kSum, kMax};
-  double res[2];
-  if(reduce_partials<2>(vals, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
-    out[0] = res[0];
-    out[1] = res[1];
-  }
+  store_partials<2>(vals, ops, partials);
 }
 
 __global__ void k_scale_active(const double* in, double* out, Lay L, double s, int kz) {
@@ -623,13 +625,15 @@ void launch_cfl(Context& c, const of_grid& g, const double* u, const double* v, 
                 double* out5) {
   const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz);
   hipLaunchKernelGGL(k_cfl, cl.grid, cl.block, 0, c.stream, u, v, w, make_lay(g, OF_FACE_X), make_lay(g, OF_FACE_Y),
-                     make_lay(g, OF_FACE_Z), g, dt, c.partials, c.counters + 0, out5, cl.kz);
+                     make_lay(g, OF_FACE_Z), g, dt, c.partials, cl.kz);
+  launch_finalize(c.stream, c.partials, cl.grid, EpiCfl{out5, dt});
   OFH_CHECK(hipGetLastError());
 }
 
 void launch_norms(Context& c, const Lay& L, const double* x, int slot) {
   const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
-  hipLaunchKernelGGL(k_norms, cl.grid, cl.block, 0, c.stream, x, L, c.partials, c.counters + 1, c.scal + slot, cl.kz);
+  hipLaunchKernelGGL(k_norms, cl.grid, cl.block, 0, c.stream, x, L, c.partials, cl.kz);
+  launch_finalize(c.stream, c.partials, cl.grid, EpiStore<2>{c.scal + slot, 2u});
   OFH_CHECK(hipGetLastError());
 }
 

@@ -14,7 +14,6 @@ struct Context {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   double* partials = nullptr;   // kMaxRed x kMaxPartials reduction partials
-  unsigned* counters = nullptr; // reduction completion counters
   double* scal = nullptr;       // device scalars
   double* host_scal = nullptr;  // pinned mirror for read-back
   // grow-only scratch pool for the per-operator entry points
@@ -50,7 +49,6 @@ enum ScalarSlot : int {
   kScalPoisson = 64,  // MGPCG scalars (see poisson.hip)
   kScalCount = 256
 };
-enum CounterSlot : int { kCounterCount = 64 };
 
 // Launch shape for a "k-column" kernel over an active box ex*ey*ez: blocks of 64x4 threads,
 // each thread walking up to kz consecutive k so that the grid stays within kMaxPartials.

@@ -64,6 +64,7 @@ enum PS : int {
   P_LEVEL = 32,    // per level: 8 slots (see LS)
   P_COUNT = 32 + 8 * 24
 };
+constexpr int kHostMirror = 8;  // leading scalars published to the host after every residual update
 enum LS : int { L_MB = 0, L_SBW, L_MX2, L_MX4, L_MX5, L_SUMB, L_SPARE1, L_SPARE2 };
 
 struct PLevel {
@@ -174,67 +175,114 @@ __device__ inline double boundary_source(const PLevel& L, int i, int j, int k) {
          (nb[2][1] - 2.0 * 0.0 + nb[2][0]) / L.hz2;
 }
 
-// ---------------------------------------------------------------- setup
-// sys = -rhs - src (active cells), p_c = warm start; sums for their means.
-__global__ void k_setup1(const double* __restrict__ rhs_g, const double* __restrict__ p_g, Lay Lc, PLevel L,
-                         double* __restrict__ sys, double* __restrict__ pc, double* partials, unsigned* counter,
-                         double* ps, int zm, int kz) {
-  const Tile t = tile(L, kz);
-  double v[2] = {0.0, 0.0};
-  if(t.i < L.nx && t.j < L.ny) {
-    for(int k = t.k0; k < t.k1; ++k) {
-      const size_t id = (size_t)t.i + (size_t)L.nx * ((size_t)t.j + (size_t)L.ny * k);
-      const size_t gi = Lc.at(t.i + 1, t.j + 1, k + 1);
-      const double s = -rhs_g[gi] + -1.0 * boundary_source(L, t.i, t.j, k);
-      const double p = p_g[gi];
-      sys[id] = s;
-      pc[id] = p;
-      v[0] += s;
-      v[1] += p;
-    }
-  }
-  const int ops[2] = {kSum, kSum};
-  double res[2];
-  if(reduce_partials<2>(v, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
-    ps[P_MSYS] = zm ? res[0] / (double)L.n : 0.0;
-    ps[P_MP] = zm ? res[1] / (double)L.n : 0.0;
-  }
+// ---------------------------------------------------------------- per-cell formulas
+// Shared by the tiled level kernels and the one-workgroup tail. Neighbour data (Nb) per axis;
+// the deferred means are subtracted on read.
+
+// Fused first two pre-smoothing sweeps at (i,j,k): x1 = (b - mb) w/D - m1 (pointwise, from
+// x0 = 0), x2 = x1 + w/D (b - mb - A x1).
+template <typename T>
+__device__ inline T smooth12_cell(const PLevel& L, const T* __restrict__ b, T mb, T m1, int i, int j, int k,
+                                  const Nb& x, const Nb& y, const Nb& z) {
+  const size_t plane = (size_t)L.nx * L.ny;
+  auto W = [&](int cx, int cy, int cz) { return T(L.winv[cx + 4 * (cy + 4 * cz)]); };
+  auto B = [&](int ii, int jj, int kk) { return b[(size_t)ii + (size_t)L.nx * jj + plane * kk] - mb; };
+  const int cx = bclass(i, L.nx), cy = bclass(j, L.ny), cz = bclass(k, L.nz);
+  const T wc = W(cx, cy, cz);
+  const T bt = B(i, j, k);
+  const T c = bt * wc - m1;
+  const T xw = sgn(x.sm, B(x.cm, j, k) * W(bclass(x.cm, L.nx), cy, cz) - m1);
+  const T xe = sgn(x.sp, B(x.cp, j, k) * W(bclass(x.cp, L.nx), cy, cz) - m1);
+  const T xs = sgn(y.sm, B(i, y.cm, k) * W(cx, bclass(y.cm, L.ny), cz) - m1);
+  const T xn = sgn(y.sp, B(i, y.cp, k) * W(cx, bclass(y.cp, L.ny), cz) - m1);
+  const T xb = sgn(z.sm, B(i, j, z.cm) * W(cx, cy, bclass(z.cm, L.nz)) - m1);
+  const T xf = sgn(z.sp, B(i, j, z.cp) * W(cx, cy, bclass(z.cp, L.nz)) - m1);
+  const T ax = lap_neg<T>(c, xw, xe, xs, xn, xb, xf, T(L.ihx2), T(L.ihy2), T(L.ihz2));
+  return c + wc * (bt + T(-1) * ax);
 }
 
-// r = (sys - m_sys) - A(p - m_p); sums of r, r^2, r w/D; initial residual and early exit.
-__global__ void k_setup2(double* __restrict__ r, const double* __restrict__ pc, PLevel L, double* partials,
-                         unsigned* counter, double* ps, double* history, double tol, int zm, int kz) {
-  const double msys = ps[P_MSYS], mp = ps[P_MP];
-  const Tile t = tile(L, kz);
-  const Col C = column(L, t);
-  double v[3] = {0.0, 0.0, 0.0};
-  if(C.in) {
-    const size_t plane = (size_t)L.nx * L.ny;
-    auto X = [&](int ii, int jj, int kk) { return pc[(size_t)ii + (size_t)L.nx * jj + plane * kk] - mp; };
-    for(int k = t.k0; k < t.k1; ++k) {
-      const Nb z = nb_axis(k, L.nz, L.per[2], L.lo_dir[2], L.hi_dir[2]);
-      const double c = X(t.i, t.j, k);
-      const double q = lap_neg<double>(c, sgn(C.x.sm, X(C.x.cm, t.j, k)), sgn(C.x.sp, X(C.x.cp, t.j, k)),
-                                       sgn(C.y.sm, X(t.i, C.y.cm, k)), sgn(C.y.sp, X(t.i, C.y.cp, k)),
-                                       sgn(z.sm, X(t.i, t.j, z.cm)), sgn(z.sp, X(t.i, t.j, z.cp)), L.ihx2, L.ihy2,
-                                       L.ihz2);
-      const size_t id = (size_t)t.i + (size_t)L.nx * t.j + plane * k;
-      const double rv = (r[id] - msys) + -1.0 * q;
-      r[id] = rv;
-      v[0] += rv;
-      v[1] += rv * rv;
-      v[2] += rv * L.winv[C.cx + 4 * (C.cy + 4 * bclass(k, L.nz))];
-    }
+// Fine residual (b - mb) - A (x2 - mx) at (i,j,k).
+template <typename T>
+__device__ inline T residual_cell(const PLevel& L, const T* __restrict__ b, const T* __restrict__ x2, T mb, T mx,
+                                  int i, int j, int k, const Nb& x, const Nb& y, const Nb& z) {
+  const size_t plane = (size_t)L.nx * L.ny;
+  auto X = [&](int ii, int jj, int kk) { return x2[(size_t)ii + (size_t)L.nx * jj + plane * kk] - mx; };
+  const T c = X(i, j, k);
+  const T ax = lap_neg<T>(c, sgn(x.sm, X(x.cm, j, k)), sgn(x.sp, X(x.cp, j, k)), sgn(y.sm, X(i, y.cm, k)),
+                          sgn(y.sp, X(i, y.cp, k)), sgn(z.sm, X(i, j, z.cm)), sgn(z.sp, X(i, j, z.cp)), T(L.ihx2),
+                          T(L.ihy2), T(L.ihz2));
+  return (b[(size_t)i + (size_t)L.nx * j + plane * k] - mb) + T(-1) * ax;
+}
+
+// Prolongation-add fused with the first post-smoothing sweep:
+// x3 = (x2 - m2) + (xc - mc)[parent]; returns x4 = x3 + w/D (b - mb - A x3).
+template <typename T>
+__device__ inline T prolong_smooth_cell(const PLevel& L, const PLevel& Cl, const T* __restrict__ b,
+                                        const T* __restrict__ x2, const T* __restrict__ xc, T mb, T m2, T mc, int i,
+                                        int j, int k, const Nb& x, const Nb& y, const Nb& z) {
+  const size_t plane = (size_t)L.nx * L.ny;
+  const size_t cplane = (size_t)Cl.nx * Cl.ny;
+  auto X3 = [&](int ii, int jj, int kk) {
+    return (x2[(size_t)ii + (size_t)L.nx * jj + plane * kk] - m2) +
+           (xc[(size_t)(ii / L.rx) + (size_t)Cl.nx * (jj / L.ry) + cplane * (kk / L.rz)] - mc);
+  };
+  const T wc = T(L.winv[bclass(i, L.nx) + 4 * (bclass(j, L.ny) + 4 * bclass(k, L.nz))]);
+  const T c = X3(i, j, k);
+  const T ax = lap_neg<T>(c, sgn(x.sm, X3(x.cm, j, k)), sgn(x.sp, X3(x.cp, j, k)), sgn(y.sm, X3(i, y.cm, k)),
+                          sgn(y.sp, X3(i, y.cp, k)), sgn(z.sm, X3(i, j, z.cm)), sgn(z.sp, X3(i, j, z.cp)), T(L.ihx2),
+                          T(L.ihy2), T(L.ihz2));
+  return c + wc * ((b[(size_t)i + (size_t)L.nx * j + plane * k] - mb) + T(-1) * ax);
+}
+
+// Second post-smoothing sweep: x4' = x4 - m4, returns x5 = x4' + w/D (bt - A x4'), bt = b - mb.
+template <typename T>
+__device__ inline T smooth_last_cell(const PLevel& L, const T* __restrict__ b, const T* __restrict__ x4, T mb, T m4,
+                                     int i, int j, int k, const Nb& x, const Nb& y, const Nb& z, T& bt) {
+  const size_t plane = (size_t)L.nx * L.ny;
+  auto X = [&](int ii, int jj, int kk) { return x4[(size_t)ii + (size_t)L.nx * jj + plane * kk] - m4; };
+  const T wc = T(L.winv[bclass(i, L.nx) + 4 * (bclass(j, L.ny) + 4 * bclass(k, L.nz))]);
+  const T c = X(i, j, k);
+  const T ax = lap_neg<T>(c, sgn(x.sm, X(x.cm, j, k)), sgn(x.sp, X(x.cp, j, k)), sgn(y.sm, X(i, y.cm, k)),
+                          sgn(y.sp, X(i, y.cp, k)), sgn(z.sm, X(i, j, z.cm)), sgn(z.sp, X(i, j, z.cp)), T(L.ihx2),
+                          T(L.ihy2), T(L.ihz2));
+  bt = b[(size_t)i + (size_t)L.nx * j + plane * k] - mb;
+  return c + wc * (bt + T(-1) * ax);
+}
+
+__device__ inline Nb nb_x(const PLevel& L, int i) { return nb_axis(i, L.nx, L.per[0], L.lo_dir[0], L.hi_dir[0]); }
+__device__ inline Nb nb_y(const PLevel& L, int j) { return nb_axis(j, L.ny, L.per[1], L.lo_dir[1], L.hi_dir[1]); }
+__device__ inline Nb nb_z(const PLevel& L, int k) { return nb_axis(k, L.nz, L.per[2], L.lo_dir[2], L.hi_dir[2]); }
+
+// ---------------------------------------------------------------- reduction epilogues
+// Run by the one-block finalize kernel after the producer. An epilogue is skipped once the
+// solve has stopped (its producer returned early and stored no partials) unless noted.
+struct EpiSetup1 {  // means of the system rhs and of the warm start
+  static constexpr int NV = 2;
+  double* ps;
+  double n;
+  int zm;
+  __device__ int op(int) const { return kSum; }
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double (&r)[2]) const {
+    ps[P_MSYS] = zm ? r[0] / n : 0.0;
+    ps[P_MP] = zm ? r[1] / n : 0.0;
   }
-  const int ops[3] = {kSum, kSum, kSum};
-  double res[3];
-  if(reduce_partials<3>(v, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
-    const double n = (double)L.n;
-    const double mr = zm ? res[0] / n : 0.0;
+};
+
+struct EpiSetup2 {  // initial residual, early exit (linsolve/poisson_solver.cpp:703-725)
+  static constexpr int NV = 3;
+  double* ps;
+  double* history;
+  double n, tol;
+  int zm;
+  __device__ int op(int) const { return kSum; }
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double (&r)[3]) const {
+    const double mr = zm ? r[0] / n : 0.0;
     ps[P_MR] = mr;
     ps[P_LEVEL + L_MB] = mr;
-    ps[P_LEVEL + L_SBW] = res[2];
-    const double rms = sqrt(fmax(res[1] - n * mr * mr, 0.0) / n);
+    ps[P_LEVEL + L_SBW] = r[2];
+    const double rms = sqrt(fmax(r[1] - n * mr * mr, 0.0) / n);
     ps[P_R0] = rms;
     ps[P_RMS] = rms;
     ps[P_REL] = rms > 0.0 ? 1.0 : 0.0;
@@ -250,334 +298,737 @@ __global__ void k_setup2(double* __restrict__ r, const double* __restrict__ pc, 
       ps[P_CONV] = 0.0;
     }
   }
+};
+
+struct EpiMean {  // ps[idx] = mean of the produced vector
+  static constexpr int NV = 1;
+  double* ps;
+  int idx;
+  double n;
+  int zm;
+  __device__ int op(int) const { return kSum; }
+  __device__ bool active() const { return !stopped(ps); }
+  __device__ void operator()(const double (&r)[1]) const { ps[idx] = zm ? r[0] / n : 0.0; }
+};
+
+struct EpiRestrict {  // coarse rhs: mean, sum of b w/D, sum
+  static constexpr int NV = 2;
+  double* ps;
+  int cslot;
+  double nc;
+  int zm;
+  __device__ int op(int) const { return kSum; }
+  __device__ bool active() const { return !stopped(ps); }
+  __device__ void operator()(const double (&r)[2]) const {
+    ps[cslot + L_MB] = zm ? r[0] / nc : 0.0;
+    ps[cslot + L_SBW] = r[1];
+    ps[cslot + L_SUMB] = r[0];
+  }
+};
+
+struct EpiSmoothLast {  // mean of x5; on the finest level r.z and beta
+  static constexpr int NV = 2;
+  double* ps;
+  int lslot;
+  double n;
+  int zm, finest;
+  __device__ int op(int) const { return kSum; }
+  __device__ bool active() const { return !stopped(ps); }
+  __device__ void operator()(const double (&r)[2]) const {
+    ps[lslot + L_MX5] = zm ? r[0] / n : 0.0;
+    if(finest) {
+      const double rz_old = ps[P_RZ];
+      ps[P_BETA] = rz_old != 0.0 ? r[1] / rz_old : 0.0;
+      ps[P_RZ] = r[1];
+    }
+  }
+};
+
+struct EpiDir {  // means of d and q, d.q, alpha or breakdown (linsolve/poisson_solver.cpp:752-762)
+  static constexpr int NV = 3;
+  double* ps;
+  double n;
+  int zm, dsel;
+  __device__ int op(int) const { return kSum; }
+  __device__ bool active() const { return !stopped(ps); }
+  __device__ void operator()(const double (&r)[3]) const {
+    const double mdn = zm ? r[0] / n : 0.0;
+    const double mq = zm ? r[1] / n : 0.0;
+    ps[P_MD] = mdn;
+    ps[P_MQ] = mq;
+    ps[P_PEND] = 0.0;
+    ps[P_DSEL] = (double)dsel;
+    const double dq = r[2] - mdn * r[1];
+    ps[P_DQ] = dq;
+    if(fabs(dq) < kSmallNumber) {
+      ps[P_DONE] = 1.0;  // breakdown: stop without convergence
+    } else {
+      ps[P_ALPHA] = ps[P_RZ] / dq;
+    }
+  }
+};
+
+// Residual RMS and convergence test (linsolve/poisson_solver.cpp:764-776). Always runs: it
+// also publishes the first scalars to the mapped host mirror, from which the host learns that
+// the solve has stopped.
+struct EpiUpdate {
+  static constexpr int NV = 3;
+  double* ps;
+  double* history;
+  double* host_out;
+  double n, tol;
+  int it, zm;
+  __device__ int op(int) const { return kSum; }
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double (&r)[3]) const {
+    if(!stopped(ps)) {
+      const double m = zm ? r[0] / n : 0.0;
+      ps[P_MR] = m;
+      ps[P_LEVEL + L_MB] = m;
+      ps[P_LEVEL + L_SBW] = r[2];
+      const double rms = sqrt(fmax(r[1] - n * m * m, 0.0) / n);
+      ps[P_ITER] = (double)(it + 1);
+      ps[P_RMS] = rms;
+      ps[P_PEND] = 1.0;
+      const double rel = ps[P_R0] > 0.0 ? rms / ps[P_R0] : 0.0;
+      ps[P_REL] = rel;
+      history[it + 1] = rms;
+      if(rel <= tol) {
+        ps[P_CONV] = 1.0;
+        ps[P_DONE] = 1.0;
+      }
+    }
+    for(int q = 0; q < kHostMirror; ++q)
+      __hip_atomic_store(host_out + q, ps[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+};
+
+struct EpiPfinal {  // final mean of p
+  static constexpr int NV = 1;
+  double* ps;
+  double n;
+  int zm;
+  __device__ int op(int) const { return kSum; }
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double (&r)[1]) const {
+    ps[P_PFINAL] = zm ? r[0] / n : 0.0;
+    ps[P_PEND] = 0.0;
+  }
+};
+
+// ---------------------------------------------------------------- setup
+// sys = -rhs - src (active cells), p_c = warm start; sums for their means.
+__global__ void k_setup1(const double* __restrict__ rhs_g, const double* __restrict__ p_g, Lay Lc, PLevel L,
+                         double* __restrict__ sys, double* __restrict__ pc, double* partials, int kz) {
+  const Tile t = tile(L, kz);
+  double v[2] = {0.0, 0.0};
+  if(t.i < L.nx && t.j < L.ny) {
+    for(int k = t.k0; k < t.k1; ++k) {
+      const size_t id = (size_t)t.i + (size_t)L.nx * ((size_t)t.j + (size_t)L.ny * k);
+      const size_t gi = Lc.at(t.i + 1, t.j + 1, k + 1);
+      const double s = -rhs_g[gi] + -1.0 * boundary_source(L, t.i, t.j, k);
+      const double p = p_g[gi];
+      sys[id] = s;
+      pc[id] = p;
+      v[0] += s;
+      v[1] += p;
+    }
+  }
+  const int ops[2] = {kSum, kSum};
+  store_partials<2>(v, ops, partials);
 }
 
-// ---------------------------------------------------------------- V-cycle kernels
-// Level rhs b of type T with deferred mean mb; iterates of type T.
+// r = (sys - m_sys) - A(p - m_p); sums of r, r^2, r w/D.
+__global__ void k_setup2(double* __restrict__ r, const double* __restrict__ pc, PLevel L, double* partials,
+                         const double* ps, int kz) {
+  const double msys = ps[P_MSYS], mp = ps[P_MP];
+  const Tile t = tile(L, kz);
+  const Col C = column(L, t);
+  double v[3] = {0.0, 0.0, 0.0};
+  if(C.in) {
+    const size_t plane = (size_t)L.nx * L.ny;
+    for(int k = t.k0; k < t.k1; ++k) {
+      const size_t id = (size_t)t.i + (size_t)L.nx * t.j + plane * k;
+      const double rv = residual_cell<double>(L, r, pc, msys, mp, t.i, t.j, k, C.x, C.y, nb_z(L, k));
+      r[id] = rv;
+      v[0] += rv;
+      v[1] += rv * rv;
+      v[2] += rv * L.winv[C.cx + 4 * (C.cy + 4 * bclass(k, L.nz))];
+    }
+  }
+  const int ops[3] = {kSum, kSum, kSum};
+  store_partials<3>(v, ops, partials);
+}
 
-// S12: x2 = x1 + w/D (b - A x1), x1 = (b - mb) w/D - m1 (fused first two pre-smoothing sweeps;
-// m1 = mean of x1 from the producer's sum of b w/D).
-template <typename T>
-__global__ void k_smooth12(const T* __restrict__ b, T* __restrict__ x2, PLevel L, double* partials, unsigned* counter,
-                           double* ps, int lslot, int zm, int kz) {
+// ---------------------------------------------------------------- tiled level kernels
+// 64x4 thread tiles over (i, j); each thread marches KZ consecutive k (unrolled) and keeps the
+// k-1 / k / k+1 values of the stencil operand in registers. The 1D grid is remapped so that
+// each XCD owns a contiguous range of tiles (neighbouring tiles share halo rows in its L2).
+// UNI: every axis periodic, so omega/D is one constant and no boundary image changes sign.
+struct HotTile {
+  int i, j, k0, k1;
+  bool in;
+};
+template <int KZ>
+__device__ inline HotTile hot_tile(const PLevel& L) {
+  const int gx = (L.nx + 63) >> 6, gy = (L.ny + 3) >> 2;
+  unsigned r = xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = (int)(r % (unsigned)gx);
+  r /= (unsigned)gx;
+  const int by = (int)(r % (unsigned)gy);
+  const int bz = (int)(r / (unsigned)gy);
+  HotTile t;
+  t.i = bx * 64 + threadIdx.x;
+  t.j = by * 4 + threadIdx.y;
+  t.k0 = bz * KZ;
+  t.k1 = min(t.k0 + KZ, L.nz);
+  t.in = t.i < L.nx && t.j < L.ny;
+  return t;
+}
+
+// z-window bookkeeping for a column: index of the k+1 neighbour (k itself past a
+// non-periodic top face, where the value is unused) and the boundary images.
+template <bool UNI>
+struct ZWin {
+  int nz;
+  bool per;
+  int slo, shi;  // sign of the image below k = 0 / above k = nz - 1 (non-periodic)
+  __device__ ZWin(const PLevel& L) : nz(L.nz), per(UNI || L.per[2] != 0), slo(L.lo_dir[2] ? -1 : 1), shi(L.hi_dir[2] ? -1 : 1) {}
+  __device__ int kprev(int k0) const { return k0 > 0 ? k0 - 1 : (per ? nz - 1 : 0); }
+  __device__ int knext(int k) const { return k + 1 < nz ? k + 1 : (per ? 0 : k); }
+  template <typename T>
+  __device__ T below(int k, T prev, T cur) const { return (!per && k == 0) ? sgn(slo, cur) : prev; }
+  template <typename T>
+  __device__ T above(int k, T next, T cur) const { return (!per && k == nz - 1) ? sgn(shi, cur) : next; }
+};
+
+template <typename T, bool UNI>
+__device__ inline T wdiag(const PLevel& L, int cx, int cy, int cz) {
+  return UNI ? T(L.winv[0]) : T(L.winv[cx + 4 * (cy + 4 * cz)]);
+}
+
+template <typename T, int KZ, bool UNI>
+__global__ void __launch_bounds__(256) k_smooth12(const T* __restrict__ b, T* __restrict__ x2, PLevel L,
+                                                  double* partials, const double* ps, int lslot, int zm) {
   if(stopped(ps)) return;
   const T mb = T(ps[lslot + L_MB]);
   const T m1 = zm ? T((ps[lslot + L_SBW] - ps[lslot + L_MB] * L.winv_sum) / (double)L.n) : T(0);
   const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
-  const Tile t = tile(L, kz);
-  const Col C = column(L, t);
+  const HotTile t = hot_tile<KZ>(L);
   double v[1] = {0.0};
-  if(C.in) {
+  if(t.in) {
+    const ZWin<UNI> zw(L);
     const size_t plane = (size_t)L.nx * L.ny;
-    auto W = [&](int cx, int cy, int cz) { return T(L.winv[cx + 4 * (cy + 4 * cz)]); };
-    const int cxm = bclass(C.x.cm, L.nx), cxp = bclass(C.x.cp, L.nx);
-    const int cym = bclass(C.y.cm, L.ny), cyp = bclass(C.y.cp, L.ny);
-    auto B = [&](int ii, int jj, int kk) { return b[(size_t)ii + (size_t)L.nx * jj + plane * kk] - mb; };
-    for(int k = t.k0; k < t.k1; ++k) {
-      const Nb z = nb_axis(k, L.nz, L.per[2], L.lo_dir[2], L.hi_dir[2]);
-      const int cz = bclass(k, L.nz), czm = bclass(z.cm, L.nz), czp = bclass(z.cp, L.nz);
-      const T wc = W(C.cx, C.cy, cz);
-      const T bt = B(t.i, t.j, k);
-      const T c = bt * wc - m1;
-      const T xw = sgn(C.x.sm, B(C.x.cm, t.j, k) * W(cxm, C.cy, cz) - m1);
-      const T xe = sgn(C.x.sp, B(C.x.cp, t.j, k) * W(cxp, C.cy, cz) - m1);
-      const T xs = sgn(C.y.sm, B(t.i, C.y.cm, k) * W(C.cx, cym, cz) - m1);
-      const T xn = sgn(C.y.sp, B(t.i, C.y.cp, k) * W(C.cx, cyp, cz) - m1);
-      const T xb = sgn(z.sm, B(t.i, t.j, z.cm) * W(C.cx, C.cy, czm) - m1);
-      const T xf = sgn(z.sp, B(t.i, t.j, z.cp) * W(C.cx, C.cy, czp) - m1);
-      const T ax = lap_neg<T>(c, xw, xe, xs, xn, xb, xf, ihx2, ihy2, ihz2);
-      const T xn2 = c + wc * (bt + T(-1) * ax);
-      x2[(size_t)t.i + (size_t)L.nx * t.j + plane * k] = xn2;
-      v[0] += (double)xn2;
+    const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+    const Nb x = nb_x(L, t.i), y = nb_y(L, t.j);
+    const int cx = bclass(t.i, L.nx), cy = bclass(t.j, L.ny);
+    const int cxm = bclass(x.cm, L.nx), cxp = bclass(x.cp, L.nx), cym = bclass(y.cm, L.ny), cyp = bclass(y.cp, L.ny);
+    const size_t oxm = (size_t)x.cm + (size_t)L.nx * t.j, oxp = (size_t)x.cp + (size_t)L.nx * t.j;
+    const size_t oym = (size_t)t.i + (size_t)L.nx * y.cm, oyp = (size_t)t.i + (size_t)L.nx * y.cp;
+    auto X1 = [&](size_t id, T w) { return (b[id] - mb) * w - m1; };
+    const int kp0 = zw.kprev(t.k0);
+    T prev = X1(col + plane * kp0, wdiag<T, UNI>(L, cx, cy, bclass(kp0, L.nz)));
+    int cz = bclass(t.k0, L.nz);
+    T wc = wdiag<T, UNI>(L, cx, cy, cz);
+    T btc = b[col + plane * t.k0] - mb;
+    T cur = btc * wc - m1;
+#pragma unroll
+    for(int kk = 0; kk < KZ; ++kk) {
+      const int k = t.k0 + kk;
+      if(k < t.k1) {
+        const int kn = zw.knext(k);
+        const int czn = bclass(kn, L.nz);
+        const T wn = wdiag<T, UNI>(L, cx, cy, czn);
+        const T btn = b[col + plane * kn] - mb;
+        const T next = btn * wn - m1;
+        const size_t pk = plane * k;
+        const T xw = sgn(x.sm, X1(oxm + pk, wdiag<T, UNI>(L, cxm, cy, cz)));
+        const T xe = sgn(x.sp, X1(oxp + pk, wdiag<T, UNI>(L, cxp, cy, cz)));
+        const T xs = sgn(y.sm, X1(oym + pk, wdiag<T, UNI>(L, cx, cym, cz)));
+        const T xn = sgn(y.sp, X1(oyp + pk, wdiag<T, UNI>(L, cx, cyp, cz)));
+        const T ax = lap_neg<T>(cur, xw, xe, xs, xn, zw.below(k, prev, cur), zw.above(k, next, cur), ihx2, ihy2, ihz2);
+        const T out = cur + wc * (btc + T(-1) * ax);
+        x2[col + pk] = out;
+        v[0] += (double)out;
+        prev = cur;
+        cur = next;
+        btc = btn;
+        wc = wn;
+        cz = czn;
+      }
     }
   }
   const int ops[1] = {kSum};
-  double res[1];
-  if(reduce_partials<1>(v, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0)
-    ps[lslot + L_MX2] = zm ? res[0] / (double)L.n : 0.0;
+  store_partials<1>(v, ops, partials);
 }
 
 // Residual of x2 restricted to the coarse level: bc = avg_children(b - A x2). Each thread
 // computes the fine residual of its own cell; x-pairs are combined across lanes, y-pairs through
-// LDS, z-pairs along the thread's k-march (kz is even).
-template <typename T>
-__global__ void k_residual_restrict(const T* __restrict__ b, const T* __restrict__ x2, PLevel L, PLevel Cl,
-                                    T* __restrict__ bc, double* partials, unsigned* counter, double* ps, int lslot,
-                                    int cslot, int zm, int kz) {
+// LDS, z-pairs along the thread's k-march (KZ is even).
+template <typename T, int KZ, bool UNI>
+__global__ void __launch_bounds__(256) k_residual_restrict(const T* __restrict__ b, const T* __restrict__ x2, PLevel L,
+                                                           PLevel Cl, T* __restrict__ bc, double* partials,
+                                                           const double* ps, int lslot, int zm) {
   if(stopped(ps)) return;
   __shared__ T ypair[4][64];
   const T mb = T(ps[lslot + L_MB]);
   const T mx = T(ps[lslot + L_MX2]);
   const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
   const T inv_children = T(1.0 / (double)(L.rx * L.ry * L.rz));
-  const Tile t = tile(L, kz);
-  const Col C = column(L, t);
+  const HotTile t = hot_tile<KZ>(L);
+  const ZWin<UNI> zw(L);
   const size_t plane = (size_t)L.nx * L.ny;
   const size_t cplane = (size_t)Cl.nx * Cl.ny;
-  auto X = [&](int ii, int jj, int kk) { return x2[(size_t)ii + (size_t)L.nx * jj + plane * kk] - mx; };
+  const int ii = min(t.i, L.nx - 1), jj = min(t.j, L.ny - 1);
+  const size_t col = (size_t)ii + (size_t)L.nx * jj;
+  const Nb x = nb_x(L, ii), y = nb_y(L, jj);
+  const size_t oxm = (size_t)x.cm + (size_t)L.nx * jj, oxp = (size_t)x.cp + (size_t)L.nx * jj;
+  const size_t oym = (size_t)ii + (size_t)L.nx * y.cm, oyp = (size_t)ii + (size_t)L.nx * y.cp;
+  auto X = [&](size_t id) { return x2[id] - mx; };
+  T prev = X(col + plane * zw.kprev(t.k0));
+  T cur = X(col + plane * t.k0);
   double v[2] = {0.0, 0.0};
   T acc = T(0);
-  for(int k = t.k0; k < t.k0 + kz; ++k) {
-    T rr = T(0);
-    if(C.in && k < L.nz) {
-      const Nb z = nb_axis(k, L.nz, L.per[2], L.lo_dir[2], L.hi_dir[2]);
-      const T c = X(t.i, t.j, k);
-      const T ax = lap_neg<T>(c, sgn(C.x.sm, X(C.x.cm, t.j, k)), sgn(C.x.sp, X(C.x.cp, t.j, k)),
-                              sgn(C.y.sm, X(t.i, C.y.cm, k)), sgn(C.y.sp, X(t.i, C.y.cp, k)),
-                              sgn(z.sm, X(t.i, t.j, z.cm)), sgn(z.sp, X(t.i, t.j, z.cp)), ihx2, ihy2, ihz2);
-      rr = (b[(size_t)t.i + (size_t)L.nx * t.j + plane * k] - mb) + T(-1) * ax;
-    }
-    if(L.rx == 2) rr += __shfl_xor(rr, 1, 64);  // x pair (lanes i, i+1)
-    if(L.ry == 2) {                             // y pair (rows j, j+1 of the tile)
-      ypair[threadIdx.y][threadIdx.x] = rr;
-      __syncthreads();
-      rr = rr + ypair[threadIdx.y ^ 1][threadIdx.x];
-      __syncthreads();
-    }
-    const bool second = ((k - t.k0) & 1) != 0;
-    acc = (L.rz == 2 && second) ? acc + rr : rr;
-    const bool complete = L.rz == 1 || second;
-    if(complete && C.in && k < L.nz && (L.rx == 1 || (t.i & 1) == 0) && (L.ry == 1 || (t.j & 1) == 0)) {
-      const int ic = t.i / L.rx, jc = t.j / L.ry, kc = k / L.rz;
-      const T val = inv_children * acc;
-      bc[(size_t)ic + (size_t)Cl.nx * jc + cplane * kc] = val;
-      v[0] += (double)val;
-      v[1] += (double)val * Cl.winv[bclass(ic, Cl.nx) + 4 * (bclass(jc, Cl.ny) + 4 * bclass(kc, Cl.nz))];
+#pragma unroll
+  for(int kk = 0; kk < KZ; ++kk) {
+    const int k = t.k0 + kk;
+    if(k < t.k1) {
+      const T next = X(col + plane * zw.knext(k));
+      const size_t pk = plane * k;
+      const T ax = lap_neg<T>(cur, sgn(x.sm, X(oxm + pk)), sgn(x.sp, X(oxp + pk)), sgn(y.sm, X(oym + pk)),
+                              sgn(y.sp, X(oyp + pk)), zw.below(k, prev, cur), zw.above(k, next, cur), ihx2, ihy2, ihz2);
+      T rr = t.in ? (b[col + pk] - mb) + T(-1) * ax : T(0);
+      prev = cur;
+      cur = next;
+      if(L.rx == 2) rr += __shfl_xor(rr, 1, 64);  // x pair (lanes i, i+1)
+      if(L.ry == 2) {                             // y pair (rows j, j+1 of the tile)
+        ypair[threadIdx.y][threadIdx.x] = rr;
+        __syncthreads();
+        rr = rr + ypair[threadIdx.y ^ 1][threadIdx.x];
+        __syncthreads();
+      }
+      const bool second = (kk & 1) != 0;
+      acc = (L.rz == 2 && second) ? acc + rr : rr;
+      const bool complete = L.rz == 1 || second;
+      if(complete && t.in && (L.rx == 1 || (t.i & 1) == 0) && (L.ry == 1 || (t.j & 1) == 0)) {
+        const int ic = t.i / L.rx, jc = t.j / L.ry, kc = k / L.rz;
+        const T val = inv_children * acc;
+        bc[(size_t)ic + (size_t)Cl.nx * jc + cplane * kc] = val;
+        v[0] += (double)val;
+        v[1] += (double)val *
+                (UNI ? Cl.winv[0] : Cl.winv[bclass(ic, Cl.nx) + 4 * (bclass(jc, Cl.ny) + 4 * bclass(kc, Cl.nz))]);
+      }
     }
   }
   const int ops[2] = {kSum, kSum};
-  double res[2];
-  if(reduce_partials<2>(v, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
-    ps[cslot + L_MB] = zm ? res[0] / (double)Cl.n : 0.0;
-    ps[cslot + L_SBW] = res[1];
-    ps[cslot + L_SUMB] = res[0];
-  }
+  store_partials<2>(v, ops, partials);
 }
 
 // Prolongation-add fused with the first post-smoothing sweep:
-// x3 = (x2 - m2) + (xc - mc)[parent];  x4 = x3 + w/D (b - A x3).
-template <typename T>
-__global__ void k_prolong_smooth(const T* __restrict__ b, const T* __restrict__ x2, const T* __restrict__ xc,
-                                 PLevel L, PLevel Cl, T* __restrict__ x4, double* partials, unsigned* counter,
-                                 double* ps, int lslot, int cslot, int zm, int kz) {
+// x3 = (x2 - m2) + (xc - mc)[parent]; x4 = x3 + w/D (b - mb - A x3).
+template <typename T, int KZ, bool UNI>
+__global__ void __launch_bounds__(256) k_prolong_smooth(const T* __restrict__ b, const T* __restrict__ x2,
+                                                        const T* __restrict__ xc, PLevel L, PLevel Cl,
+                                                        T* __restrict__ x4, double* partials, const double* ps,
+                                                        int lslot, int cslot) {
   if(stopped(ps)) return;
   const T mb = T(ps[lslot + L_MB]);
   const T m2 = T(ps[lslot + L_MX2]);
   const T mc = T(ps[cslot + L_MX5]);
   const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
-  const Tile t = tile(L, kz);
-  const Col C = column(L, t);
+  const HotTile t = hot_tile<KZ>(L);
   double v[1] = {0.0};
-  if(C.in) {
+  if(t.in) {
+    const ZWin<UNI> zw(L);
     const size_t plane = (size_t)L.nx * L.ny;
     const size_t cplane = (size_t)Cl.nx * Cl.ny;
-    auto X3 = [&](int ii, int jj, int kk) {
-      return (x2[(size_t)ii + (size_t)L.nx * jj + plane * kk] - m2) +
-             (xc[(size_t)(ii / L.rx) + (size_t)Cl.nx * (jj / L.ry) + cplane * (kk / L.rz)] - mc);
-    };
-    for(int k = t.k0; k < t.k1; ++k) {
-      const Nb z = nb_axis(k, L.nz, L.per[2], L.lo_dir[2], L.hi_dir[2]);
-      const T wc = T(L.winv[C.cx + 4 * (C.cy + 4 * bclass(k, L.nz))]);
-      const T c = X3(t.i, t.j, k);
-      const T ax = lap_neg<T>(c, sgn(C.x.sm, X3(C.x.cm, t.j, k)), sgn(C.x.sp, X3(C.x.cp, t.j, k)),
-                              sgn(C.y.sm, X3(t.i, C.y.cm, k)), sgn(C.y.sp, X3(t.i, C.y.cp, k)),
-                              sgn(z.sm, X3(t.i, t.j, z.cm)), sgn(z.sp, X3(t.i, t.j, z.cp)), ihx2, ihy2, ihz2);
-      const size_t id = (size_t)t.i + (size_t)L.nx * t.j + plane * k;
-      const T xn = c + wc * ((b[id] - mb) + T(-1) * ax);
-      x4[id] = xn;
-      v[0] += (double)xn;
+    const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+    const Nb x = nb_x(L, t.i), y = nb_y(L, t.j);
+    const int cx = bclass(t.i, L.nx), cy = bclass(t.j, L.ny);
+    // parent columns of the cell and of its x / y neighbours
+    const size_t pc = (size_t)(t.i / L.rx) + (size_t)Cl.nx * (t.j / L.ry);
+    const size_t pxm = (size_t)(x.cm / L.rx) + (size_t)Cl.nx * (t.j / L.ry);
+    const size_t pxp = (size_t)(x.cp / L.rx) + (size_t)Cl.nx * (t.j / L.ry);
+    const size_t pym = (size_t)(t.i / L.rx) + (size_t)Cl.nx * (y.cm / L.ry);
+    const size_t pyp = (size_t)(t.i / L.rx) + (size_t)Cl.nx * (y.cp / L.ry);
+    const size_t oxm = (size_t)x.cm + (size_t)L.nx * t.j, oxp = (size_t)x.cp + (size_t)L.nx * t.j;
+    const size_t oym = (size_t)t.i + (size_t)L.nx * y.cm, oyp = (size_t)t.i + (size_t)L.nx * y.cp;
+    auto X3 = [&](size_t fine, size_t coarse) { return (x2[fine] - m2) + (xc[coarse] - mc); };
+    const int kp0 = zw.kprev(t.k0);
+    T prev = X3(col + plane * kp0, pc + cplane * (kp0 / L.rz));
+    T cur = X3(col + plane * t.k0, pc + cplane * (t.k0 / L.rz));
+#pragma unroll
+    for(int kk = 0; kk < KZ; ++kk) {
+      const int k = t.k0 + kk;
+      if(k < t.k1) {
+        const int kn = zw.knext(k);
+        const T next = X3(col + plane * kn, pc + cplane * (kn / L.rz));
+        const size_t pk = plane * k, cpk = cplane * (k / L.rz);
+        const T ax = lap_neg<T>(cur, sgn(x.sm, X3(oxm + pk, pxm + cpk)), sgn(x.sp, X3(oxp + pk, pxp + cpk)),
+                                sgn(y.sm, X3(oym + pk, pym + cpk)), sgn(y.sp, X3(oyp + pk, pyp + cpk)),
+                                zw.below(k, prev, cur), zw.above(k, next, cur), ihx2, ihy2, ihz2);
+        const T wc = wdiag<T, UNI>(L, cx, cy, bclass(k, L.nz));
+        const T out = cur + wc * ((b[col + pk] - mb) + T(-1) * ax);
+        x4[col + pk] = out;
+        v[0] += (double)out;
+        prev = cur;
+        cur = next;
+      }
     }
   }
   const int ops[1] = {kSum};
-  double res[1];
-  if(reduce_partials<1>(v, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0)
-    ps[lslot + L_MX4] = zm ? res[0] / (double)L.n : 0.0;
+  store_partials<1>(v, ops, partials);
 }
 
-// Second post-smoothing sweep: x5 = x4' + w/D (b - A x4'), x4' = x4 - m4. On the finest level
-// also r.z and beta (r = b - mb, z = x5 - m5; the sum of r is zero up to rounding).
-template <typename T>
-__global__ void k_smooth_last(const T* __restrict__ b, const T* __restrict__ x4, PLevel L, T* __restrict__ x5,
-                              double* partials, unsigned* counter, double* ps, int lslot, int zm, int finest, int kz) {
+// Second post-smoothing sweep; also sums of x5 and (b - mb) x5 (r.z on the finest level,
+// where b is the CG residual and z = x5 - m5; the sum of r is zero up to rounding).
+template <typename T, int KZ, bool UNI>
+__global__ void __launch_bounds__(256) k_smooth_last(const T* __restrict__ b, const T* __restrict__ x4, PLevel L,
+                                                     T* __restrict__ x5, double* partials, const double* ps,
+                                                     int lslot) {
   if(stopped(ps)) return;
   const T mb = T(ps[lslot + L_MB]);
   const T m4 = T(ps[lslot + L_MX4]);
   const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
-  const Tile t = tile(L, kz);
-  const Col C = column(L, t);
+  const HotTile t = hot_tile<KZ>(L);
   double v[2] = {0.0, 0.0};
-  if(C.in) {
+  if(t.in) {
+    const ZWin<UNI> zw(L);
     const size_t plane = (size_t)L.nx * L.ny;
-    auto X = [&](int ii, int jj, int kk) { return x4[(size_t)ii + (size_t)L.nx * jj + plane * kk] - m4; };
-    for(int k = t.k0; k < t.k1; ++k) {
-      const Nb z = nb_axis(k, L.nz, L.per[2], L.lo_dir[2], L.hi_dir[2]);
-      const T wc = T(L.winv[C.cx + 4 * (C.cy + 4 * bclass(k, L.nz))]);
-      const T c = X(t.i, t.j, k);
-      const T ax = lap_neg<T>(c, sgn(C.x.sm, X(C.x.cm, t.j, k)), sgn(C.x.sp, X(C.x.cp, t.j, k)),
-                              sgn(C.y.sm, X(t.i, C.y.cm, k)), sgn(C.y.sp, X(t.i, C.y.cp, k)),
-                              sgn(z.sm, X(t.i, t.j, z.cm)), sgn(z.sp, X(t.i, t.j, z.cp)), ihx2, ihy2, ihz2);
-      const size_t id = (size_t)t.i + (size_t)L.nx * t.j + plane * k;
-      const T bt = b[id] - mb;
-      const T xn = c + wc * (bt + T(-1) * ax);
-      x5[id] = xn;
-      v[0] += (double)xn;
-      v[1] += (double)bt * (double)xn;
+    const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+    const Nb x = nb_x(L, t.i), y = nb_y(L, t.j);
+    const int cx = bclass(t.i, L.nx), cy = bclass(t.j, L.ny);
+    const size_t oxm = (size_t)x.cm + (size_t)L.nx * t.j, oxp = (size_t)x.cp + (size_t)L.nx * t.j;
+    const size_t oym = (size_t)t.i + (size_t)L.nx * y.cm, oyp = (size_t)t.i + (size_t)L.nx * y.cp;
+    auto X = [&](size_t id) { return x4[id] - m4; };
+    T prev = X(col + plane * zw.kprev(t.k0));
+    T cur = X(col + plane * t.k0);
+#pragma unroll
+    for(int kk = 0; kk < KZ; ++kk) {
+      const int k = t.k0 + kk;
+      if(k < t.k1) {
+        const T next = X(col + plane * zw.knext(k));
+        const size_t pk = plane * k;
+        const T ax = lap_neg<T>(cur, sgn(x.sm, X(oxm + pk)), sgn(x.sp, X(oxp + pk)), sgn(y.sm, X(oym + pk)),
+                                sgn(y.sp, X(oyp + pk)), zw.below(k, prev, cur), zw.above(k, next, cur), ihx2, ihy2,
+                                ihz2);
+        const T wc = wdiag<T, UNI>(L, cx, cy, bclass(k, L.nz));
+        const T bt = b[col + pk] - mb;
+        const T out = cur + wc * (bt + T(-1) * ax);
+        x5[col + pk] = out;
+        v[0] += (double)out;
+        v[1] += (double)bt * (double)out;
+        prev = cur;
+        cur = next;
+      }
     }
   }
   const int ops[2] = {kSum, kSum};
-  double res[2];
-  if(reduce_partials<2>(v, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
-    const double m5 = zm ? res[0] / (double)L.n : 0.0;
-    ps[lslot + L_MX5] = m5;
-    if(finest) {
-      const double rz_new = res[1];
-      const double rz_old = ps[P_RZ];
-      ps[P_BETA] = rz_old != 0.0 ? rz_new / rz_old : 0.0;
-      ps[P_RZ] = rz_new;
-    }
-  }
+  store_partials<2>(v, ops, partials);
 }
 
-// Coarsest level: b - mb, Gaussian elimination with the pre-factored row operations of the
-// reference's partial-pivoting elimination (linsolve/poisson_solver.cpp:482-567), mean removal.
+// ---------------------------------------------------------------- one-workgroup tail
+// Levels l0..nl-1 of the V-cycle (the small ones) in a single workgroup: the same per-cell
+// formulas, block-wide sums for the deferred means, and the coarsest-level elimination with the
+// pre-factored row operations of the reference's partial-pivoting Gaussian elimination
+// (linsolve/poisson_solver.cpp:482-567). One launch replaces ~8 launches per level.
+constexpr int kMaxLevels = 24;
+constexpr int kTailThreads = 1024;
+constexpr int kTailCells = 4096;  // levels of at most this many cells run in the tail
+
 template <typename T>
-__global__ void k_coarse_solve(const T* __restrict__ b, T* __restrict__ x, int n, const int* __restrict__ perm,
-                               const double* __restrict__ factor, const double* __restrict__ U, double* work,
-                               double* ps, int lslot, int zm) {
+struct TailBufs {
+  T* b[kMaxLevels];
+  T* xa[kMaxLevels];
+  T* xb[kMaxLevels];
+};
+
+template <typename T>
+__device__ inline double tail_sum(double v) {
+  double a[1] = {v};
+  const int ops[1] = {kSum};
+  block_reduce<1>(a, ops);
+  return a[0];  // thread 0
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads)
+    k_vtail(const PLevel* __restrict__ lvs, int l0, int nl, TailBufs<T> B, const int* __restrict__ perm,
+            const double* __restrict__ factor, const double* __restrict__ U, double* __restrict__ work, double* ps,
+            int zm) {
   if(stopped(ps)) return;
-  if(threadIdx.x != 0) return;
-  const double mb = (double)T(ps[lslot + L_MB]);
-  for(int i = 0; i < n; ++i) work[i] = (double)(b[i] - T(mb));
-  if(zm) work[n - 1] = 0.0;
-  for(int p = 0; p < n; ++p) {
-    const int best = perm[p];
-    if(best != p) {
-      const double t = work[p];
-      work[p] = work[best];
-      work[best] = t;
+  __shared__ double lm[kMaxLevels][4];  // per level: mb, m2, m4, m5
+  __shared__ double lsbw[kMaxLevels];   // sum of raw b w/D
+  __shared__ double rz_sh;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int s0 = P_LEVEL + 8 * l0;
+  if(tid == 0) {
+    lm[l0][0] = ps[s0 + L_MB];
+    lsbw[l0] = ps[s0 + L_SBW];
+  }
+  __syncthreads();
+  // ---- down
+  for(int l = l0; l < nl - 1; ++l) {
+    const PLevel& L = lvs[l];
+    const PLevel& Cl = lvs[l + 1];
+    const T* b = B.b[l];
+    T* x2 = B.xa[l];
+    const T mb = T(lm[l][0]);
+    const T m1 = zm ? T((lsbw[l] - lm[l][0] * L.winv_sum) / (double)L.n) : T(0);
+    const int pxy = L.nx * L.ny;
+    double acc = 0.0;
+    for(int id = tid; id < L.n; id += nt) {
+      const int k = id / pxy, rem = id - k * pxy, j = rem / L.nx, i = rem - j * L.nx;
+      const T xv = smooth12_cell<T>(L, b, mb, m1, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
+      x2[id] = xv;
+      acc += (double)xv;
     }
-    for(int row = p + 1; row < n; ++row) {
-      const double f = factor[(size_t)row * n + p];
-      if(f != 0.0) work[row] -= f * work[p];
+    acc = tail_sum<T>(acc);
+    if(tid == 0) lm[l][1] = zm ? acc / (double)L.n : 0.0;
+    __syncthreads();
+    const T mx = T(lm[l][1]);
+    const T inv_children = T(1.0 / (double)(L.rx * L.ry * L.rz));
+    T* bc = B.b[l + 1];
+    const int cxy = Cl.nx * Cl.ny;
+    double s0c = 0.0, s1c = 0.0;
+    for(int id = tid; id < Cl.n; id += nt) {
+      const int kc = id / cxy, rem = id - kc * cxy, jc = rem / Cl.nx, ic = rem - jc * Cl.nx;
+      const int i0 = ic * L.rx, j0 = jc * L.ry, k0 = kc * L.rz;
+      auto R = [&](int i, int j, int k) {
+        return residual_cell<T>(L, b, x2, mb, mx, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
+      };
+      // same combination order as the tiled kernel: x pairs, then y pairs, then z pairs
+      auto row = [&](int j, int k) {
+        T r = R(i0, j, k);
+        if(L.rx == 2) r = r + R(i0 + 1, j, k);
+        return r;
+      };
+      auto pl = [&](int k) {
+        T r = row(j0, k);
+        if(L.ry == 2) r = r + row(j0 + 1, k);
+        return r;
+      };
+      T a = pl(k0);
+      if(L.rz == 2) a = a + pl(k0 + 1);
+      const T val = inv_children * a;
+      bc[id] = val;
+      s0c += (double)val;
+      s1c += (double)val * Cl.winv[bclass(ic, Cl.nx) + 4 * (bclass(jc, Cl.ny) + 4 * bclass(kc, Cl.nz))];
+    }
+    s0c = tail_sum<T>(s0c);
+    s1c = tail_sum<T>(s1c);
+    if(tid == 0) {
+      lm[l + 1][0] = zm ? s0c / (double)Cl.n : 0.0;
+      lsbw[l + 1] = s1c;
+    }
+    __syncthreads();
+  }
+  // ---- coarsest: elimination with the recorded row swaps and multipliers
+  {
+    const int lc = nl - 1;
+    const int n = lvs[lc].n;
+    const T* b = B.b[lc];
+    T* x = B.xa[lc];
+    const T mb = T(lm[lc][0]);
+    for(int i = tid; i < n; i += nt) work[i] = (double)(b[i] - mb);
+    __syncthreads();
+    if(zm && tid == 0) work[n - 1] = 0.0;
+    __syncthreads();
+    for(int p = 0; p < n; ++p) {
+      if(tid == 0) {
+        const int best = perm[p];
+        if(best != p) {
+          const double t = work[p];
+          work[p] = work[best];
+          work[best] = t;
+        }
+      }
+      __syncthreads();
+      const double wp = work[p];
+      for(int row = p + 1 + tid; row < n; row += nt) {
+        const double f = factor[(size_t)row * n + p];
+        if(f != 0.0) work[row] -= f * wp;
+      }
+      __syncthreads();
+    }
+    if(tid == 0) {
+      for(int row = n - 1; row >= 0; --row) {
+        double s = work[row];
+        for(int c = row + 1; c < n; ++c) s -= U[(size_t)row * n + c] * work[c];
+        work[row] = s / U[(size_t)row * n + row];
+      }
+      if(zm) {
+        double s = 0.0;
+        for(int i = 0; i < n; ++i) s += work[i];
+        const double m = s / (double)n;
+        for(int i = 0; i < n; ++i) work[i] -= m;
+      }
+      lm[lc][3] = 0.0;
+    }
+    __syncthreads();
+    double rz = 0.0;
+    for(int i = tid; i < n; i += nt) {
+      const T xv = T(work[i]);
+      x[i] = xv;
+      rz += (double)(b[i] - mb) * (double)xv;
+    }
+    if(lc == 0) {  // the finest level is the coarsest: r.z of the direct solve
+      rz = tail_sum<T>(rz);
+      if(tid == 0) rz_sh = rz;
+    }
+    __syncthreads();
+  }
+  // ---- up
+  for(int l = nl - 2; l >= l0; --l) {
+    const PLevel& L = lvs[l];
+    const PLevel& Cl = lvs[l + 1];
+    const T* b = B.b[l];
+    const T mb = T(lm[l][0]), m2 = T(lm[l][1]), mc = T(lm[l + 1][3]);
+    const int pxy = L.nx * L.ny;
+    double acc = 0.0;
+    for(int id = tid; id < L.n; id += nt) {
+      const int k = id / pxy, rem = id - k * pxy, j = rem / L.nx, i = rem - j * L.nx;
+      const T xv =
+          prolong_smooth_cell<T>(L, Cl, b, B.xa[l], B.xa[l + 1], mb, m2, mc, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
+      B.xb[l][id] = xv;
+      acc += (double)xv;
+    }
+    acc = tail_sum<T>(acc);
+    if(tid == 0) lm[l][2] = zm ? acc / (double)L.n : 0.0;
+    __syncthreads();
+    const T m4 = T(lm[l][2]);
+    double s5 = 0.0, srz = 0.0;
+    for(int id = tid; id < L.n; id += nt) {
+      const int k = id / pxy, rem = id - k * pxy, j = rem / L.nx, i = rem - j * L.nx;
+      T bt;
+      const T xv = smooth_last_cell<T>(L, b, B.xb[l], mb, m4, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k), bt);
+      B.xa[l][id] = xv;
+      s5 += (double)xv;
+      srz += (double)bt * (double)xv;
+    }
+    s5 = tail_sum<T>(s5);
+    srz = tail_sum<T>(srz);
+    if(tid == 0) {
+      lm[l][3] = zm ? s5 / (double)L.n : 0.0;
+      if(l == 0) rz_sh = srz;
+    }
+    __syncthreads();
+  }
+  if(tid == 0) {
+    ps[s0 + L_MX5] = lm[l0][3];
+    if(l0 == 0) {
+      const double rz_old = ps[P_RZ];
+      ps[P_BETA] = rz_old != 0.0 ? rz_sh / rz_old : 0.0;
+      ps[P_RZ] = rz_sh;
     }
   }
-  for(int row = n - 1; row >= 0; --row) {
-    double s = work[row];
-    for(int c = row + 1; c < n; ++c) s -= U[(size_t)row * n + c] * work[c];
-    work[row] = s / U[(size_t)row * n + row];
-  }
-  if(zm) {
-    double s = 0.0;
-    for(int i = 0; i < n; ++i) s += work[i];
-    const double m = s / (double)n;
-    for(int i = 0; i < n; ++i) work[i] -= m;
-  }
-  for(int i = 0; i < n; ++i) x[i] = T(work[i]);
-  ps[lslot + L_MX5] = 0.0;
 }
 
 // ---------------------------------------------------------------- PCG vector kernels
-// Fused: deferred p += alpha_prev (d_old - md_old); d = (z - mz) + beta (d_old - md_old)
-// (first: d = z - mz); q = A d; sums of d, q, d.q; alpha = rz / dq or breakdown.
+// Search direction (pointwise): d = (z - mz) + beta (d_old - md) (first: d = z - mz), and the
+// deferred p += alpha_prev (d_old - md) of the previous residual update.
+constexpr int kPointIlp = 4;
 template <typename T>
-__global__ void k_dir_apply(const T* __restrict__ z, const double* __restrict__ d_old, double* __restrict__ d_new,
-                            double* __restrict__ q, double* __restrict__ p, PLevel L, double* partials,
-                            unsigned* counter, double* ps, int first, int zm, int dsel, int kz) {
+__global__ void __launch_bounds__(256) k_direction(const T* __restrict__ z, const double* __restrict__ d_old,
+                                                   double* __restrict__ d_new, double* __restrict__ p, int n,
+                                                   const double* ps, int first) {
   if(stopped(ps)) return;
   const double mz = ps[P_LEVEL + L_MX5];
   const double md = ps[P_MD];
   const double beta = first ? 0.0 : ps[P_BETA];
   const double alpha_prev = ps[P_ALPHA];
   const int pend = !first && ps[P_PEND] != 0.0;
-  const Tile t = tile(L, kz);
-  const Col C = column(L, t);
-  double v[3] = {0.0, 0.0, 0.0};
-  if(C.in) {
-    const size_t plane = (size_t)L.nx * L.ny;
-    auto D = [&](int ii, int jj, int kk) {
-      const size_t id = (size_t)ii + (size_t)L.nx * jj + plane * kk;
-      const double zt = (double)z[id] - mz;
-      return first ? zt : zt + beta * (d_old[id] - md);
-    };
-    for(int k = t.k0; k < t.k1; ++k) {
-      const Nb zz = nb_axis(k, L.nz, L.per[2], L.lo_dir[2], L.hi_dir[2]);
-      const size_t id = (size_t)t.i + (size_t)L.nx * t.j + plane * k;
-      const double c = D(t.i, t.j, k);
-      const double qv = lap_neg<double>(c, sgn(C.x.sm, D(C.x.cm, t.j, k)), sgn(C.x.sp, D(C.x.cp, t.j, k)),
-                                        sgn(C.y.sm, D(t.i, C.y.cm, k)), sgn(C.y.sp, D(t.i, C.y.cp, k)),
-                                        sgn(zz.sm, D(t.i, t.j, zz.cm)), sgn(zz.sp, D(t.i, t.j, zz.cp)), L.ihx2, L.ihy2,
-                                        L.ihz2);
-      if(pend) p[id] = p[id] + alpha_prev * (d_old[id] - md);
-      d_new[id] = c;
-      q[id] = qv;
-      v[0] += c;
-      v[1] += qv;
-      v[2] += c * qv;
+  const int stride = gridDim.x * 256 * kPointIlp;
+  for(int base = blockIdx.x * 256 * kPointIlp + threadIdx.x; base < n; base += stride) {
+    double zt[kPointIlp], dv[kPointIlp], pv[kPointIlp];
+#pragma unroll
+    for(int e = 0; e < kPointIlp; ++e) {
+      const int id = base + e * 256;
+      if(id < n) {
+        zt[e] = (double)z[id] - mz;
+        dv[e] = first ? 0.0 : d_old[id];
+        pv[e] = pend ? p[id] : 0.0;
+      }
     }
-  }
-  const int ops[3] = {kSum, kSum, kSum};
-  double res[3];
-  if(reduce_partials<3>(v, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
-    const double n = (double)L.n;
-    const double mdn = zm ? res[0] / n : 0.0;
-    const double mq = zm ? res[1] / n : 0.0;
-    ps[P_MD] = mdn;
-    ps[P_MQ] = mq;
-    ps[P_PEND] = 0.0;
-    ps[P_DSEL] = (double)dsel;
-    const double dq = res[2] - mdn * res[1];
-    ps[P_DQ] = dq;
-    if(fabs(dq) < kSmallNumber) {
-      ps[P_DONE] = 1.0;  // breakdown: stop without convergence (linsolve/poisson_solver.cpp:756-759)
-    } else {
-      ps[P_ALPHA] = ps[P_RZ] / dq;
+#pragma unroll
+    for(int e = 0; e < kPointIlp; ++e) {
+      const int id = base + e * 256;
+      if(id < n) {
+        d_new[id] = first ? zt[e] : zt[e] + beta * (dv[e] - md);
+        if(pend) p[id] = pv[e] + alpha_prev * (dv[e] - md);
+      }
     }
   }
 }
 
-// r = (r - mr) - alpha (q - mq); residual RMS, convergence test; the matching p update is
-// deferred to the next direction kernel (or the final update).
-template <typename T>
-__global__ void k_update(double* __restrict__ r, const double* __restrict__ q, T* __restrict__ b_lo, PLevel L,
-                         double* partials, unsigned* counter, double* ps, double* history, double tol, int it, int zm,
-                         int write_lo, int kz) {
+// q = A d; sums of d, q, d.q.
+template <int KZ, bool UNI>
+__global__ void __launch_bounds__(256) k_apply_dot(const double* __restrict__ d, double* __restrict__ q, PLevel L,
+                                                   double* partials, const double* ps) {
   if(stopped(ps)) return;
-  const double alpha = ps[P_ALPHA], mr = ps[P_MR], mq = ps[P_MQ];
-  const Tile t = tile(L, kz);
+  const HotTile t = hot_tile<KZ>(L);
   double v[3] = {0.0, 0.0, 0.0};
-  if(t.i < L.nx && t.j < L.ny) {
-    const int cxy = bclass(t.i, L.nx) + 4 * bclass(t.j, L.ny);
+  if(t.in) {
+    const ZWin<UNI> zw(L);
     const size_t plane = (size_t)L.nx * L.ny;
-    for(int k = t.k0; k < t.k1; ++k) {
-      const size_t id = (size_t)t.i + (size_t)L.nx * t.j + plane * k;
-      const double rn = (r[id] - mr) + -alpha * (q[id] - mq);
-      r[id] = rn;
-      if(write_lo) b_lo[id] = T(rn);
-      v[0] += rn;
-      v[1] += rn * rn;
-      v[2] += rn * L.winv[cxy + 16 * bclass(k, L.nz)];
+    const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+    const Nb x = nb_x(L, t.i), y = nb_y(L, t.j);
+    const size_t oxm = (size_t)x.cm + (size_t)L.nx * t.j, oxp = (size_t)x.cp + (size_t)L.nx * t.j;
+    const size_t oym = (size_t)t.i + (size_t)L.nx * y.cm, oyp = (size_t)t.i + (size_t)L.nx * y.cp;
+    double prev = d[col + plane * zw.kprev(t.k0)];
+    double cur = d[col + plane * t.k0];
+#pragma unroll
+    for(int kk = 0; kk < KZ; ++kk) {
+      const int k = t.k0 + kk;
+      if(k < t.k1) {
+        const double next = d[col + plane * zw.knext(k)];
+        const size_t pk = plane * k;
+        const double qv = lap_neg<double>(cur, sgn(x.sm, d[oxm + pk]), sgn(x.sp, d[oxp + pk]), sgn(y.sm, d[oym + pk]),
+                                          sgn(y.sp, d[oyp + pk]), zw.below(k, prev, cur), zw.above(k, next, cur),
+                                          L.ihx2, L.ihy2, L.ihz2);
+        q[col + pk] = qv;
+        v[0] += cur;
+        v[1] += qv;
+        v[2] += cur * qv;
+        prev = cur;
+        cur = next;
+      }
     }
   }
   const int ops[3] = {kSum, kSum, kSum};
-  double res[3];
-  if(reduce_partials<3>(v, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
-    const double n = (double)L.n;
-    const double m = zm ? res[0] / n : 0.0;
-    ps[P_MR] = m;
-    ps[P_LEVEL + L_MB] = m;
-    ps[P_LEVEL + L_SBW] = res[2];
-    const double rms = sqrt(fmax(res[1] - n * m * m, 0.0) / n);
-    ps[P_ITER] = (double)(it + 1);
-    ps[P_RMS] = rms;
-    ps[P_PEND] = 1.0;
-    const double rel = ps[P_R0] > 0.0 ? rms / ps[P_R0] : 0.0;
-    ps[P_REL] = rel;
-    history[it + 1] = rms;
-    if(rel <= tol) {
-      ps[P_CONV] = 1.0;
-      ps[P_DONE] = 1.0;
+  store_partials<3>(v, ops, partials);
+}
+
+// r = (r - mr) - alpha (q - mq); sums of r, r^2, r w/D; the matching p update is deferred to
+// the next direction kernel (or the final update).
+template <typename T, int KZ, bool UNI>
+__global__ void __launch_bounds__(256) k_update(double* __restrict__ r, const double* __restrict__ q,
+                                                T* __restrict__ b_lo, PLevel L, double* partials, const double* ps,
+                                                int write_lo) {
+  if(stopped(ps)) return;
+  const double alpha = ps[P_ALPHA], mr = ps[P_MR], mq = ps[P_MQ];
+  const HotTile t = hot_tile<KZ>(L);
+  double v[3] = {0.0, 0.0, 0.0};
+  if(t.in) {
+    const int cxy = bclass(t.i, L.nx) + 4 * bclass(t.j, L.ny);
+    const size_t plane = (size_t)L.nx * L.ny;
+    const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+#pragma unroll
+    for(int kk = 0; kk < KZ; ++kk) {
+      const int k = t.k0 + kk;
+      if(k < t.k1) {
+        const size_t id = col + plane * k;
+        const double rn = (r[id] - mr) + -alpha * (q[id] - mq);
+        r[id] = rn;
+        if(write_lo) b_lo[id] = T(rn);
+        v[0] += rn;
+        v[1] += rn * rn;
+        v[2] += rn * (UNI ? L.winv[0] : L.winv[cxy + 16 * bclass(k, L.nz)]);
+      }
     }
   }
+  const int ops[3] = {kSum, kSum, kSum};
+  store_partials<3>(v, ops, partials);
 }
 
 template <typename T>
@@ -586,9 +1037,9 @@ __global__ void k_cast(const double* __restrict__ in, T* __restrict__ out, int n
   for(int id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x) out[id] = T(in[id]);
 }
 
-// Pending pressure update of the last iteration, then the mean of p.
+// Pending pressure update of the last iteration, then the sum of p.
 __global__ void k_pfinal(double* __restrict__ pc, const double* __restrict__ d0, const double* __restrict__ d1, int n,
-                         double* partials, unsigned* counter, double* ps, int zm) {
+                         double* partials, const double* ps) {
   const int pend = ps[P_PEND] != 0.0;
   const double* d = ps[P_DSEL] != 0.0 ? d1 : d0;
   const double alpha = ps[P_ALPHA], md = ps[P_MD];
@@ -602,11 +1053,7 @@ __global__ void k_pfinal(double* __restrict__ pc, const double* __restrict__ d0,
     v[0] += p;
   }
   const int ops[1] = {kSum};
-  double res[1];
-  if(reduce_partials<1>(v, ops, partials, counter, res) && threadIdx.x == 0) {
-    ps[P_PFINAL] = zm ? res[0] / (double)n : 0.0;
-    ps[P_PEND] = 0.0;
-  }
+  store_partials<1>(v, ops, partials);
 }
 
 __global__ void k_pout(const double* __restrict__ pc, PLevel L, Lay Lc, double* __restrict__ p_g, const double* ps,
@@ -666,6 +1113,40 @@ LevelLaunch level_launch(const PLevel& L) {
   return l;
 }
 
+// Launch shape of the hot level kernels: 1D grid of 64x4 tiles, KZ = 8 planes per thread when
+// that still gives >= 1024 tiles, else 2.
+struct HotLaunch {
+  dim3 grid, block;
+  int kz;
+  bool uni;
+};
+HotLaunch hot_launch(const PLevel& L) {
+  HotLaunch h;
+  h.block = dim3(64, 4, 1);
+  const size_t plane = (size_t)((L.nx + 63) / 64) * (size_t)((L.ny + 3) / 4);
+  h.kz = plane * (size_t)((L.nz + 7) / 8) >= 1024 ? 8 : 2;
+  h.grid = dim3((unsigned)(plane * (size_t)((L.nz + h.kz - 1) / h.kz)));
+  h.uni = L.per[0] && L.per[1] && L.per[2];
+  if(h.grid.x > (unsigned)kMaxPartials) throw InvalidArgument("grid too large for the reduction buffer");
+  return h;
+}
+
+// Instantiates KERN<pre..., KZ, UNI> for the launch shape h.
+#define OFH_HOT_LAUNCH(KERN, h, stream, ...)                                                  \
+  do {                                                                                        \
+    if((h).kz == 8) {                                                                         \
+      if((h).uni)                                                                             \
+        hipLaunchKernelGGL((KERN, 8, true>), (h).grid, (h).block, 0, stream, __VA_ARGS__);    \
+      else                                                                                    \
+        hipLaunchKernelGGL((KERN, 8, false>), (h).grid, (h).block, 0, stream, __VA_ARGS__);   \
+    } else {                                                                                  \
+      if((h).uni)                                                                             \
+        hipLaunchKernelGGL((KERN, 2, true>), (h).grid, (h).block, 0, stream, __VA_ARGS__);    \
+      else                                                                                    \
+        hipLaunchKernelGGL((KERN, 2, false>), (h).grid, (h).block, 0, stream, __VA_ARGS__);   \
+    }                                                                                         \
+  } while(0)
+
 }  // namespace
 
 // ------------------------------------------------------------------ solver object
@@ -683,8 +1164,11 @@ class PoissonSolver {
   std::vector<void*> lb, lxa, lxb;
   double* ps = nullptr;       // scalar block
   double* history = nullptr;  // residual history (device)
-  double* host_ps = nullptr;  // pinned copies (2 slots)
+  double* host_ps = nullptr;  // mapped pinned mirror of the leading scalars (2 slots)
+  double* host_ps_dev = nullptr;
   hipEvent_t ev[2]{};
+  PLevel* dlv = nullptr;      // level descriptors for the tail kernel
+  int l0 = 0;                 // first level handled by the one-workgroup tail
   // coarse solve data
   int nc = 0;
   int* perm = nullptr;
@@ -848,6 +1332,7 @@ PoissonSolver::~PoissonSolver() {
   fr(factor);
   fr(U);
   fr(work);
+  fr(dlv);
   if(host_ps) (void)hipHostFree(host_ps);
   for(auto& e : ev)
     if(e) (void)hipEventDestroy(e);
@@ -877,7 +1362,13 @@ void PoissonSolver::build() {
     dz = hz;
     lv.push_back(make_level(nx, ny, nz, dx, dy, dz, bcs));
   }
-  if((int)lv.size() > 24) throw InvalidArgument("multigrid hierarchy deeper than supported");
+  if((int)lv.size() > kMaxLevels) throw InvalidArgument("multigrid hierarchy deeper than supported");
+  l0 = (int)lv.size() - 1;
+  for(int l = 0; l < (int)lv.size(); ++l)
+    if(lv[l].n <= kTailCells) {
+      l0 = l;
+      break;
+    }
   const size_t n0 = (size_t)lv[0].n;
   OFH_CHECK(hipMalloc(&pc, n0 * sizeof(double)));
   OFH_CHECK(hipMalloc(&r, n0 * sizeof(double)));
@@ -898,7 +1389,11 @@ void PoissonSolver::build() {
   }
   OFH_CHECK(hipMalloc(&ps, P_COUNT * sizeof(double)));
   OFH_CHECK(hipMemsetAsync(ps, 0, P_COUNT * sizeof(double), c.stream));
-  OFH_CHECK(hipHostMalloc(&host_ps, 2 * P_COUNT * sizeof(double), hipHostMallocDefault));
+  OFH_CHECK(hipHostMalloc(&host_ps, 2 * kHostMirror * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+  OFH_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&host_ps_dev), host_ps, 0));
+  for(int q = 0; q < 2 * kHostMirror; ++q) host_ps[q] = 0.0;
+  OFH_CHECK(hipMalloc(&dlv, lv.size() * sizeof(PLevel)));
+  OFH_CHECK(hipMemcpy(dlv, lv.data(), lv.size() * sizeof(PLevel), hipMemcpyHostToDevice));
   for(auto& e : ev) OFH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   std::vector<int> hperm;
   std::vector<double> hf, hU;
@@ -915,48 +1410,58 @@ void PoissonSolver::build() {
 
 template <typename T>
 void PoissonSolver::vcycle() {
-  const int L = (int)lv.size();
+  const int nl = (int)lv.size();
   const int zmi = zm ? 1 : 0;
-  unsigned* cnt = c.counters + 8;
   auto slot = [](int l) { return P_LEVEL + 8 * l; };
   // finest rhs: CG residual (FP64) or its FP32 copy (mixed)
-  auto bptr = [&](int l) -> const T* {
-    if(l == 0 && precision == OF_PRECISION_FP64) return reinterpret_cast<const T*>(r);
-    return static_cast<const T*>(lb[l]);
+  auto bptr = [&](int l) -> T* {
+    if(l == 0 && precision == OF_PRECISION_FP64) return reinterpret_cast<T*>(r);
+    return static_cast<T*>(lb[l]);
   };
-  for(int l = 0; l < L - 1; ++l) {
+  for(int l = 0; l < l0; ++l) {
     const PLevel& P = lv[l];
     const PLevel& C = lv[l + 1];
-    const LevelLaunch ll = level_launch(P);
+    const HotLaunch h = hot_launch(P);
     {
       ProfScope prof_(c, "k_smooth12", l == 0);
-      hipLaunchKernelGGL(k_smooth12<T>, ll.grid, ll.block, 0, c.stream, bptr(l), static_cast<T*>(lxa[l]), P,
-                         c.partials, cnt, ps, slot(l), zmi, ll.kz);
+      OFH_HOT_LAUNCH(k_smooth12<T, h, c.stream, bptr(l), static_cast<T*>(lxa[l]), P, c.partials, ps, slot(l), zmi);
     }
+    launch_finalize(c.stream, c.partials, h.grid, EpiMean{ps, slot(l) + L_MX2, (double)P.n, zmi});
     {
       ProfScope prof_(c, "k_residual_restrict", l == 0);
-      hipLaunchKernelGGL(k_residual_restrict<T>, ll.grid, ll.block, 0, c.stream, bptr(l),
-                         static_cast<const T*>(lxa[l]), P, C, static_cast<T*>(lb[l + 1]), c.partials, cnt, ps,
-                         slot(l), slot(l + 1), zmi, ll.kz);
+      OFH_HOT_LAUNCH(k_residual_restrict<T, h, c.stream, bptr(l), static_cast<const T*>(lxa[l]), P, C,
+                     static_cast<T*>(lb[l + 1]), c.partials, ps, slot(l), zmi);
     }
+    launch_finalize(c.stream, c.partials, h.grid, EpiRestrict{ps, slot(l + 1), (double)C.n, zmi});
   }
-  hipLaunchKernelGGL(k_coarse_solve<T>, 1, 64, 0, c.stream, static_cast<const T*>(lb[L - 1]),
-                     static_cast<T*>(lxa[L - 1]), nc, perm, factor, U, work, ps, slot(L - 1), zmi);
-  for(int l = L - 2; l >= 0; --l) {
+  {
+    TailBufs<T> tb{};
+    for(int l = l0; l < nl; ++l) {
+      tb.b[l] = bptr(l);
+      tb.xa[l] = static_cast<T*>(lxa[l]);
+      tb.xb[l] = static_cast<T*>(lxb[l]);
+    }
+    ProfScope prof_(c, "k_vtail");
+    hipLaunchKernelGGL(k_vtail<T>, dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm, factor, U, work,
+                       ps, zmi);
+  }
+  for(int l = l0 - 1; l >= 0; --l) {
     const PLevel& P = lv[l];
     const PLevel& C = lv[l + 1];
-    const LevelLaunch ll = level_launch(P);
+    const HotLaunch h = hot_launch(P);
     {
       ProfScope prof_(c, "k_prolong_smooth", l == 0);
-      hipLaunchKernelGGL(k_prolong_smooth<T>, ll.grid, ll.block, 0, c.stream, bptr(l),
-                         static_cast<const T*>(lxa[l]), static_cast<const T*>(lxa[l + 1]), P, C,
-                         static_cast<T*>(lxb[l]), c.partials, cnt, ps, slot(l), slot(l + 1), zmi, ll.kz);
+      OFH_HOT_LAUNCH(k_prolong_smooth<T, h, c.stream, bptr(l), static_cast<const T*>(lxa[l]),
+                     static_cast<const T*>(lxa[l + 1]), P, C, static_cast<T*>(lxb[l]), c.partials, ps, slot(l),
+                     slot(l + 1));
     }
+    launch_finalize(c.stream, c.partials, h.grid, EpiMean{ps, slot(l) + L_MX4, (double)P.n, zmi});
     {
       ProfScope prof_(c, "k_smooth_last", l == 0);
-      hipLaunchKernelGGL(k_smooth_last<T>, ll.grid, ll.block, 0, c.stream, bptr(l), static_cast<const T*>(lxb[l]),
-                         P, static_cast<T*>(lxa[l]), c.partials, cnt, ps, slot(l), zmi, l == 0 ? 1 : 0, ll.kz);
+      OFH_HOT_LAUNCH(k_smooth_last<T, h, c.stream, bptr(l), static_cast<const T*>(lxb[l]), P,
+                     static_cast<T*>(lxa[l]), c.partials, ps, slot(l));
     }
+    launch_finalize(c.stream, c.partials, h.grid, EpiSmoothLast{ps, slot(l), (double)P.n, zmi, l == 0 ? 1 : 0});
   }
 }
 
@@ -970,38 +1475,60 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   const PLevel& F = lv[0];
   const Lay Lc = make_lay(g, OF_CELL);
   const int zmi = zm ? 1 : 0;
-  unsigned* cnt = c.counters + 8;
   const LevelLaunch ll = level_launch(F);
   const bool mixed = precision == OF_PRECISION_MIXED;
+  const double n0 = (double)F.n;
+  const HotLaunch hf = hot_launch(F);
+  const unsigned pgrid = std::min<unsigned>(div_up((size_t)F.n, 256 * kPointIlp), 8192u);
   OFH_CHECK(hipMemsetAsync(ps, 0, P_COUNT * sizeof(double), c.stream));
-  hipLaunchKernelGGL(k_setup1, ll.grid, ll.block, 0, c.stream, rhs, p, Lc, F, r, pc, c.partials, cnt, ps, zmi, ll.kz);
-  hipLaunchKernelGGL(k_setup2, ll.grid, ll.block, 0, c.stream, r, pc, F, c.partials, cnt, ps, history, tol, zmi,
-                     ll.kz);
+  hipLaunchKernelGGL(k_setup1, ll.grid, ll.block, 0, c.stream, rhs, p, Lc, F, r, pc, c.partials, ll.kz);
+  launch_finalize(c.stream, c.partials, ll.grid, EpiSetup1{ps, n0, zmi});
+  hipLaunchKernelGGL(k_setup2, ll.grid, ll.block, 0, c.stream, r, pc, F, c.partials, ps, ll.kz);
+  launch_finalize(c.stream, c.partials, ll.grid, EpiSetup2{ps, history, n0, tol, zmi});
   if(mixed)  // FP32 copy of the initial residual: the rhs of the finest FP32 level
     hipLaunchKernelGGL(k_cast<T>, 2048, 256, 0, c.stream, r, static_cast<T*>(lb[0]), F.n, ps);
   vcycle<T>();
   int it = 0;
   for(; it < maxit; ++it) {
     {
-      ProfScope prof_(c, "k_dir_apply");
-      hipLaunchKernelGGL(k_dir_apply<T>, ll.grid, ll.block, 0, c.stream, static_cast<const T*>(lxa[0]),
-                         d[(it + 1) % 2], d[it % 2], q, pc, F, c.partials, cnt, ps, it == 0 ? 1 : 0, zmi, it % 2,
-                         ll.kz);
+      ProfScope prof_(c, "k_direction");
+      hipLaunchKernelGGL(k_direction<T>, dim3(pgrid), dim3(256), 0, c.stream, static_cast<const T*>(lxa[0]),
+                         d[(it + 1) % 2], d[it % 2], pc, F.n, ps, it == 0 ? 1 : 0);
     }
     {
-      ProfScope prof_(c, "k_update");
-      hipLaunchKernelGGL(k_update<T>, ll.grid, ll.block, 0, c.stream, r, q, mixed ? static_cast<T*>(lb[0]) : nullptr,
-                         F, c.partials, cnt, ps, history, tol, it, zmi, mixed ? 1 : 0, ll.kz);
+      ProfScope prof_(c, "k_apply_dot");
+      if(hf.kz == 8) {
+        if(hf.uni)
+          hipLaunchKernelGGL((k_apply_dot<8, true>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+        else
+          hipLaunchKernelGGL((k_apply_dot<8, false>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+      } else {
+        if(hf.uni)
+          hipLaunchKernelGGL((k_apply_dot<2, true>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+        else
+          hipLaunchKernelGGL((k_apply_dot<2, false>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+      }
     }
-    OFH_CHECK(hipMemcpyAsync(host_ps + (it % 2) * P_COUNT, ps, 8 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    launch_finalize(c.stream, c.partials, hf.grid, EpiDir{ps, n0, zmi, it % 2});
+    {
+      ProfScope prof_(c, "k_update");
+      OFH_HOT_LAUNCH(k_update<T, hf, c.stream, r, q, mixed ? static_cast<T*>(lb[0]) : nullptr, F, c.partials, ps,
+                     mixed ? 1 : 0);
+    }
+    launch_finalize(c.stream, c.partials, hf.grid,
+                    EpiUpdate{ps, history, host_ps_dev + (it % 2) * kHostMirror, n0, tol, it, zmi});
     OFH_CHECK(hipEventRecord(ev[it % 2], c.stream));
     if(it > 0) {
       OFH_CHECK(hipEventSynchronize(ev[(it - 1) % 2]));
-      if(host_ps[((it - 1) % 2) * P_COUNT + P_DONE] != 0.0) break;
+      if(reinterpret_cast<volatile double*>(host_ps)[((it - 1) % 2) * kHostMirror + P_DONE] != 0.0) break;
     }
     vcycle<T>();
   }
-  hipLaunchKernelGGL(k_pfinal, 2048, 256, 0, c.stream, pc, d[0], d[1], F.n, c.partials, cnt, ps, zmi);
+  {
+    const dim3 pg(2048);
+    hipLaunchKernelGGL(k_pfinal, pg, dim3(256), 0, c.stream, pc, d[0], d[1], F.n, c.partials, ps);
+    launch_finalize(c.stream, c.partials, pg, EpiPfinal{ps, n0, zmi});
+  }
   hipLaunchKernelGGL(k_pout, ll.grid, ll.block, 0, c.stream, pc, F, Lc, p, ps, ll.kz);
   OFH_CHECK(hipGetLastError());
   std::vector<double> hps(P_COUNT);

@@ -21,7 +21,7 @@ namespace {
 __global__ void k_step_diag(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
                             const double* __restrict__ uo, const double* __restrict__ vo,
                             const double* __restrict__ wo, Lay Lu, Lay Lv, Lay Lw, of_grid g, double dt,
-                            double* partials, unsigned* counter, double* out, int kz) {
+                            double* partials, int kz) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   // 0 sum div^2, 1 max|div|, 2 max cfl, 3 max|uc|, 4 max|vc|, 5 max|wc|, 6 max|du|, 7 non-finite
@@ -54,11 +54,9 @@ __global__ void k_step_diag(const double* __restrict__ u, const double* __restri
     }
   }
   const int ops[8] = {kSum, kMax, kMax, kMax, kMax, kMax, kMax, kSum};
-  double res[8];
-  if(reduce_partials<8>(vals, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) {
-    for(int q = 0; q < 8; ++q) out[q] = res[q];
-  }
+  store_partials<8>(vals, ops, partials);
 }
+constexpr unsigned kStepDiagMaxMask = 0x7eu;  // values 1..6 are maxima
 
 // p_total += phi; p_total += (-nu dt / 2) rhs   (solver/taylor_green.cpp:607-608)
 __global__ void k_pressure_update(double* __restrict__ pt, const double* __restrict__ phi,
@@ -107,7 +105,7 @@ __global__ void k_tg_init(double* f, Lay L, int loc, of_grid g, int three_d, int
 
 // kinetic energy sum (solver/taylor_green.cpp:269-288)
 __global__ void k_kinetic(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
-                          Lay Lu, Lay Lv, Lay Lw, of_grid g, double* partials, unsigned* counter, double* out, int kz) {
+                          Lay Lu, Lay Lv, Lay Lw, of_grid g, double* partials, int kz) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   double vals[1] = {0.0};
@@ -121,8 +119,7 @@ __global__ void k_kinetic(const double* __restrict__ u, const double* __restrict
     }
   }
   const int ops[1] = {kSum};
-  double res[1];
-  if(reduce_partials<1>(vals, ops, partials, counter, res) && threadIdx.x == 0 && threadIdx.y == 0) out[0] = res[0];
+  store_partials<1>(vals, ops, partials);
 }
 
 }  // namespace
@@ -281,8 +278,9 @@ void Stepper::step() {
       ProfScope prof_(c, "k_step_diag", true);
       hipLaunchKernelGGL(k_step_diag, cl.grid, cl.block, 0, c.stream, corr[0], corr[1], corr[2], vel[0], vel[1], vel[2],
                          make_lay(g, OF_FACE_X), make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), g, dt, c.partials,
-                         c.counters + 2, diag, cl.kz);
+                         cl.kz);
     }
+    launch_finalize(c.stream, c.partials, cl.grid, EpiStore<8>{diag, kStepDiagMaxMask});
   }
   OFH_CHECK(hipGetLastError());
   double h[8];
@@ -312,7 +310,8 @@ void Stepper::step() {
 double Stepper::kinetic_energy() {
   const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz);
   hipLaunchKernelGGL(k_kinetic, cl.grid, cl.block, 0, c.stream, vel[0], vel[1], vel[2], make_lay(g, OF_FACE_X),
-                     make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), g, c.partials, c.counters + 3, diag + 16, cl.kz);
+                     make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), g, c.partials, cl.kz);
+  launch_finalize(c.stream, c.partials, cl.grid, EpiStore<1>{diag + 16, 0u});
   double e = 0.0;
   OFH_CHECK(hipMemcpyAsync(&e, diag + 16, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   OFH_CHECK(hipStreamSynchronize(c.stream));
