@@ -113,6 +113,211 @@ __global__ void k_line_solve(double* f, Lay L, int axis, LineSys S, const double
   if(S.copy_dup) base[S.e * st] = base[S.b * st];
 }
 
+// x-lines (unit stride along the line): one wave owns 64 lines; the line data moves through
+// LDS in chunks of 64 lines x kLX elements so that every global access is a coalesced row
+// segment (loads issued in batches of 16), and each lane then runs the recurrence of its own
+// line out of LDS. Same arithmetic as k_line_solve.
+constexpr int kLT = 64;
+constexpr int kLX = 32;
+__global__ void __launch_bounds__(kLT) k_line_solve_x(double* f, Lay L, LineSys S, const double* __restrict__ piv,
+                                                      const double* __restrict__ mu, const double* __restrict__ z) {
+  __shared__ double T[kLT][kLX + 1];
+  const int lane = threadIdx.x;
+  const int nl1 = L.ay, nlines = L.ay * L.az;
+  const int line0 = blockIdx.x * kLT;
+  const int nr = min(kLT, nlines - line0);
+  const int n = S.n;
+  const bool mine = lane < nr;
+  // storage offset of this lane's line (32-bit: storage sizes stay below 2^31 elements)
+  const int my_off = mine ? (int)L.at(0, 1 + (line0 + lane) % nl1, 1 + (line0 + lane) / nl1) : 0;
+  if(S.cyclic && n == 1) {
+    if(mine) {
+      f[my_off + S.b] = f[my_off + S.b] / S.single_pivot;
+      if(S.copy_dup) f[my_off + S.e] = f[my_off + S.b];
+    }
+    return;
+  }
+  const int col = lane % kLX, half = lane / kLX;  // two lanes per row segment of kLX elements
+  auto load = [&](int c0, int cn) {
+    for(int r0 = 0; r0 < kLT; r0 += 32) {
+      double v[16];
+#pragma unroll
+      for(int e = 0; e < 16; ++e) {
+        const int r = r0 + 2 * e + half;
+        const int off = __shfl(my_off, r, 64);
+        v[e] = (r < nr && col < cn) ? f[off + S.b + c0 + col] : 0.0;
+      }
+#pragma unroll
+      for(int e = 0; e < 16; ++e) T[r0 + 2 * e + half][col] = v[e];
+    }
+    __syncthreads();
+  };
+  auto store = [&](int c0, int cn) {
+    __syncthreads();
+    for(int r0 = 0; r0 < kLT; r0 += 32) {
+#pragma unroll
+      for(int e = 0; e < 16; ++e) {
+        const int r = r0 + 2 * e + half;
+        const int off = __shfl(my_off, r, 64);
+        if(r < nr && col < cn) f[off + S.b + c0 + col] = T[r][col];
+      }
+    }
+    __syncthreads();
+  };
+  double mr = 0.0;
+  for(int c0 = 0; c0 < n; c0 += kLX) {  // forward elimination
+    const int cn = min(kLX, n - c0);
+    load(c0, cn);
+    if(mine) {
+      for(int t = 0; t < cn; ++t) {
+        const int tg = c0 + t;
+        double r = T[lane][t];
+        if(tg == 0) {
+          if(n == 1) {
+            r = r + S.add_lo;
+            r = r + S.add_hi;
+          } else {
+            r = r + S.add_lo;
+          }
+          mr = r / piv[0];
+        } else {
+          if(tg == n - 1) r = r + S.add_hi;
+          mr = (r - S.lo * mr) / piv[tg];
+        }
+        T[lane][t] = mr;
+      }
+    }
+    store(c0, cn);
+  }
+  double x = mr;  // back substitution; x[n-1] = mr[n-1] is stored already
+  const double x_last = x;
+  for(int c0 = ((n - 1) / kLX) * kLX; c0 >= 0; c0 -= kLX) {
+    const int cn = min(kLX, n - c0);
+    load(c0, cn);
+    if(mine) {
+      for(int t = cn - 1; t >= 0; --t) {
+        const int tg = c0 + t;
+        if(tg == n - 1) continue;
+        x = T[lane][t] - mu[tg] * x;
+        T[lane][t] = x;
+      }
+    }
+    store(c0, cn);
+  }
+  if(S.cyclic) {  // Sherman-Morrison correction
+    const double fac = (x + S.uc * x_last / S.gamma) / S.den;
+    double first = 0.0;
+    for(int c0 = 0; c0 < n; c0 += kLX) {
+      const int cn = min(kLX, n - c0);
+      load(c0, cn);
+      if(mine) {
+        for(int t = 0; t < cn; ++t) T[lane][t] = T[lane][t] - fac * z[c0 + t];
+        if(c0 == 0) first = T[lane][0];
+      }
+      store(c0, cn);
+    }
+    if(S.copy_dup && mine) f[my_off + S.e] = first;
+  }
+}
+
+// y- and z-lines (lanes along i: coalesced): one thread per line as in k_line_solve, with the
+// line streamed through double-buffered register chunks so that the loads of the next chunk
+// overlap the recurrence of the current one.
+constexpr int kLC = 16;
+__global__ void __launch_bounds__(256) k_line_solve_s(double* f, Lay L, int axis, LineSys S,
+                                                      const double* __restrict__ piv, const double* __restrict__ mu,
+                                                      const double* __restrict__ z) {
+  const int o1 = axis == 2 ? 1 : 2;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t1 = blockIdx.y * blockDim.y + threadIdx.y;
+  if(t0 >= L.ax || t1 >= L.a(o1)) return;
+  int cc[3];
+  cc[axis] = 0;
+  cc[0] = 1 + t0;
+  cc[o1] = 1 + t1;
+  double* base = f + L.at(cc[0], cc[1], cc[2]);
+  const ptrdiff_t st = axis == 1 ? (ptrdiff_t)L.sx : (ptrdiff_t)L.sx * L.sy;
+  const int n = S.n;
+  if(S.cyclic && n == 1) {
+    base[S.b * st] = base[S.b * st] / S.single_pivot;
+    if(S.copy_dup) base[S.e * st] = base[S.b * st];
+    return;
+  }
+  auto ld = [&](int c0, double(&v)[kLC]) {
+#pragma unroll
+    for(int e = 0; e < kLC; ++e)
+      if(c0 + e < n) v[e] = base[(ptrdiff_t)(S.b + c0 + e) * st];
+  };
+  auto stv = [&](int c0, const double(&v)[kLC]) {
+#pragma unroll
+    for(int e = 0; e < kLC; ++e)
+      if(c0 + e < n) base[(ptrdiff_t)(S.b + c0 + e) * st] = v[e];
+  };
+  double cur[kLC], nxt[kLC];
+  double mr = 0.0;
+  ld(0, cur);
+  for(int c0 = 0; c0 < n; c0 += kLC) {  // forward elimination
+    if(c0 + kLC < n) ld(c0 + kLC, nxt);
+#pragma unroll
+    for(int e = 0; e < kLC; ++e) {
+      const int tg = c0 + e;
+      if(tg < n) {
+        double r = cur[e];
+        if(tg == 0) {
+          if(n == 1) {
+            r = r + S.add_lo;
+            r = r + S.add_hi;
+          } else {
+            r = r + S.add_lo;
+          }
+          mr = r / piv[0];
+        } else {
+          if(tg == n - 1) r = r + S.add_hi;
+          mr = (r - S.lo * mr) / piv[tg];
+        }
+        cur[e] = mr;
+      }
+    }
+    stv(c0, cur);
+#pragma unroll
+    for(int e = 0; e < kLC; ++e) cur[e] = nxt[e];
+  }
+  double x = mr;  // back substitution
+  const double x_last = x;
+  const int clast = ((n - 1) / kLC) * kLC;
+  ld(clast, cur);
+  for(int c0 = clast; c0 >= 0; c0 -= kLC) {
+    if(c0 > 0) ld(c0 - kLC, nxt);
+#pragma unroll
+    for(int e = kLC - 1; e >= 0; --e) {
+      const int tg = c0 + e;
+      if(tg < n - 1) {
+        x = cur[e] - mu[tg] * x;
+        cur[e] = x;
+      }
+    }
+    stv(c0, cur);
+#pragma unroll
+    for(int e = 0; e < kLC; ++e) cur[e] = nxt[e];
+  }
+  if(S.cyclic) {
+    const double fac = (x + S.uc * x_last / S.gamma) / S.den;
+    double first = 0.0;
+    ld(0, cur);
+    for(int c0 = 0; c0 < n; c0 += kLC) {
+      if(c0 + kLC < n) ld(c0 + kLC, nxt);
+#pragma unroll
+      for(int e = 0; e < kLC; ++e)
+        if(c0 + e < n) cur[e] = cur[e] - fac * z[c0 + e];
+      if(c0 == 0) first = cur[0];
+      stv(c0, cur);
+#pragma unroll
+      for(int e = 0; e < kLC; ++e) cur[e] = nxt[e];
+    }
+    if(S.copy_dup) base[S.e * st] = first;
+  }
+}
+
 inline bool dirichlet_velocity(const of_bc& c) {  // solver/projection.cpp:45-49
   return c.type == OF_NO_SLIP_WALL || c.type == OF_PRESCRIBED_VELOCITY || c.type == OF_SYMMETRY;
 }
@@ -265,13 +470,18 @@ int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha
     const Lay L = make_lay(g, OF_FACE_X + comp);
     for(const Pending& p : sweeps) {
       if(p.comp != comp) continue;
-      const int o0 = p.axis == 0 ? 1 : 0, o1 = p.axis == 2 ? 1 : 2;
-      const dim3 block(64, 4, 1);
-      const dim3 grid((unsigned)((L.a(o0) + 63) / 64), (unsigned)((L.a(o1) + 3) / 4), 1);
-      {
-        ProfScope prof_(c, "k_line_solve", true);
-        hipLaunchKernelGGL(k_line_solve, grid, block, 0, c.stream, out[comp], L, p.axis, p.S, dev + p.off,
-                           dev + p.off + nmax, dev + p.off + 2 * nmax);
+      const int o1 = p.axis == 2 ? 1 : 2;
+      const double* piv = dev + p.off;
+      const double* mu = dev + p.off + nmax;
+      const double* zv = dev + p.off + 2 * nmax;
+      if(p.axis == 0) {
+        ProfScope prof_(c, "k_line_solve_x", true);
+        hipLaunchKernelGGL(k_line_solve_x, dim3(div_up((size_t)L.ay * L.az, kLT)), dim3(kLT), 0, c.stream, out[comp], L,
+                           p.S, piv, mu, zv);
+      } else {
+        ProfScope prof_(c, "k_line_solve_s", true);
+        const dim3 grid((unsigned)((L.ax + 63) / 64), (unsigned)((L.a(o1) + 3) / 4), 1);
+        hipLaunchKernelGGL(k_line_solve_s, grid, dim3(64, 4, 1), 0, c.stream, out[comp], L, p.axis, p.S, piv, mu, zv);
       }
       launch_component_bc(c, g, b, comp, out[comp]);
     }

@@ -738,10 +738,15 @@ constexpr int kTailCells = 4096;  // levels of at most this many cells run in th
 
 template <typename T>
 struct TailBufs {
-  T* b[kMaxLevels];
+  T* b[kMaxLevels];  // global arrays
   T* xa[kMaxLevels];
   T* xb[kMaxLevels];
+  int ob[kMaxLevels];  // LDS offsets (elements of T) when the tail levels are LDS-resident
+  int oxa[kMaxLevels];
+  int oxb[kMaxLevels];
+  int owork;  // LDS offset of the coarse work vector (doubles)
 };
+constexpr int kTailLdsBytes = 152 * 1024;
 
 template <typename T>
 __device__ inline double tail_sum(double v) {
@@ -751,12 +756,27 @@ __device__ inline double tail_sum(double v) {
   return a[0];  // thread 0
 }
 
-template <typename T>
+// LDS: every tail level lives in one LDS buffer (b of level l0 is copied in, x of level l0
+// copied out); otherwise the levels are read and written in global memory.
+template <typename T, bool LDS>
 __global__ void __launch_bounds__(kTailThreads)
     k_vtail(const PLevel* __restrict__ lvs, int l0, int nl, TailBufs<T> B, const int* __restrict__ perm,
-            const double* __restrict__ factor, const double* __restrict__ U, double* __restrict__ work, double* ps,
+            const double* __restrict__ factor, const double* __restrict__ U, double* __restrict__ gwork, double* ps,
             int zm) {
   if(stopped(ps)) return;
+  __shared__ double sbuf[kTailLdsBytes / sizeof(double)];
+  T* sT = reinterpret_cast<T*>(sbuf);
+  auto Bp = [&](int l) -> T* { return LDS ? sT + B.ob[l] : B.b[l]; };
+  auto XA = [&](int l) -> T* { return LDS ? sT + B.oxa[l] : B.xa[l]; };
+  auto XB = [&](int l) -> T* { return LDS ? sT + B.oxb[l] : B.xb[l]; };
+  double* work = LDS ? sbuf + B.owork : gwork;
+  if(LDS) {
+    const int n0 = lvs[l0].n;
+    const T* src = B.b[l0];
+    T* dst = Bp(l0);
+    for(int id = threadIdx.x; id < n0; id += blockDim.x) dst[id] = src[id];
+    __syncthreads();
+  }
   __shared__ double lm[kMaxLevels][4];  // per level: mb, m2, m4, m5
   __shared__ double lsbw[kMaxLevels];   // sum of raw b w/D
   __shared__ double rz_sh;
@@ -771,8 +791,8 @@ __global__ void __launch_bounds__(kTailThreads)
   for(int l = l0; l < nl - 1; ++l) {
     const PLevel& L = lvs[l];
     const PLevel& Cl = lvs[l + 1];
-    const T* b = B.b[l];
-    T* x2 = B.xa[l];
+    const T* b = Bp(l);
+    T* x2 = XA(l);
     const T mb = T(lm[l][0]);
     const T m1 = zm ? T((lsbw[l] - lm[l][0] * L.winv_sum) / (double)L.n) : T(0);
     const int pxy = L.nx * L.ny;
@@ -788,7 +808,7 @@ __global__ void __launch_bounds__(kTailThreads)
     __syncthreads();
     const T mx = T(lm[l][1]);
     const T inv_children = T(1.0 / (double)(L.rx * L.ry * L.rz));
-    T* bc = B.b[l + 1];
+    T* bc = Bp(l + 1);
     const int cxy = Cl.nx * Cl.ny;
     double s0c = 0.0, s1c = 0.0;
     for(int id = tid; id < Cl.n; id += nt) {
@@ -827,8 +847,8 @@ __global__ void __launch_bounds__(kTailThreads)
   {
     const int lc = nl - 1;
     const int n = lvs[lc].n;
-    const T* b = B.b[lc];
-    T* x = B.xa[lc];
+    const T* b = Bp(lc);
+    T* x = XA(lc);
     const T mb = T(lm[lc][0]);
     for(int i = tid; i < n; i += nt) work[i] = (double)(b[i] - mb);
     __syncthreads();
@@ -882,15 +902,15 @@ __global__ void __launch_bounds__(kTailThreads)
   for(int l = nl - 2; l >= l0; --l) {
     const PLevel& L = lvs[l];
     const PLevel& Cl = lvs[l + 1];
-    const T* b = B.b[l];
+    const T* b = Bp(l);
     const T mb = T(lm[l][0]), m2 = T(lm[l][1]), mc = T(lm[l + 1][3]);
     const int pxy = L.nx * L.ny;
     double acc = 0.0;
     for(int id = tid; id < L.n; id += nt) {
       const int k = id / pxy, rem = id - k * pxy, j = rem / L.nx, i = rem - j * L.nx;
       const T xv =
-          prolong_smooth_cell<T>(L, Cl, b, B.xa[l], B.xa[l + 1], mb, m2, mc, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
-      B.xb[l][id] = xv;
+          prolong_smooth_cell<T>(L, Cl, b, XA(l), XA(l + 1), mb, m2, mc, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
+      XB(l)[id] = xv;
       acc += (double)xv;
     }
     acc = tail_sum<T>(acc);
@@ -901,8 +921,8 @@ __global__ void __launch_bounds__(kTailThreads)
     for(int id = tid; id < L.n; id += nt) {
       const int k = id / pxy, rem = id - k * pxy, j = rem / L.nx, i = rem - j * L.nx;
       T bt;
-      const T xv = smooth_last_cell<T>(L, b, B.xb[l], mb, m4, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k), bt);
-      B.xa[l][id] = xv;
+      const T xv = smooth_last_cell<T>(L, b, XB(l), mb, m4, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k), bt);
+      XA(l)[id] = xv;
       s5 += (double)xv;
       srz += (double)bt * (double)xv;
     }
@@ -913,6 +933,12 @@ __global__ void __launch_bounds__(kTailThreads)
       if(l == 0) rz_sh = srz;
     }
     __syncthreads();
+  }
+  if(LDS) {
+    const int n0 = lvs[l0].n;
+    const T* src = XA(l0);
+    T* dst = B.xa[l0];
+    for(int id = threadIdx.x; id < n0; id += blockDim.x) dst[id] = src[id];
   }
   if(tid == 0) {
     ps[s0 + L_MX5] = lm[l0][3];
@@ -1436,14 +1462,26 @@ void PoissonSolver::vcycle() {
   }
   {
     TailBufs<T> tb{};
+    size_t tot = 0;  // LDS layout in elements of T
     for(int l = l0; l < nl; ++l) {
       tb.b[l] = bptr(l);
       tb.xa[l] = static_cast<T*>(lxa[l]);
       tb.xb[l] = static_cast<T*>(lxb[l]);
+      tb.ob[l] = (int)tot;
+      tb.oxa[l] = (int)(tot + lv[l].n);
+      tb.oxb[l] = (int)(tot + 2 * (size_t)lv[l].n);
+      tot += 3 * (size_t)lv[l].n;
     }
+    const size_t work_off = (tot * sizeof(T) + sizeof(double) - 1) / sizeof(double);
+    tb.owork = (int)work_off;
+    const bool in_lds = (work_off + (size_t)nc) * sizeof(double) <= (size_t)kTailLdsBytes;
     ProfScope prof_(c, "k_vtail");
-    hipLaunchKernelGGL(k_vtail<T>, dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm, factor, U, work,
-                       ps, zmi);
+    if(in_lds)
+      hipLaunchKernelGGL((k_vtail<T, true>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm, factor,
+                         U, work, ps, zmi);
+    else
+      hipLaunchKernelGGL((k_vtail<T, false>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm, factor,
+                         U, work, ps, zmi);
   }
   for(int l = l0 - 1; l >= 0; --l) {
     const PLevel& P = lv[l];
