@@ -8,10 +8,14 @@ TVD (van Leer) advection, exact ADI Helmholtz predictor, MGPCG pressure solve at
 tolerance 1e-10. Every step runs in both precisions: FP64 (the reference's arithmetic) and
 MIXED (FP32 multigrid V-cycle inside the FP64 outer CG).
 
-Prints one JSON line. `value` is mixed-precision steps/s; the FP64 figures, the
-mixed-vs-FP64 field difference, the max |div u| over all steps, the roofline of the dominant
-kernel (live HIP-event timing inside the timed region) and the reference CPU baseline
-(timed on this host) are reported beside it.
+Prints one JSON line. `value` is mixed-precision steps/s; beside it: the FP64 figures, the
+mixed-vs-FP64 field difference, the max |div u| over all steps, the whole-step HBM rate from
+the algorithmic byte model, the roofline of the dominant kernel (live HIP-event timing of its
+launches inside the timed region, on the stream they run on) with the HBM traffic measured by
+rocprofv3 counters (profiles/), and the reference CPU path timed on this host.
+
+--copy-calibration runs a plain 1 GiB device-to-device copy three times and exits: the known
+byte count that scales the FETCH_SIZE / WRITE_SIZE counters.
 """
 from __future__ import annotations
 
@@ -19,14 +23,16 @@ import argparse
 import json
 import math
 import os
-import subprocess
+import platform
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "3D TG 256³ steps/s mixed vs FP64 at ‖div u‖∞≤1e-10; HBM GB/s; adjoint s/step"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "pmc_traffic_tg256.md")
 
 
 def parse():
@@ -36,62 +42,101 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
-    ap.add_argument("--profile-kernel", default="k_smooth12")
+    ap.add_argument("--profile-kernel", default="k_apply_dot")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-n", type=int, default=128, help="grid of the bounded CPU baseline sample")
+    ap.add_argument("--copy-calibration", action="store_true")
     return ap.parse_args()
 
 
-# Algorithmic HBM bytes per launch of the finest-level kernels: (distinct arrays read +
-# arrays written) x cells x element size; halo re-reads and caches excluded.
+# Algorithmic HBM bytes per launch of the finest-level kernels: (distinct arrays read + arrays
+# written) x cells x element size; halo re-reads and caches excluded. t = multigrid element size.
 def kernel_bytes(name, n_cells, mixed):
-    f64, lo = 8, (4 if mixed else 8)
+    f64, t = 8, (4 if mixed else 8)
     table = {
-        # b (rhs) read, x2 written
-        "k_smooth12": 2 * lo,
-        # x2, b read; coarse b (1/8) written
-        "k_residual_restrict": 2 * lo + lo / 8,
-        # x2, b, coarse x (1/8) read; x4 written
-        "k_prolong_smooth": 3 * lo + lo / 8,
-        # x4, b read; x5 written
-        "k_smooth_last": 3 * lo,
-        # d read, q written
-        "k_apply_dot": 2 * f64,
-        # p, d, r, q read; p, r written (+ FP32 copy of r in mixed)
-        "k_update": 6 * f64 + (4 if mixed else 0),
-        # z, d read; d written
-        "k_direction": 2 * f64 + lo,
+        "k_apply_dot": 2 * f64,                         # d read; q written
+        "k_direction": t + 2 * f64 + 2 * f64,           # z, d_old, p read; d_new, p written
+        "k_update": 3 * f64 + (4 if mixed else 0),      # r, q read; r (+ FP32 copy) written
+        "k_smooth12": 2 * t,                            # b read; x2 written
+        "k_residual_restrict": 2 * t + t / 8,           # b, x2 read; coarse b written
+        "k_prolong_smooth": 3 * t + t / 8,              # b, x2, coarse x read; x4 written
+        "k_smooth_last": 3 * t,                         # b, x4 read; x5 written
+        "k_predictor_rhs": 7 * f64,                     # q, 2 other components, p, A_prev; A, rhs
+        "k_line_solve_x": 2 * f64,                      # line read and written once
+        "k_line_solve_s": 2 * f64,
     }
     return table[name] * n_cells
 
 
-def cpu_baseline(n, nu, cfl, final_time, steps):
-    """Times the reference's own CPU path (oracle/_ref/ref_golden, built from the reference
-    sources) on this host, one thread; falls back to the oracle restatement if the reference
-    binary is absent."""
+def step_bytes_model(n, iters, mixed):
+    """Algorithmic HBM bytes of one step (passes over an n^3 FP64 array):
+    predictor RHS 3 x 7, ADI staging copies 3 x 2 + sweeps 9 x 2, pressure RHS 4, Poisson
+    setup/finish 12 (+1.5 FP32 cast), correction 9, pressure update 4, diagnostics 6; per MGPCG
+    iteration: CG direction/apply/update 10 and the V-cycle 10.25 x 8/7 (hierarchy) in the
+    multigrid precision (x 1/2 for FP32)."""
+    fixed = 21 + 6 + 18 + 4 + 12 + (1.5 if mixed else 0.0) + 9 + 4 + 6
+    per_it = 10.0 + 10.25 * 8.0 / 7.0 * (0.5 if mixed else 1.0)
+    return (fixed + iters * per_it) * n ** 3 * 8
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
+    try:
+        text = open(TRAFFIC_TABLE).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and cells[0] == kernel:
+            try:
+                return (float(cells[4]) + float(cells[5])) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
+def host_cpu():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(n_target, n, nu, cfl):
+    """The reference's own CPU path (oracle/_ref/ref_golden, built from the reference sources,
+    -O3, no fast-math) on this host, one thread, for one step of the same case at n^3; the
+    steps/s is scaled to n_target^3 by the cell ratio. Falls back to the oracle restatement."""
     from oracle import refrun
 
+    L = 2 * math.pi
+    final_time = 100 * (cfl / (3 * n / L)) * (1 - 1e-14)
+    scale = (n / n_target) ** 3
     if refrun.available():
         t0 = time.perf_counter()
         _, meta = refrun.run("tg_steps", (), (), timeout=1800, nx=n, ny=n, nz=n, viscosity=nu, cfl=cfl,
-                             final_time=final_time, tol=1e-10, maxit=400, steps=steps)
+                             final_time=final_time, tol=1e-10, maxit=400, steps=1)
         wall = time.perf_counter() - t0
-        el = meta.get("elapsed_seconds", wall)
-        return {"value": steps / el, "unit": "steps/s", "cores": 1, "kind": "reference",
-                "sample": f"{steps} step(s) of the same {n}^3 Taylor-Green case (reference-convention "
-                          f"initial field, FP64, tol 1e-10), reference CPU sources built -O3 without "
-                          f"fast-math, 1 thread; {el:.1f} s for the steps",
-                "pressure_iterations": meta.get("pressure_iterations")}
-    from oracle import ofref
+        el = float(meta.get("elapsed_seconds", wall))
+        kind, what = "reference", "the reference CPU sources built -O3 without fast-math"
+        iters = meta.get("pressure_iterations")
+    else:
+        from oracle import ofref
 
-    c = ofref.tg_config(n, n, n, nu, cfl, final_time, 400, 1e-10)
-    s = ofref.tg_init(c, convention=1)
-    t0 = time.perf_counter()
-    ofref.tg_steps(c, s, steps)
-    el = time.perf_counter() - t0
-    return {"value": steps / el, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} step(s) of the {n}^3 case with the oracle restatement, 1 thread; {el:.1f} s",
-            "pressure_iterations": s.metrics.pressure_iterations}
+        c = ofref.tg_config(n, n, n, nu, cfl, final_time, 400, 1e-10)
+        s = ofref.tg_init(c, convention=1)
+        t0 = time.perf_counter()
+        ofref.tg_steps(c, s, 1)
+        el = time.perf_counter() - t0
+        kind, what = "port", "the oracle's FP64 restatement of the reference"
+        iters = s.metrics.pressure_iterations
+    return {"value": scale / el, "unit": "steps/s", "cores": 1, "kind": kind,
+            "sample": f"1 step of the same Taylor-Green case at {n}^3 ({n ** 3} cells, FP64, MGPCG tol 1e-10, "
+                      f"{iters} iterations) with {what}, 1 thread: {el:.2f} s; value = 1/{el:.2f} s scaled to "
+                      f"{n_target}^3 by the cell ratio {scale:g}",
+            "host_cpu": host_cpu(), "host_logical_cpus": os.cpu_count()}
 
 
 def run_precision(sa, ctx, n, nu, dt, precision, warmup, steps, prof_kernel):
@@ -131,6 +176,19 @@ def rel_l2(a, b):
     return float(np.sqrt(np.mean((x - y) ** 2)) / max(np.sqrt(np.mean(y * y)), 1e-300))
 
 
+def copy_calibration():
+    import torch
+
+    n = 1 << 27  # 2^27 doubles = 1 GiB
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}))
+    return 0
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -138,8 +196,8 @@ def main():
                                    "(multi-GPU sharding is a design note, not a measured configuration)",
                           "n_gpus": a.gpus}))
         return 2
-    import torch
-
+    if a.copy_calibration:
+        return copy_calibration()
     import solver_amd as sa
 
     n = a.n
@@ -163,12 +221,20 @@ def main():
     r = res[main_prec]
     value = a.steps / r["seconds"]
     n_cells = n ** 3
+
+    def step_hbm(rr, mixed):
+        it = sum(rr["iters"]) / len(rr["iters"])
+        b = step_bytes_model(n, it, mixed)
+        gbs = b / (rr["seconds"] / a.steps) / 1e9
+        return {"algorithmic_bytes_per_step": b, "achieved_GBs": gbs, "peak_GBs": HBM_PEAK_GBS,
+                "frac": gbs / HBM_PEAK_GBS}
+
     prof = r["prof"]
     avg_ms = prof["total_ms"] / max(prof["launches"], 1)
     nbytes = kernel_bytes(a.profile_kernel, n_cells, main_prec == sa.MIXED)
     achieved = nbytes / (avg_ms * 1e-3) / 1e9 if prof["launches"] else None
     out = {
-        "metric": "3D TG 256^3 steps/s mixed vs FP64 at ||div u||_inf <= 1e-10",
+        "metric": METRIC,
         "value": value,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -184,22 +250,26 @@ def main():
                    "nx": n, "ny": n, "nz": n, "dt": dt, "viscosity": nu, "parallelism": "single GPU"},
         "mgpcg_iterations_per_step": sum(r["iters"]) / len(r["iters"]),
         "max_div_linf": max(x["div_linf"] for x in res.values()),
+        "step_hbm": step_hbm(r, main_prec == sa.MIXED),
         "roofline": {"bound": "hbm", "kernel": a.profile_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": measured_traffic(a.profile_kernel),
                      "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": avg_ms,
                      "launches": prof["launches"]},
+        "adjoint_s_per_step": None,
     }
     if sa.FP64 in res:
         f = res[sa.FP64]
         out["fp64"] = {"steps_per_s": a.steps / f["seconds"], "ms_per_step": 1e3 * f["seconds"] / a.steps,
-                       "mgpcg_iterations_per_step": sum(f["iters"]) / len(f["iters"]), "max_div_linf": f["div_linf"]}
+                       "mgpcg_iterations_per_step": sum(f["iters"]) / len(f["iters"]), "max_div_linf": f["div_linf"],
+                       "step_hbm": step_hbm(f, False)}
         if sa.MIXED in res:
             out["mixed_speedup_vs_fp64"] = value / out["fp64"]["steps_per_s"]
             out["mixed_vs_fp64_rel_l2"] = {k: rel_l2(x, y) for k, x, y in
                                            zip("uvwp", res[sa.MIXED]["fields"], res[sa.FP64]["fields"])}
     if not a.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(n, nu, cfl, final_time, a.cpu_steps)
+            out["cpu_baseline"] = cpu_baseline(n, a.cpu_n, nu, cfl)
         except Exception as e:  # reported, never fatal to the GPU measurement
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(out))

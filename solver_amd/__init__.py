@@ -119,6 +119,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m solver_amd.build`")
+        # The library and torch must share one HIP runtime: device pointers come from torch's
+        # allocator. torch ships its own libamdhip64.so.7; loading torch first makes the
+        # library's NEEDED entry bind to that runtime instead of a second copy from /opt/rocm.
+        import torch  # noqa: F401
+
         _lib = C.CDLL(LIB_PATH)
         _lib.of_last_error.restype = C.c_char_p
         _lib.of_build_info.restype = C.c_char_p
