@@ -46,7 +46,93 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128, help="grid of the bounded CPU baseline sample")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--adjoint", choices=["c4", "c5", "none"], default="c4",
+                    help="adjoint workload timed beside the forward step (c5: 512^3, about 160 GB of HBM)")
+    ap.add_argument("--adjoint-fd", type=int, default=0,
+                    help="central finite-difference checks of the adjoint gradient along this many seeded directions")
+    ap.add_argument("--adjoint-only", action="store_true", help="skip the forward benchmark")
     return ap.parse_args()
+
+
+ADJOINT_SEED = 20261203
+# 128^3 Re=400, 200 steps, checkpoint every 20, FP64, gradient of the final kinetic energy
+# w.r.t. the initial velocity; 512^3 Re=1600, 100 steps, checkpoint every 10, mixed precision,
+# gradient w.r.t. a seeded forcing field f = (A sin(kx x) sin(ky y), 0, 0), A = 1e-2.
+ADJOINT_CASES = {
+    "c4": dict(n=128, nu=1.0 / 400.0, steps=200, every=20, mixed=False, forced=False, tol=1e-12),
+    "c5": dict(n=512, nu=1.0 / 1600.0, steps=100, every=10, mixed=True, forced=True, tol=1e-10),
+}
+
+
+def run_adjoint(sa, ctx, name, fd_dirs):
+    """Forward run with checkpoints, then the reverse sweep (window recompute + adjoint steps).
+    Initial state: the Taylor-Green field plus the seeded divergence-free perturbation of RMS
+    1e-2. Optional central finite differences of the final kinetic energy along seeded
+    directions (initial velocity, or forcing for the forced case)."""
+    import torch
+
+    from solver_amd import inputs
+
+    cs = ADJOINT_CASES[name]
+    n, nu, steps, every = cs["n"], cs["nu"], cs["steps"], cs["every"]
+    prec = sa.MIXED if cs["mixed"] else sa.FP64
+    L = 2 * math.pi
+    dt, ns = sa.taylor_green_dt(n, n, n, 0.5, steps * (0.5 / (3 * n / L)) * (1 - 1e-14))
+    assert ns == steps, ns
+    st = sa.Stepper(ctx, sa.TAYLOR_GREEN, n, n, n, nu, dt, 400, cs["tol"], prec)
+    st.initialize(convention=1)
+    vel, p, _, _, _ = st.download()
+    st.close()
+    pert = inputs.perturbation(n, ADJOINT_SEED, 1e-2)
+    u0 = [torch.from_numpy(a + b).cuda() for a, b in zip(vel, pert)]
+    p0 = torch.from_numpy(p).cuda()
+    del vel, pert, p
+    adj = sa.Adjoint(ctx, n, n, n, nu, dt, every, 400, cs["tol"], prec)
+    if cs["forced"]:
+        adj.set_forcing([torch.from_numpy(f).cuda() for f in inputs.forcing(n, ADJOINT_SEED, 1e-2)])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ke, m = adj.forward(u0, p0, steps)
+    t1 = time.perf_counter()
+    gu, _, gf = adj.backward(want_pressure=False, want_forcing=cs["forced"])
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out = {"workload": f"{n}^3 Re={round(1 / nu)} {steps} steps, checkpoint every {every}, "
+                       f"{'mixed' if cs['mixed'] else 'fp64'}, d KE(T) / d {'f' if cs['forced'] else 'u0'}",
+           "n": n, "steps": steps, "checkpoint_every": every, "dt": dt, "poisson_tolerance": cs["tol"],
+           "forward_s": t1 - t0, "reverse_s": t2 - t1, "reverse_s_per_step": (t2 - t1) / steps,
+           "gradient_s_per_step": (t2 - t0) / steps, "final_kinetic_energy": ke,
+           "max_div_linf": m.max_divergence_linf}
+    if cs["forced"]:
+        out["forcing_wavenumbers"] = list(inputs.forcing_wavenumbers(ADJOINT_SEED))
+    grad = gf if cs["forced"] else gu
+    errs = []
+    for d in range(1, fd_dirs + 1):
+        dirn = [torch.from_numpy(a).cuda() for a in inputs.perturbation(n, ADJOINT_SEED + d, 1.0)]
+        pred = float(sum((g * x).sum() for g, x in zip(grad, dirn)))
+        best = None
+        for eps in (1e-5, 1e-6, 1e-7):
+            ks = []
+            for sgn in (1.0, -1.0):
+                if cs["forced"]:
+                    base = inputs.forcing(n, ADJOINT_SEED, 1e-2)
+                    adj.set_forcing([torch.from_numpy(b).cuda() + sgn * eps * x for b, x in zip(base, dirn)])
+                    ks.append(adj.forward(u0, p0, steps)[0])
+                else:
+                    ks.append(adj.forward([a + sgn * eps * x for a, x in zip(u0, dirn)], p0, steps)[0])
+            e = abs((ks[0] - ks[1]) / (2 * eps) - pred) / abs(pred)
+            best = e if best is None else min(best, e)
+        errs.append(best)
+        print(f"# adjoint {name}: direction {d}: <grad, d> = {pred:.10e}, best central-difference rel. error "
+              f"{best:.2e}", flush=True)
+    if errs:
+        out["fd_directions"] = len(errs)
+        out["fd_max_rel_error"] = max(errs)
+        out["fd_eps"] = [1e-5, 1e-6, 1e-7]
+    adj.close()
+    del u0, p0, grad, gu, gf
+    torch.cuda.empty_cache()
+    return out
 
 
 # Algorithmic HBM bytes per launch of the finest-level kernels: (distinct arrays read + arrays
@@ -209,6 +295,11 @@ def main():
     dt, nsteps = sa.taylor_green_dt(n, n, n, cfl, final_time)
     assert nsteps == 100, nsteps
     ctx = sa.Context(0)
+    if a.adjoint_only:
+        r = run_adjoint(sa, ctx, a.adjoint if a.adjoint != "none" else "c4", a.adjoint_fd)
+        print(json.dumps({"metric": "adjoint s/step", "value": r["reverse_s_per_step"], "unit": "s/step",
+                          "higher_is_better": False, "adjoint": r}))
+        return 0
     precs = {"both": [sa.FP64, sa.MIXED], "fp64": [sa.FP64], "mixed": [sa.MIXED]}[a.precision]
     res = {}
     for prec in precs:
@@ -267,6 +358,14 @@ def main():
             out["mixed_speedup_vs_fp64"] = value / out["fp64"]["steps_per_s"]
             out["mixed_vs_fp64_rel_l2"] = {k: rel_l2(x, y) for k, x, y in
                                            zip("uvwp", res[sa.MIXED]["fields"], res[sa.FP64]["fields"])}
+    if a.adjoint != "none":
+        for k in list(res):
+            res[k].pop("fields", None)
+        r_adj = run_adjoint(sa, ctx, a.adjoint, a.adjoint_fd)
+        print(f"# adjoint {a.adjoint}: forward {r_adj['forward_s']:.2f} s, reverse {r_adj['reverse_s']:.2f} s "
+              f"({r_adj['reverse_s_per_step'] * 1e3:.1f} ms/step)", flush=True)
+        out["adjoint_s_per_step"] = r_adj["reverse_s_per_step"]
+        out["adjoint"] = r_adj
     if not a.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(n, a.cpu_n, nu, cfl)

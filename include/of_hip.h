@@ -289,6 +289,67 @@ int of_stepper_run(of_stepper* s, int steps, of_step_metrics* metrics);
 int of_stepper_run_batched(of_stepper* s, int steps, of_step_metrics* metrics);
 /* Kinetic energy of the current state (solver/taylor_green.cpp:269-288). */
 int of_stepper_kinetic_energy(of_stepper* s, double* out);
+/* Body force f (device face arrays, velocity layout; copied) added to the Taylor-Green
+ * predictor right-hand side as rhs += dt f after the reference's terms
+ * (solver/taylor_green.cpp:227-256 has no such term). NULL arrays clear it. */
+int of_stepper_set_forcing(of_stepper* s, const double* fu, const double* fv, const double* fw);
+
+/* ---------------------------------------------------------------- discrete adjoint */
+/* The reference has no derivative code. These entry points add the discrete adjoint and the
+ * tangent-linear model of the device step above for the three-dimensional periodic
+ * Taylor-Green case. All field pointers are device pointers in the reference storage layout.
+ *
+ * Per-operator transposes (periodic boundaries only; `bcs` must be all periodic). Outputs
+ * are overwritten over the whole ghosted storage.
+ *   of_apply_velocity_boundary_conditions  (solver/projection.cpp:233-300)  -> in place
+ *   of_apply_pressure_boundary_conditions  (solver/projection.cpp:207-231)  -> in place
+ *   of_compute_gradient    (operators/discrete_operators.cpp:64-96)   gbar -> pbar
+ *   of_compute_divergence  (operators/discrete_operators.cpp:98-113)  dbar -> ubar
+ *   of_solve_predictor_adi (solver/projection.hpp:49-52)              ybar -> rbar
+ * and the linearised advection (solver/momentum_terms.cpp:88-312, branches frozen at u):
+ *   tangent: out = dA(u)[du];  adjoint: bar = dA(u)^T[abar]. */
+int of_apply_velocity_boundary_conditions_adjoint(of_context* ctx, const of_grid* grid, const of_bc_set* bcs,
+                                                  double* u, double* v, double* w);
+int of_apply_pressure_boundary_conditions_adjoint(of_context* ctx, const of_grid* grid,
+                                                  const of_pressure_bc_set* bcs, double* p);
+int of_compute_gradient_adjoint(of_context* ctx, const of_grid* grid, const double* gx, const double* gy,
+                                const double* gz, double* p);
+int of_compute_divergence_adjoint(of_context* ctx, const of_grid* grid, const double* d, double* u, double* v,
+                                  double* w);
+int of_solve_predictor_adi_adjoint(of_context* ctx, const of_grid* grid, const of_bc_set* bcs, double alpha,
+                                   const double* yu, const double* yv, const double* yw, double* ru, double* rv,
+                                   double* rw);
+int of_compute_advection_tangent(of_context* ctx, const of_grid* grid, int scheme, const double* u, const double* v,
+                                 const double* w, const double* du, const double* dv, const double* dw, double* au,
+                                 double* av, double* aw);
+int of_compute_advection_adjoint(of_context* ctx, const of_grid* grid, int scheme, const double* u, const double* v,
+                                 const double* w, const double* abar_u, const double* abar_v, const double* abar_w,
+                                 double* ubar, double* vbar, double* wbar);
+
+/* Checkpointed adjoint run: the forward pass stores the state every `checkpoint_every` steps;
+ * the reverse sweep recomputes one window at a time. */
+typedef struct of_adjoint of_adjoint;
+int of_adjoint_create(of_context* ctx, const of_stepper_config* cfg, int checkpoint_every, of_adjoint** out);
+int of_adjoint_destroy(of_adjoint* a);
+/* forcing as for of_stepper_set_forcing; NULL clears it */
+int of_adjoint_set_forcing(of_adjoint* a, const double* fu, const double* fv, const double* fw);
+/* Runs `steps` steps from (u0, v0, w0, p0); p0 may be NULL (zero). Only active entries are
+ * read. The final kinetic energy and metrics are returned. */
+int of_adjoint_forward(of_adjoint* a, const double* u0, const double* v0, const double* w0, const double* p0,
+                       int steps, double* kinetic_energy, of_step_metrics* metrics);
+/* Copies the final velocity (ghosted storage). */
+int of_adjoint_final_velocity(of_adjoint* a, double* u, double* v, double* w);
+/* Reverse sweep. seed_*: adjoint of the final velocity storage; all three NULL selects the
+ * gradient of the final kinetic energy. Outputs (NULL to skip): gradients w.r.t. the active
+ * entries of u0, v0, w0, p0 and the forcing field (inactive entries zero). */
+int of_adjoint_backward(of_adjoint* a, const double* seed_u, const double* seed_v, const double* seed_w,
+                        double* gu, double* gv, double* gw, double* gp, double* gfu, double* gfv, double* gfw);
+/* Tangent-linear run along (du0, dv0, dw0, dp0, dfu, dfv, dfw) (NULL: zero) from the initial
+ * state of the last forward pass. Outputs: the final velocity perturbation (NULL to skip) and
+ * the directional derivative of the final kinetic energy (NULL to skip). */
+int of_adjoint_tangent(of_adjoint* a, const double* du0, const double* dv0, const double* dw0, const double* dp0,
+                       const double* dfu, const double* dfv, const double* dfw, double* du, double* dv, double* dw,
+                       double* dkinetic_energy);
 
 #ifdef __cplusplus
 }

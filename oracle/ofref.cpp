@@ -410,78 +410,125 @@ static void divergence(const Vel& v, Field& d) {
 // ---------------------------------------------------------------- advection
 constexpr double kSmallDenominator = 1.0e-14;  // solver/momentum_terms.cpp:62
 
-static inline double van_leer(double r) { return (r + std::abs(r)) / (1.0 + std::abs(r)); }  // :72-79
-static inline double safe_ratio(double n, double d) { return std::abs(d) < kSmallDenominator ? 0.0 : n / d; }  // :81-86
+// Forward-mode dual number (value, derivative) for the tangent-linear restatement. Branches
+// test the value only, so every branch is frozen at the base state. |x| takes the derivative
+// sign of -x at x = 0, which makes the van Leer limiter's derivative 0 at its kink r = 0.
+struct Dual {
+  double v, d;
+};
+static inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+static inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+static inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+static inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+static inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+static inline Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+static inline Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+static inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+static inline double value_of(double x) { return x; }
+static inline double value_of(Dual x) { return x.v; }
+static inline double abs_of(double x) { return std::abs(x); }
+static inline Dual abs_of(Dual x) { return x.v > 0.0 ? x : -x; }
+template <typename T>
+static inline T zero_of() { return T{}; }
+
+template <typename T>
+static inline T van_leer(T r) { return (r + abs_of(r)) / (1.0 + abs_of(r)); }  // :72-79
+template <typename T>
+static inline T safe_ratio(T n, T d) { return std::abs(value_of(d)) < kSmallDenominator ? zero_of<T>() : n / d; }  // :81-86
 
 // solver/momentum_terms.cpp:88-197: face value at the interface between storage index
 // `right`-1 and `right` along `axis`; `at(o)` reads the field at offset o along the axis
-// relative to `right`. scheme: 0 tvd, 1 upwind, 2 central.
-template <typename At>
-static inline double reconstruct(At&& at, int right, int max_index, double vel, int scheme) {
-  const double left = at(-1), rgt = at(0);
+// relative to `right`. scheme: 0 tvd, 1 upwind, 2 central. T is double, or Dual for the
+// tangent (the same operations on (value, derivative) pairs).
+template <typename T, typename At>
+static inline T reconstruct(At&& at, int right, int max_index, T vel, int scheme) {
+  const T left = at(-1), rgt = at(0);
   if(scheme == 2) return 0.5 * (left + rgt);
-  if(scheme == 1 || std::abs(vel) < kSmallDenominator) return vel >= 0.0 ? left : rgt;
-  if(vel >= 0.0) {
+  if(scheme == 1 || std::abs(value_of(vel)) < kSmallDenominator) return value_of(vel) >= 0.0 ? left : rgt;
+  if(value_of(vel) >= 0.0) {
     if(right - 2 < 0) return left;
-    const double dd = rgt - left;
-    const double r = safe_ratio(left - at(-2), dd);
+    const T dd = rgt - left;
+    const T r = safe_ratio(left - at(-2), dd);
     return left + 0.5 * van_leer(r) * dd;
   }
   if(right + 1 > max_index) return rgt;
-  const double d = rgt - left;
-  const double r = safe_ratio(at(1) - rgt, d);
+  const T d = rgt - left;
+  const T r = safe_ratio(at(1) - rgt, d);
   return rgt - 0.5 * van_leer(r) * d;
 }
 
-static double rec_x(const Field& f, int i, int j, int k, double vel, int scheme) {
-  return reconstruct([&](int o) { return f(i + o, j, k); }, i, f.L.s[0] - 1, vel, scheme);
-}
-static double rec_y(const Field& f, int i, int j, int k, double vel, int scheme) {
-  return reconstruct([&](int o) { return f(i, j + o, k); }, j, f.L.s[1] - 1, vel, scheme);
-}
-static double rec_z(const Field& f, int i, int j, int k, double vel, int scheme) {
-  return reconstruct([&](int o) { return f(i, j, k + o); }, k, f.L.s[2] - 1, vel, scheme);
+// Reads component c of the velocity at (i, j, k) as T: the value, or (value, perturbation).
+template <typename T>
+struct VelReader;
+template <>
+struct VelReader<double> {
+  const Vel& v;
+  double operator()(int c, int i, int j, int k) const { return v.c(c)(i, j, k); }
+};
+template <>
+struct VelReader<Dual> {
+  const Vel& v;
+  const Vel& dv;
+  Dual operator()(int c, int i, int j, int k) const { return Dual{v.c(c)(i, j, k), dv.c(c)(i, j, k)}; }
+};
+
+// solver/momentum_terms.cpp:199-312 (component c advected by the MAC velocity); `put` stores
+// the T-valued result.
+template <typename T, typename Put>
+static void advection_component_t(const VelReader<T>& V, const Layout& Lq, int c, int scheme, Put&& put) {
+  const Grid& g = Lq.g;
+  auto rec = [&](int axis, int i, int j, int k, T vel) {
+    const int right = axis == 0 ? i : axis == 1 ? j : k;
+    return reconstruct<T>(
+        [&](int o) { return V(c, i + (axis == 0 ? o : 0), j + (axis == 1 ? o : 0), k + (axis == 2 ? o : 0)); }, right,
+        Lq.s[axis] - 1, vel, scheme);
+  };
+  for_range(Lq.active(), [&](int i, int j, int k) {
+    T wv, ev, sv, nv, bv, fv;
+    if(c == 0) {
+      wv = 0.5 * (V(0, i - 1, j, k) + V(0, i, j, k));
+      ev = 0.5 * (V(0, i, j, k) + V(0, i + 1, j, k));
+      sv = 0.5 * (V(1, i - 1, j, k) + V(1, i, j, k));
+      nv = 0.5 * (V(1, i - 1, j + 1, k) + V(1, i, j + 1, k));
+      bv = 0.5 * (V(2, i - 1, j, k) + V(2, i, j, k));
+      fv = 0.5 * (V(2, i - 1, j, k + 1) + V(2, i, j, k + 1));
+    } else if(c == 1) {
+      wv = 0.5 * (V(0, i, j - 1, k) + V(0, i, j, k));
+      ev = 0.5 * (V(0, i + 1, j - 1, k) + V(0, i + 1, j, k));
+      sv = 0.5 * (V(1, i, j - 1, k) + V(1, i, j, k));
+      nv = 0.5 * (V(1, i, j, k) + V(1, i, j + 1, k));
+      bv = 0.5 * (V(2, i, j - 1, k) + V(2, i, j, k));
+      fv = 0.5 * (V(2, i, j - 1, k + 1) + V(2, i, j, k + 1));
+    } else {
+      wv = 0.5 * (V(0, i, j, k - 1) + V(0, i, j, k));
+      ev = 0.5 * (V(0, i + 1, j, k - 1) + V(0, i + 1, j, k));
+      sv = 0.5 * (V(1, i, j, k - 1) + V(1, i, j, k));
+      nv = 0.5 * (V(1, i, j + 1, k - 1) + V(1, i, j + 1, k));
+      bv = 0.5 * (V(2, i, j, k - 1) + V(2, i, j, k));
+      fv = 0.5 * (V(2, i, j, k) + V(2, i, j, k + 1));
+    }
+    const T wf = wv * rec(0, i, j, k, wv);
+    const T ef = ev * rec(0, i + 1, j, k, ev);
+    const T sf = sv * rec(1, i, j, k, sv);
+    const T nf = nv * rec(1, i, j + 1, k, nv);
+    const T bf = bv * rec(2, i, j, k, bv);
+    const T ff = fv * rec(2, i, j, k + 1, fv);
+    put(i, j, k, (ef - wf) / g.dx + (nf - sf) / g.dy + (ff - bf) / g.dz);
+  });
 }
 
-// solver/momentum_terms.cpp:199-312 (component c advected by the MAC velocity)
 static void advection_component(const Vel& v, int c, int scheme, Field& out) {
-  const Field& q = v.c(c);
-  const Grid& g = q.L.g;
-  // offsets of the two transporting faces that are averaged to the flux face
-  const int oi = c == 0, oj = c == 1, ok = c == 2;
-  for_range(q.L.active(), [&](int i, int j, int k) {
-    double wv, ev, sv, nv, bv, fv;
-    if(c == 0) {
-      wv = 0.5 * (v.x(i - 1, j, k) + v.x(i, j, k));
-      ev = 0.5 * (v.x(i, j, k) + v.x(i + 1, j, k));
-      sv = 0.5 * (v.y(i - 1, j, k) + v.y(i, j, k));
-      nv = 0.5 * (v.y(i - 1, j + 1, k) + v.y(i, j + 1, k));
-      bv = 0.5 * (v.z(i - 1, j, k) + v.z(i, j, k));
-      fv = 0.5 * (v.z(i - 1, j, k + 1) + v.z(i, j, k + 1));
-    } else if(c == 1) {
-      wv = 0.5 * (v.x(i, j - 1, k) + v.x(i, j, k));
-      ev = 0.5 * (v.x(i + 1, j - 1, k) + v.x(i + 1, j, k));
-      sv = 0.5 * (v.y(i, j - 1, k) + v.y(i, j, k));
-      nv = 0.5 * (v.y(i, j, k) + v.y(i, j + 1, k));
-      bv = 0.5 * (v.z(i, j - 1, k) + v.z(i, j, k));
-      fv = 0.5 * (v.z(i, j - 1, k + 1) + v.z(i, j, k + 1));
-    } else {
-      wv = 0.5 * (v.x(i, j, k - 1) + v.x(i, j, k));
-      ev = 0.5 * (v.x(i + 1, j, k - 1) + v.x(i + 1, j, k));
-      sv = 0.5 * (v.y(i, j, k - 1) + v.y(i, j, k));
-      nv = 0.5 * (v.y(i, j + 1, k - 1) + v.y(i, j + 1, k));
-      bv = 0.5 * (v.z(i, j, k - 1) + v.z(i, j, k));
-      fv = 0.5 * (v.z(i, j, k) + v.z(i, j, k + 1));
-    }
-    (void)oi; (void)oj; (void)ok;
-    const double wf = wv * rec_x(q, i, j, k, wv, scheme);
-    const double ef = ev * rec_x(q, i + 1, j, k, ev, scheme);
-    const double sf = sv * rec_y(q, i, j, k, sv, scheme);
-    const double nf = nv * rec_y(q, i, j + 1, k, nv, scheme);
-    const double bf = bv * rec_z(q, i, j, k, bv, scheme);
-    const double ff = fv * rec_z(q, i, j, k + 1, fv, scheme);
-    out(i, j, k) = (ef - wf) / g.dx + (nf - sf) / g.dy + (ff - bf) / g.dz;
-  });
+  advection_component_t<double>(VelReader<double>{v}, v.c(c).L, c, scheme,
+                                [&](int i, int j, int k, double a) { out(i, j, k) = a; });
+}
+
+// Tangent of the advection at v along dv with the branches frozen at v (dual numbers).
+static void advection_tangent(const Vel& v, const Vel& dv, int scheme, Vel& out) {
+  for(int c = 0; c < 3; ++c) {
+    Field& o = out.c(c);
+    advection_component_t<Dual>(VelReader<Dual>{v, dv}, v.c(c).L, c, scheme,
+                                [&](int i, int j, int k, Dual a) { o(i, j, k) = a.d; });
+  }
 }
 
 static void advection(const Vel& v, int scheme, Vel& out) {  // solver/momentum_terms.cpp:353-360
@@ -577,11 +624,21 @@ static void axpy_vel(Vel& d, const Vel& s, double scale) {
 }
 
 // solver/taylor_green.cpp:227-256 (cavity variant solver/lid_driven_cavity.cpp:489-518)
+// The terms after the advection, with the advection `adv` given. Linear in (cur, p_total,
+// prev, adv), so it also assembles the tangent from the perturbations.
+static void assemble_predictor_rhs(const Vel& cur, const Field& p_total, const Vel* prev, const Vel& adv, double nu,
+                                   double dt, bool cavity, Vel& rhs);
+
 static void predictor_rhs(const Vel& cur, const Field& p_total, const Vel* prev, int scheme, double nu, double dt,
                           bool cavity, Vel& adv, Vel& rhs) {
+  advection(cur, scheme, adv);
+  assemble_predictor_rhs(cur, p_total, prev, adv, nu, dt, cavity, rhs);
+}
+
+static void assemble_predictor_rhs(const Vel& cur, const Field& p_total, const Vel* prev, const Vel& adv, double nu,
+                                   double dt, bool cavity, Vel& rhs) {
   const Grid& g = cur.x.L.g;
   Vel dif(g), grad(g), fc(g);
-  advection(cur, scheme, adv);
   diffusion(cur, nu, dif);
   gradient(p_total, grad);
   const double alpha = 0.5 * nu * dt;
@@ -1195,7 +1252,7 @@ static double divergence_linf(const Vel& v) {
 // solver/lid_driven_cavity.cpp:562-623.
 static void fractional_step(const BCSet& bcs, double nu, double dt, int scheme, bool cavity, int max_it, double tol,
                             Vel& velocity, Vel& adv_prev, Field& p_total, int& has_prev, StepMetrics& m,
-                            const char* name, bool want_linf) {
+                            const char* name, bool want_linf, const Vel* force = nullptr) {
   const Grid& g = velocity.x.L.g;
   const double alpha = 0.5 * nu * dt;
   const ProjOpts o{dt, 1.0, max_it, tol};
@@ -1204,6 +1261,8 @@ static void fractional_step(const BCSet& bcs, double nu, double dt, int scheme, 
   cur = velocity;
   apply_velocity_bc(bcs, cur);
   predictor_rhs(cur, p_total, has_prev ? &adv_prev : nullptr, scheme, nu, dt, cavity, adv, rhs);
+  // body force of this project (the reference has none): rhs += dt f after the reference's terms
+  if(force) axpy_vel(rhs, *force, dt);
   pred = rhs;
   predictor_adi(rhs, alpha, bcs, pred);
   phi.fill(0.0);
@@ -1238,6 +1297,43 @@ static void fractional_step(const BCSet& bcs, double nu, double dt, int scheme, 
   n.pressure_relative_residual = pd.poisson.relative_residual;
   n.divergence_linf = want_linf ? std::max(m.divergence_linf, divergence_linf(corr)) : 0.0;
   m = n;
+}
+
+// Tangent-linear model of fractional_step for periodic boundaries (every ghost relation is then
+// homogeneous and linear): advances the perturbation (dvel, dadv_prev, dp) of the state along
+// with a force perturbation dforce, with the advection linearised at the base state; the base
+// state is then advanced by fractional_step. Every other stage is linear and is applied to the
+// perturbation as it is to the state.
+static void fractional_step_tl(const BCSet& bcs, double nu, double dt, int scheme, int max_it, double tol,
+                               Vel& velocity, Vel& adv_prev, Field& p_total, int& has_prev, StepMetrics& m,
+                               const Vel* force, Vel& dvel, Vel& dadv_prev, Field& dp, const Vel* dforce) {
+  for(int f = 0; f < 6; ++f)
+    if(bcs.f[f].type != kPeriodic) throw std::invalid_argument("the tangent-linear step needs periodic boundaries");
+  const Grid& g = velocity.x.L.g;
+  const double alpha = 0.5 * nu * dt;
+  const ProjOpts o{dt, 1.0, max_it, tol};
+  Vel cur(g), dcur(g), dadv(g), drhs(g), dpred(g), dcorr(g), dif(g);
+  Field dphi(make_layout(g, kCell)), dprhs(make_layout(g, kCell));
+  cur = velocity;
+  apply_velocity_bc(bcs, cur);
+  dcur = dvel;
+  apply_velocity_bc(bcs, dcur);
+  advection_tangent(cur, dcur, scheme, dadv);
+  assemble_predictor_rhs(dcur, dp, has_prev ? &dadv_prev : nullptr, dadv, nu, dt, false, drhs);
+  if(dforce) axpy_vel(drhs, *dforce, dt);
+  dpred = drhs;
+  predictor_adi(drhs, alpha, bcs, dpred);
+  dphi.fill(0.0);
+  const ProjDiag pd = project_velocity(dpred, bcs, o, dphi, dcorr, &dprhs, nullptr);
+  if(!pd.poisson.converged) throw std::runtime_error("tangent-linear pressure solve did not converge");
+  axpy_active(dp, dphi, 1.0);
+  axpy_active(dp, dprhs, -0.5 * nu * dt);
+  diffusion(dcorr, nu, dif);
+  apply_total_pressure_bc(bcs, dif, dp);
+  dvel = dcorr;
+  dadv_prev = dadv;
+  fractional_step(bcs, nu, dt, scheme, false, max_it, tol, velocity, adv_prev, p_total, has_prev, m, "Taylor-Green",
+                  false, force);
 }
 
 static double pi() { return std::acos(-1.0); }
@@ -1547,6 +1643,96 @@ int ofref_tg_steps(const TGConfig* c, int steps, int want_linf, double* u, doubl
     for(int s = 0; s < steps; ++s)
       fractional_step(b, c->viscosity, dt, 0, false, c->max_iterations, c->tolerance, vel, prev, pf, *has_prev, *m,
                       "Taylor-Green", want_linf != 0);
+  });
+}
+
+// Forward run with an optional body force, and optionally the tangent-linear model alongside
+// it. The initial state is prepared as the device adjoint driver prepares it: only active
+// entries of (u, v, w, p) and of the perturbation (du, dv, dw, dp) are read, ghosts are filled,
+// A_prev = 0. On return (u, v, w, p) and (du, dv, dw, dp) hold the final state and
+// perturbation; ke_out / dke_out the final kinetic energy and its directional derivative.
+int ofref_tg_run_tl(const TGConfig* c, int scheme, int steps, double* fu, double* fv, double* fw, double* u,
+                    double* v, double* w, double* p, double* dfu, double* dfv, double* dfw, double* du, double* dv,
+                    double* dw, double* dp, double* ke_out, double* dke_out) {
+  return guarded([&] {
+    if(steps < 0) throw std::invalid_argument("run_taylor_green_steps expects a non-negative step count");
+    const Grid g = tg_grid(*c);
+    const BCSet b = tg_bcs(*c);
+    const PBCSet pb = derive_pressure_correction_bc(b);
+    const double dt = tg_dt(*c);
+    Vel vel(g, u, v, w), prev(g);
+    Field pf(make_layout(g, kCell), p);
+    auto keep_active = [](Field& f) {
+      Field t(f.L);
+      copy_active(f, t);
+      f = t;
+    };
+    for(int a = 0; a < 3; ++a) keep_active(vel.c(a));
+    keep_active(pf);
+    apply_velocity_bc(b, vel);
+    apply_pressure_bc(pb, pf);
+    const bool forced = fu != nullptr;
+    Vel force(g);
+    if(forced) force = Vel(g, fu, fv, fw);
+    int has_prev = 0;
+    StepMetrics m{};
+    m.dt = dt;
+    const bool tl = du != nullptr;
+    Vel dvel(g), dprev(g), dforce(g);
+    Field dpf(make_layout(g, kCell));
+    if(tl) {
+      dvel = Vel(g, du, dv, dw);
+      for(int a = 0; a < 3; ++a) keep_active(dvel.c(a));
+      if(dp) {
+        dpf = Field(make_layout(g, kCell), dp);
+        keep_active(dpf);
+      }
+      apply_velocity_bc(b, dvel);
+      apply_pressure_bc(pb, dpf);
+      if(dfu) dforce = Vel(g, dfu, dfv, dfw);
+    }
+    for(int s = 0; s < steps; ++s) {
+      if(tl)
+        fractional_step_tl(b, c->viscosity, dt, scheme, c->max_iterations, c->tolerance, vel, prev, pf, has_prev, m,
+                           forced ? &force : nullptr, dvel, dprev, dpf, dfu ? &dforce : nullptr);
+      else
+        fractional_step(b, c->viscosity, dt, scheme, false, c->max_iterations, c->tolerance, vel, prev, pf, has_prev,
+                        m, "Taylor-Green", false, forced ? &force : nullptr);
+    }
+    if(ke_out) *ke_out = kinetic_energy(vel);
+    if(tl && dke_out) {
+      double s = 0.0;
+      size_t n = 0;
+      for_range(make_layout(g, kCell).active(), [&](int i, int j, int k) {
+        const double uc = 0.5 * (vel.x(i, j, k) + vel.x(i + 1, j, k));
+        const double vc = 0.5 * (vel.y(i, j, k) + vel.y(i, j + 1, k));
+        const double wc = 0.5 * (vel.z(i, j, k) + vel.z(i, j, k + 1));
+        const double duc = 0.5 * (dvel.x(i, j, k) + dvel.x(i + 1, j, k));
+        const double dvc = 0.5 * (dvel.y(i, j, k) + dvel.y(i, j + 1, k));
+        const double dwc = 0.5 * (dvel.z(i, j, k) + dvel.z(i, j, k + 1));
+        s += uc * duc + vc * dvc + wc * dwc;
+        ++n;
+      });
+      *dke_out = s / (double)n;
+    }
+    // (u, v, w, p) are updated in place through the views; the perturbation is copied back
+    if(tl) {
+      Vel dout(g, du, dv, dw);
+      dout = dvel;
+      if(dp) {
+        Field dpo(make_layout(g, kCell), dp);
+        dpo = dpf;
+      }
+    }
+  });
+}
+
+// Linearised advection at (u, v, w) along (du, dv, dw) (dual numbers; branches frozen).
+int ofref_advection_tangent(const Grid* g, int scheme, double* u, double* v, double* w, double* du, double* dv,
+                            double* dw, double* au, double* av, double* aw) {
+  return guarded([&] {
+    Vel vel(*g, u, v, w), dvel(*g, du, dv, dw), out(*g, au, av, aw);
+    advection_tangent(vel, dvel, scheme, out);
   });
 }
 

@@ -195,6 +195,15 @@ def advection(g, vel, scheme=0):
     return out
 
 
+def advection_tangent(g, vel, dvel, scheme=0):
+    """Linearised advection at vel along dvel, branches frozen at vel (dual numbers)."""
+    out = zeros_vel(g)
+    vel = [np.ascontiguousarray(a, dtype=np.float64).copy() for a in vel]
+    dvel = [np.ascontiguousarray(a, dtype=np.float64).copy() for a in dvel]
+    _call("ofref_advection_tangent", C.byref(g), C.c_int(scheme), *map(_p, vel), *map(_p, dvel), *map(_p, out))
+    return out
+
+
 def diffusion(g, nu, vel):
     out = zeros_vel(g)
     _call("ofref_diffusion", C.byref(g), C.c_double(nu), *map(_p, vel), *map(_p, out))
@@ -330,6 +339,31 @@ def tg_steps(c: TGConfig, s: State, steps: int, want_linf=False):
     _call("ofref_tg_steps", C.byref(c), C.c_int(steps), C.c_int(int(want_linf)), *map(_p, s.vel),
           _p(s.p), *map(_p, s.adv), C.byref(s.has_prev), C.byref(s.metrics))
     return s
+
+
+def tg_run_tl(c: TGConfig, steps, vel, p=None, force=None, dvel=None, dp=None, dforce=None, scheme=0):
+    """Runs `steps` steps from (vel, p) with an optional body force, and the tangent-linear model
+    along (dvel, dp, dforce) when dvel is given. Only active entries of the inputs are read.
+    Returns (final velocity, final pressure, final kinetic energy, final velocity perturbation
+    or None, directional derivative of the final kinetic energy or None)."""
+    g = tg_grid(c)
+
+    def own(a, loc):
+        return zeros(g, loc) if a is None else np.ascontiguousarray(a, dtype=np.float64).copy()
+
+    u = [own(a, loc) for a, loc in zip(vel, (FACE_X, FACE_Y, FACE_Z))]
+    pp = own(p, CELL)
+    fu = [own(a, loc) for a, loc in zip(force, (FACE_X, FACE_Y, FACE_Z))] if force is not None else None
+    tl = dvel is not None
+    du = [own(a, loc) for a, loc in zip(dvel, (FACE_X, FACE_Y, FACE_Z))] if tl else None
+    dpp = own(dp, CELL) if tl else None
+    dfu = [own(a, loc) for a, loc in zip(dforce, (FACE_X, FACE_Y, FACE_Z))] if dforce is not None else None
+    none3 = [None, None, None]
+    ke, dke = C.c_double(0), C.c_double(0)
+    _call("ofref_tg_run_tl", C.byref(c), C.c_int(scheme), C.c_int(steps),
+          *(map(_p, fu) if fu else none3), *map(_p, u), _p(pp), *(map(_p, dfu) if dfu else none3),
+          *(map(_p, du) if tl else none3), _p(dpp) if tl else None, C.byref(ke), C.byref(dke))
+    return u, pp, ke.value, du, (dke.value if tl else None)
 
 
 def kinetic_energy(g, vel):

@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsolver_amd.so")
@@ -110,6 +111,12 @@ EXPORTS = [
     "of_apply_poisson_operator", "of_taylor_green_dt", "of_lid_driven_cavity_dt", "of_stepper_create",
     "of_stepper_destroy", "of_stepper_initialize", "of_stepper_upload", "of_stepper_download",
     "of_stepper_device_fields", "of_stepper_run", "of_stepper_run_batched", "of_stepper_kinetic_energy",
+    "of_stepper_set_forcing",
+    "of_apply_velocity_boundary_conditions_adjoint", "of_apply_pressure_boundary_conditions_adjoint",
+    "of_compute_gradient_adjoint", "of_compute_divergence_adjoint", "of_solve_predictor_adi_adjoint",
+    "of_compute_advection_tangent", "of_compute_advection_adjoint",
+    "of_adjoint_create", "of_adjoint_destroy", "of_adjoint_set_forcing", "of_adjoint_forward",
+    "of_adjoint_final_velocity", "of_adjoint_backward", "of_adjoint_tangent",
 ]
 
 
@@ -231,6 +238,9 @@ class Context:
         if not torch.cuda.is_available():
             raise RuntimeError("solver_amd needs a HIP device")
         self.device = device
+        # objects whose native state refers to this context (steppers, adjoint runs); they are
+        # closed before the context, whatever order the garbage collector finalises them in
+        self._dependents = weakref.WeakSet()
         h = C.c_void_p()
         s = C.c_void_p(stream) if stream else C.c_void_p(None)
         call("of_context_create", device, s, C.byref(h))
@@ -238,6 +248,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for d in list(self._dependents):
+                d.close()
             call("of_context_destroy", self.h)
             self.h = None
 
@@ -277,6 +289,7 @@ class Stepper:
         h = C.c_void_p()
         call("of_stepper_create", ctx.h, C.byref(cfg), C.byref(h))
         self.h = h
+        ctx._dependents.add(self)
         self.nx, self.ny, self.nz = nx, ny, nz
 
     def close(self):
@@ -335,3 +348,83 @@ class Stepper:
         arr = (C.c_void_p * 7)()
         call("of_stepper_device_fields", self.h, arr)
         return [arr[i] for i in range(7)]
+
+    def set_forcing(self, f=None):
+        """Body force (three float64 device tensors in the velocity layout, or None)."""
+        _torch().cuda.synchronize()
+        fs = [None] * 3 if f is None else list(f)
+        call("of_stepper_set_forcing", self.h, *map(ptr, fs))
+        self.ctx.sync()
+
+
+class Adjoint:
+    """Checkpointed discrete adjoint and tangent-linear model of the periodic 3D Taylor-Green
+    step (device tensors in the reference storage layout)."""
+
+    def __init__(self, ctx: Context, nx, ny, nz, viscosity, dt, checkpoint_every, poisson_max_iterations=400,
+                 poisson_tolerance=1e-10, precision=FP64, scheme=0):
+        self.ctx = ctx
+        self.cfg = StepperConfig(TAYLOR_GREEN, nx, ny, nz, viscosity, dt, scheme, poisson_max_iterations,
+                                 poisson_tolerance, precision, 0.0)
+        h = C.c_void_p()
+        call("of_adjoint_create", ctx.h, C.byref(self.cfg), int(checkpoint_every), C.byref(h))
+        self.h = h
+        ctx._dependents.add(self)
+        self.shape = (nx, ny, nz)
+
+    def close(self):
+        if self.h:
+            call("of_adjoint_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _new(self, loc):
+        torch = _torch()
+        return torch.zeros(storage_shape(*self.shape, loc), dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+
+    def set_forcing(self, f=None):
+        _torch().cuda.synchronize()
+        fs = [None] * 3 if f is None else list(f)
+        call("of_adjoint_set_forcing", self.h, *map(ptr, fs))
+        self.ctx.sync()
+
+    def forward(self, vel, p=None, steps=1):
+        """Runs `steps` steps from vel = (u, v, w) and p; returns (final kinetic energy, metrics)."""
+        _torch().cuda.synchronize()
+        ke = C.c_double()
+        m = StepMetrics()
+        call("of_adjoint_forward", self.h, *map(ptr, vel), ptr(p), int(steps), C.byref(ke), C.byref(m))
+        return ke.value, m
+
+    def final_velocity(self):
+        out = [self._new(loc) for loc in (FACE_X, FACE_Y, FACE_Z)]
+        call("of_adjoint_final_velocity", self.h, *map(ptr, out))
+        return out
+
+    def backward(self, seed=None, want_pressure=True, want_forcing=True):
+        """Gradients w.r.t. (u0, v0, w0), p0 and the forcing field of <seed, u^N> (seed None: of
+        the final kinetic energy)."""
+        torch = _torch()
+        torch.cuda.synchronize()
+        gu = [self._new(loc) for loc in (FACE_X, FACE_Y, FACE_Z)]
+        gp = self._new(CELL) if want_pressure else None
+        gf = [self._new(loc) for loc in (FACE_X, FACE_Y, FACE_Z)] if want_forcing else [None] * 3
+        sd = [None] * 3 if seed is None else [s.contiguous() for s in seed]
+        call("of_adjoint_backward", self.h, *map(ptr, sd), *map(ptr, gu), ptr(gp), *map(ptr, gf))
+        return gu, gp, (gf if want_forcing else None)
+
+    def tangent(self, dvel=None, dp=None, dforce=None):
+        """Tangent-linear run from the last forward pass's initial state: returns the final velocity
+        perturbation and the directional derivative of the final kinetic energy."""
+        _torch().cuda.synchronize()
+        dv = [None] * 3 if dvel is None else list(dvel)
+        df = [None] * 3 if dforce is None else list(dforce)
+        out = [self._new(loc) for loc in (FACE_X, FACE_Y, FACE_Z)]
+        dke = C.c_double()
+        call("of_adjoint_tangent", self.h, *map(ptr, dv), ptr(dp), *map(ptr, df), *map(ptr, out), C.byref(dke))
+        return out, dke.value

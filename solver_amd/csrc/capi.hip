@@ -2,6 +2,7 @@
 // reference interface, translating C++ exceptions into status codes plus a thread-local
 // message that carries the reference's exception text.
 
+#include "adjoint.hpp"
 #include "stepper.hpp"
 
 #include <cmath>
@@ -48,6 +49,7 @@ Context::~Context() {
 }
 
 bool axis_periodic(const of_bc_set& b, int axis);
+bool pressure_axis_periodic(const of_pressure_bc_set& b, int axis);
 
 }  // namespace ofh
 
@@ -55,6 +57,11 @@ using namespace ofh;
 
 struct of_context {
   Context ctx;
+};
+
+struct of_adjoint {
+  AdjointRun* a;
+  int device;
 };
 struct of_stepper {
   Stepper* s;
@@ -111,6 +118,18 @@ void check_projection_options(const of_projection_options* o, const char* fn) {
 }
 
 size_t fsize(const of_grid& g, int loc) { return make_lay(g, loc).size() * sizeof(double); }
+
+void need_periodic(const of_bc_set* b) {
+  need(b, "bcs");
+  for(int a = 0; a < 3; ++a)
+    if(!axis_periodic(*b, a)) throw InvalidArgument("adjoint boundary conditions support periodic faces only");
+}
+
+AdjointRun& A(of_adjoint* a) {
+  need(a, "adjoint");
+  OFH_CHECK(hipSetDevice(a->device));
+  return *a->a;
+}
 
 }  // namespace
 
@@ -622,6 +641,192 @@ int of_stepper_kinetic_energy(of_stepper* s, double* out) {
     need(out, "out");
     OFH_CHECK(hipSetDevice(s->s->c.device));
     *out = s->s->kinetic_energy();
+  });
+}
+
+int of_stepper_set_forcing(of_stepper* s, const double* fu, const double* fv, const double* fw) {
+  return guarded([&] {
+    need(s, "stepper");
+    OFH_CHECK(hipSetDevice(s->s->c.device));
+    const double* f[3] = {fu, fv, fw};
+    s->s->set_forcing(fu || fv || fw ? f : nullptr);
+  });
+}
+
+// ------------------------------------------------------------------ discrete adjoint
+int of_apply_velocity_boundary_conditions_adjoint(of_context* c, const of_grid* g, const of_bc_set* b, double* u,
+                                                  double* v, double* w) {
+  return guarded([&] {
+    validate_grid(g);
+    need_periodic(b);
+    need(u, "u"), need(v, "v"), need(w, "w");
+    double* f[3] = {u, v, w};
+    launch_velocity_bc_adjoint(C(c), *g, f);
+    OFH_CHECK(hipGetLastError());
+  });
+}
+
+int of_apply_pressure_boundary_conditions_adjoint(of_context* c, const of_grid* g, const of_pressure_bc_set* b,
+                                                  double* p) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    for(int a = 0; a < 3; ++a)
+      if(!pressure_axis_periodic(*b, a)) throw InvalidArgument("adjoint boundary conditions support periodic faces only");
+    need(p, "p");
+    launch_pressure_bc_adjoint(C(c), *g, p);
+    OFH_CHECK(hipGetLastError());
+  });
+}
+
+int of_compute_gradient_adjoint(of_context* c, const of_grid* g, const double* gx, const double* gy, const double* gz,
+                                double* p) {
+  return guarded([&] {
+    validate_grid(g);
+    need(gx, "gx"), need(gy, "gy"), need(gz, "gz"), need(p, "p");
+    Context& x = C(c);
+    OFH_CHECK(hipMemsetAsync(p, 0, fsize(*g, OF_CELL), x.stream));
+    const double* gb[3] = {gx, gy, gz};
+    launch_gradient_adjoint(x, *g, 1.0, gb, p);
+    OFH_CHECK(hipGetLastError());
+  });
+}
+
+int of_compute_divergence_adjoint(of_context* c, const of_grid* g, const double* d, double* u, double* v, double* w) {
+  return guarded([&] {
+    validate_grid(g);
+    need(d, "d"), need(u, "u"), need(v, "v"), need(w, "w");
+    Context& x = C(c);
+    double* ub[3] = {u, v, w};
+    for(int a = 0; a < 3; ++a) OFH_CHECK(hipMemsetAsync(ub[a], 0, fsize(*g, OF_FACE_X + a), x.stream));
+    launch_divergence_adjoint(x, *g, 1.0, d, ub);
+    OFH_CHECK(hipGetLastError());
+  });
+}
+
+int of_solve_predictor_adi_adjoint(of_context* c, const of_grid* g, const of_bc_set* b, double alpha, const double* yu,
+                                   const double* yv, const double* yw, double* ru, double* rv, double* rw) {
+  return guarded([&] {
+    validate_grid(g);
+    need_periodic(b);
+    need(yu, "yu"), need(yv, "yv"), need(yw, "yw"), need(ru, "ru"), need(rv, "rv"), need(rw, "rw");
+    if(alpha < 0.0) throw InvalidArgument("solve_predictor_adi expects a non-negative alpha");
+    const double* y[3] = {yu, yv, yw};
+    double* r[3] = {ru, rv, rw};
+    predictor_adi_adjoint(C(c), *g, *b, alpha, y, r);
+  });
+}
+
+int of_compute_advection_tangent(of_context* c, const of_grid* g, int scheme, const double* u, const double* v,
+                                 const double* w, const double* du, const double* dv, const double* dw, double* au,
+                                 double* av, double* aw) {
+  return guarded([&] {
+    validate_grid(g);
+    if(scheme < 0 || scheme > 2) throw InvalidArgument("unknown advection scheme");
+    need(u, "u"), need(v, "v"), need(w, "w"), need(du, "du"), need(dv, "dv"), need(dw, "dw");
+    need(au, "au"), need(av, "av"), need(aw, "aw");
+    Context& x = C(c);
+    double* out[3] = {au, av, aw};
+    for(int a = 0; a < 3; ++a) OFH_CHECK(hipMemsetAsync(out[a], 0, fsize(*g, OF_FACE_X + a), x.stream));
+    const double* vel[3] = {u, v, w};
+    const double* dvel[3] = {du, dv, dw};
+    launch_advection_tangent(x, *g, scheme, vel, dvel, out);
+    OFH_CHECK(hipGetLastError());
+  });
+}
+
+int of_compute_advection_adjoint(of_context* c, const of_grid* g, int scheme, const double* u, const double* v,
+                                 const double* w, const double* bu, const double* bv, const double* bw, double* ubar,
+                                 double* vbar, double* wbar) {
+  return guarded([&] {
+    validate_grid(g);
+    if(scheme < 0 || scheme > 2) throw InvalidArgument("unknown advection scheme");
+    need(u, "u"), need(v, "v"), need(w, "w"), need(bu, "abar_u"), need(bv, "abar_v"), need(bw, "abar_w");
+    need(ubar, "ubar"), need(vbar, "vbar"), need(wbar, "wbar");
+    Context& x = C(c);
+    double* out[3] = {ubar, vbar, wbar};
+    for(int a = 0; a < 3; ++a) OFH_CHECK(hipMemsetAsync(out[a], 0, fsize(*g, OF_FACE_X + a), x.stream));
+    const double* vel[3] = {u, v, w};
+    const double* ab[3] = {bu, bv, bw};
+    launch_advection_adjoint(x, *g, scheme, vel, ab, out);
+    OFH_CHECK(hipGetLastError());
+  });
+}
+
+int of_adjoint_create(of_context* c, const of_stepper_config* cfg, int every, of_adjoint** out) {
+  return guarded([&] {
+    need(cfg, "config");
+    need(out, "out");
+    Context& x = C(c);
+    auto* h = new of_adjoint{nullptr, x.device};
+    try {
+      h->a = adjoint_create(x, *cfg, every);
+    } catch(...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int of_adjoint_destroy(of_adjoint* a) {
+  return guarded([&] {
+    if(!a) return;
+    (void)hipSetDevice(a->device);
+    adjoint_destroy(a->a);
+    delete a;
+  });
+}
+
+int of_adjoint_set_forcing(of_adjoint* a, const double* fu, const double* fv, const double* fw) {
+  return guarded([&] {
+    AdjointRun& r = A(a);
+    const double* f[3] = {fu, fv, fw};
+    adjoint_set_forcing(&r, fu || fv || fw ? f : nullptr);
+  });
+}
+
+int of_adjoint_forward(of_adjoint* a, const double* u0, const double* v0, const double* w0, const double* p0,
+                       int steps, double* ke, of_step_metrics* m) {
+  return guarded([&] {
+    AdjointRun& r = A(a);
+    const double* u[3] = {u0, v0, w0};
+    const double e = adjoint_forward(&r, u, p0, steps, m);
+    if(ke) *ke = e;
+  });
+}
+
+int of_adjoint_final_velocity(of_adjoint* a, double* u, double* v, double* w) {
+  return guarded([&] {
+    AdjointRun& r = A(a);
+    double* out[3] = {u, v, w};
+    adjoint_final_velocity(&r, out);
+  });
+}
+
+int of_adjoint_backward(of_adjoint* a, const double* su, const double* sv, const double* sw, double* gu, double* gv,
+                        double* gw, double* gp, double* gfu, double* gfv, double* gfw) {
+  return guarded([&] {
+    AdjointRun& r = A(a);
+    if((su || sv || sw) && !(su && sv && sw)) throw InvalidArgument("the seed needs all three components or none");
+    const double* seed[3] = {su, sv, sw};
+    double* g[3] = {gu, gv, gw};
+    double* gf[3] = {gfu, gfv, gfw};
+    adjoint_backward(&r, su ? seed : nullptr, g, gp, gf);
+  });
+}
+
+int of_adjoint_tangent(of_adjoint* a, const double* du0, const double* dv0, const double* dw0, const double* dp0,
+                       const double* dfu, const double* dfv, const double* dfw, double* du, double* dv, double* dw,
+                       double* dke) {
+  return guarded([&] {
+    AdjointRun& r = A(a);
+    if((dfu || dfv || dfw) && !(dfu && dfv && dfw))
+      throw InvalidArgument("the forcing direction needs all three components or none");
+    const double* d0[3] = {du0, dv0, dw0};
+    const double* df[3] = {dfu, dfv, dfw};
+    double* out[3] = {du, dv, dw};
+    adjoint_tangent(&r, d0, dp0, dfu ? df : nullptr, out, dke);
   });
 }
 

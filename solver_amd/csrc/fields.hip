@@ -610,6 +610,14 @@ __global__ void k_scale_active(const double* in, double* out, Lay L, double s, i
   for(int k = kb; k < ke; ++k) out[L.at(i, j, k)] = in[L.at(i, j, k)] * s;
 }
 
+__global__ void k_axpy_active(const double* __restrict__ x, double* __restrict__ y, Lay L, double a, int kz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > L.ax || j > L.ay) return;
+  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + L.az);
+  for(int k = kb; k < ke; ++k) y[L.at(i, j, k)] = y[L.at(i, j, k)] + a * x[L.at(i, j, k)];
+}
+
 // solver/projection.cpp:1082-1093: out = pred + scale * (phi(i) - phi(i-1))/h on every
 // active face (scale = -dt/density); phi carries ghosts.
 __global__ void k_correct(const double* __restrict__ pred, Lay L, const double* __restrict__ phi, Lay Lp, int axis,
@@ -798,6 +806,12 @@ void launch_norms(Context& c, const Lay& L, const double* x, int slot) {
 void launch_scale_active(Context& c, const Lay& L, double scale, const double* in, double* out) {
   const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
   hipLaunchKernelGGL(k_scale_active, cl.grid, cl.block, 0, c.stream, in, out, L, scale, cl.kz);
+  OFH_CHECK(hipGetLastError());
+}
+
+void launch_axpy_active(Context& c, const Lay& L, double a, const double* x, double* y) {
+  const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
+  hipLaunchKernelGGL(k_axpy_active, cl.grid, cl.block, 0, c.stream, x, y, L, a, cl.kz);
   OFH_CHECK(hipGetLastError());
 }
 

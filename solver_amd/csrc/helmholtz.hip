@@ -12,11 +12,14 @@
 #include "ofh.hpp"
 
 #include <cmath>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 namespace ofh {
 
 void launch_component_bc(Context& c, const of_grid& g, const of_bc_set& b, int comp, double* f);
+void launch_component_bc_adjoint(Context& c, const of_grid& g, int comp, double* f);
 bool axis_periodic(const of_bc_set& b, int axis);
 
 namespace {
@@ -343,26 +346,28 @@ void tangential_relation(const of_bc& c, int comp, double& coef, double& cst) {
 
 }  // namespace
 
-// Returns the reference's line_solves count.
-int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* ru, const double* rv,
-                  const double* rw, double* ou, double* ov, double* ow) {
-  if(alpha < 0.0) throw InvalidArgument("solve_predictor_adi expects a non-negative alpha");
-  const double* rhs[3] = {ru, rv, rw};
-  double* out[3] = {ou, ov, ow};
-  int line_solves = 0;
-  const int nmax = std::max(g.nx, std::max(g.ny, g.nz)) + 2;
-  double* dev = static_cast<double*>(c.scratch_get(7, sizeof(double) * 3 * nmax * 9));
-  std::vector<double> host(3 * (size_t)nmax * 9, 0.0);
-  struct Pending {
-    int comp, axis;
-    LineSys S;
-    size_t off;
-  };
+namespace {
+
+struct Pending {
+  int comp, axis;
+  LineSys S;
+  size_t off;
+};
+
+// Line systems of every (component, axis) sweep with their Thomas / Sherman-Morrison factors,
+// computed on the host with the reference's recurrences and staged in a device buffer.
+struct AdiPlan {
   std::vector<Pending> sweeps;
-  for(int comp = 0; comp < 3; ++comp) {
-    const Lay L = make_lay(g, OF_FACE_X + comp);
-    OFH_CHECK(hipMemcpyAsync(out[comp], rhs[comp], L.size() * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  }
+  double* dev = nullptr;
+  int nmax = 0;
+  int line_solves = 0;
+};
+
+AdiPlan build_adi_plan(const of_grid& g, const of_bc_set& b, double alpha) {
+  AdiPlan P;
+  P.nmax = std::max(g.nx, std::max(g.ny, g.nz)) + 2;
+  const int nmax = P.nmax;
+  std::vector<double> host(3 * (size_t)nmax * 9, 0.0);
   for(int comp = 0; comp < 3; ++comp) {
     const Lay L = make_lay(g, OF_FACE_X + comp);
     for(int axis = 0; axis < 3; ++axis) {
@@ -457,38 +462,158 @@ int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha
       std::copy(mu.begin(), mu.end(), host.begin() + off + nmax);
       std::copy(zv.begin(), zv.end(), host.begin() + off + 2 * nmax);
       const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
-      line_solves += L.a(o0) * L.a(o1);
-      sweeps.push_back(Pending{comp, axis, S, off});
+      P.line_solves += L.a(o0) * L.a(o1);
+      P.sweeps.push_back(Pending{comp, axis, S, off});
     }
   }
-  OFH_CHECK(hipMemcpyAsync(dev, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
-  // hipMemcpyAsync from pageable memory returns after the copy has been staged, so `host`
-  // may be released when this function returns.
+  // a fresh buffer, filled by a blocking copy before any launch can read it
+  OFH_CHECK(hipMalloc(&P.dev, host.size() * sizeof(double)));
+  OFH_CHECK(hipMemcpy(P.dev, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+  return P;
+}
+
+// Plans depend only on (device, grid, boundary conditions, alpha); a stepper reuses one plan
+// for every step, so the factors are built and uploaded once instead of once per solve.
+struct AdiCacheEntry {
+  int device;
+  of_grid g;
+  of_bc_set b;
+  double alpha;
+  AdiPlan plan;
+};
+constexpr size_t kAdiCacheCap = 16;
+std::mutex g_adi_mutex;
+std::vector<std::unique_ptr<AdiCacheEntry>> g_adi_cache;  // most recently used last
+
+bool same_plan_key(const AdiCacheEntry& e, int device, const of_grid& g, const of_bc_set& b, double alpha) {
+  if(e.device != device || e.alpha != alpha) return false;
+  if(e.g.nx != g.nx || e.g.ny != g.ny || e.g.nz != g.nz || e.g.dx != g.dx || e.g.dy != g.dy || e.g.dz != g.dz)
+    return false;
+  for(int f = 0; f < 6; ++f) {
+    const of_bc &x = e.b.faces[f], &y = b.faces[f];
+    if(x.type != y.type) return false;
+    for(int k = 0; k < 3; ++k)
+      if(x.velocity[k] != y.velocity[k]) return false;
+  }
+  return true;
+}
+
+// Returns a copy of the cached plan (the sweep list and the device address of its factors).
+AdiPlan plan_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha) {
+  std::lock_guard<std::mutex> lock(g_adi_mutex);
+  for(size_t i = 0; i < g_adi_cache.size(); ++i) {
+    if(!same_plan_key(*g_adi_cache[i], c.device, g, b, alpha)) continue;
+    std::unique_ptr<AdiCacheEntry> e = std::move(g_adi_cache[i]);
+    g_adi_cache.erase(g_adi_cache.begin() + (ptrdiff_t)i);
+    g_adi_cache.push_back(std::move(e));
+    return g_adi_cache.back()->plan;
+  }
+  if(g_adi_cache.size() >= kAdiCacheCap) {
+    // the least recently used plan may still be read by queued line solves on any stream
+    OFH_CHECK(hipDeviceSynchronize());
+    OFH_CHECK(hipFree(g_adi_cache.front()->plan.dev));
+    g_adi_cache.erase(g_adi_cache.begin());
+  }
+  auto e = std::make_unique<AdiCacheEntry>();
+  e->device = c.device;
+  e->g = g;
+  e->b = b;
+  e->alpha = alpha;
+  e->plan = build_adi_plan(g, b, alpha);
+  g_adi_cache.push_back(std::move(e));
+  return g_adi_cache.back()->plan;
+}
+
+void launch_sweep(Context& c, const Lay& L, const Pending& p, const AdiPlan& P, double* f, const LineSys& S) {
+  const int o1 = p.axis == 2 ? 1 : 2;
+  const double* piv = P.dev + p.off;
+  const double* mu = P.dev + p.off + P.nmax;
+  const double* zv = P.dev + p.off + 2 * P.nmax;
+  if(p.axis == 0) {
+    ProfScope prof_(c, "k_line_solve_x", true);
+    hipLaunchKernelGGL(k_line_solve_x, dim3(div_up((size_t)L.ay * L.az, kLT)), dim3(kLT), 0, c.stream, f, L, S, piv,
+                       mu, zv);
+  } else {
+    ProfScope prof_(c, "k_line_solve_s", true);
+    const dim3 grid((unsigned)((L.ax + 63) / 64), (unsigned)((L.a(o1) + 3) / 4), 1);
+    hipLaunchKernelGGL(k_line_solve_s, grid, dim3(64, 4, 1), 0, c.stream, f, L, p.axis, S, piv, mu, zv);
+  }
+}
+
+// Transpose of the duplicated-face copy of a periodic normal sweep (x[e] := x[b]):
+// x_bar[b] += x_bar[e], x_bar[e] = 0, for every line of the component along `axis`.
+__global__ void k_dup_fold(double* f, Lay L, int axis, int b, int e) {
+  const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t1 = blockIdx.y * blockDim.y + threadIdx.y;
+  if(t0 >= L.a(o0) || t1 >= L.a(o1)) return;
+  int cb[3], ce[3];
+  cb[axis] = b;
+  ce[axis] = e;
+  cb[o0] = ce[o0] = 1 + t0;
+  cb[o1] = ce[o1] = 1 + t1;
+  const size_t ib = L.at(cb[0], cb[1], cb[2]), ie = L.at(ce[0], ce[1], ce[2]);
+  f[ib] = f[ib] + f[ie];
+  f[ie] = 0.0;
+}
+
+}  // namespace
+
+// Returns the reference's line_solves count.
+int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* ru, const double* rv,
+                  const double* rw, double* ou, double* ov, double* ow) {
+  if(alpha < 0.0) throw InvalidArgument("solve_predictor_adi expects a non-negative alpha");
+  const double* rhs[3] = {ru, rv, rw};
+  double* out[3] = {ou, ov, ow};
+  const AdiPlan P = plan_adi(c, g, b, alpha);
+  for(int comp = 0; comp < 3; ++comp) {
+    const Lay L = make_lay(g, OF_FACE_X + comp);
+    OFH_CHECK(hipMemcpyAsync(out[comp], rhs[comp], L.size() * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  }
   for(int comp = 0; comp < 3; ++comp) {
     launch_component_bc(c, g, b, comp, out[comp]);
     if(alpha == 0.0) continue;
     const Lay L = make_lay(g, OF_FACE_X + comp);
-    for(const Pending& p : sweeps) {
+    for(const Pending& p : P.sweeps) {
       if(p.comp != comp) continue;
-      const int o1 = p.axis == 2 ? 1 : 2;
-      const double* piv = dev + p.off;
-      const double* mu = dev + p.off + nmax;
-      const double* zv = dev + p.off + 2 * nmax;
-      if(p.axis == 0) {
-        ProfScope prof_(c, "k_line_solve_x", true);
-        hipLaunchKernelGGL(k_line_solve_x, dim3(div_up((size_t)L.ay * L.az, kLT)), dim3(kLT), 0, c.stream, out[comp], L,
-                           p.S, piv, mu, zv);
-      } else {
-        ProfScope prof_(c, "k_line_solve_s", true);
-        const dim3 grid((unsigned)((L.ax + 63) / 64), (unsigned)((L.a(o1) + 3) / 4), 1);
-        hipLaunchKernelGGL(k_line_solve_s, grid, dim3(64, 4, 1), 0, c.stream, out[comp], L, p.axis, p.S, piv, mu, zv);
-      }
+      launch_sweep(c, L, p, P, out[comp], p.S);
       launch_component_bc(c, g, b, comp, out[comp]);
     }
   }
   OFH_CHECK(hipGetLastError());
-  OFH_CHECK(hipStreamSynchronize(c.stream));  // `host` staging buffer lifetime
-  return alpha == 0.0 ? 0 : line_solves;
+  return alpha == 0.0 ? 0 : P.line_solves;
+}
+
+// Transpose of predictor_adi as a linear map rhs -> out (periodic boundaries): the sweeps in
+// reverse order with the transposed ghost fill between them; each cyclic line matrix is
+// symmetric, so its transposed solve is the same solve once the duplicated face is folded back.
+void predictor_adi_adjoint(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* const* ybar,
+                           double* const* rbar) {
+  for(int a = 0; a < 3; ++a)
+    if(!axis_periodic(b, a)) throw InvalidArgument("the ADI adjoint supports periodic boundaries");
+  const AdiPlan P = plan_adi(c, g, b, alpha);
+  for(int comp = 0; comp < 3; ++comp) {
+    const Lay L = make_lay(g, OF_FACE_X + comp);
+    if(rbar[comp] != ybar[comp])
+      OFH_CHECK(hipMemcpyAsync(rbar[comp], ybar[comp], L.size() * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    launch_component_bc_adjoint(c, g, comp, rbar[comp]);
+    if(alpha == 0.0) continue;
+    for(auto it = P.sweeps.rbegin(); it != P.sweeps.rend(); ++it) {
+      const Pending& p = *it;
+      if(p.comp != comp) continue;
+      LineSys S = p.S;
+      if(S.copy_dup) {
+        const int o0 = p.axis == 0 ? 1 : 0, o1 = p.axis == 2 ? 1 : 2;
+        const dim3 block(64, 4, 1);
+        const dim3 grid((unsigned)((L.a(o0) + 63) / 64), (unsigned)((L.a(o1) + 3) / 4), 1);
+        hipLaunchKernelGGL(k_dup_fold, grid, block, 0, c.stream, rbar[comp], L, p.axis, S.b, S.e);
+        S.copy_dup = 0;
+      }
+      launch_sweep(c, L, p, P, rbar[comp], S);
+      launch_component_bc_adjoint(c, g, comp, rbar[comp]);
+    }
+  }
+  OFH_CHECK(hipGetLastError());
 }
 
 }  // namespace ofh

@@ -81,10 +81,15 @@ void launch_predictor_rhs(Context& c, const of_grid& g, int variant, int scheme,
 void launch_norms(Context& c, const Lay& L, const double* x, int slot);
 // scaled copy of the active range: out = scale * in
 void launch_scale_active(Context& c, const Lay& L, double scale, const double* in, double* out);
+// y = y + a * x on the active range
+void launch_axpy_active(Context& c, const Lay& L, double a, const double* x, double* y);
 
 // ------------------------------------------------------------------ helmholtz.hip
 int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* ru,
                   const double* rv, const double* rw, double* ou, double* ov, double* ow);
+// Transpose of predictor_adi (rhs -> out) for periodic boundaries; rbar may alias ybar.
+void predictor_adi_adjoint(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* const* ybar,
+                           double* const* rbar);
 
 // ------------------------------------------------------------------ poisson.hip
 struct PoissonResult {

@@ -215,6 +215,24 @@ void Stepper::total_pressure_bc(const double* const* v) {
   launch_total_pressure_bc(c, g, bcs, dif[0], dif[1], dif[2], p_total);
 }
 
+void Stepper::set_forcing(const double* const* f) {
+  if(f == nullptr || f[0] == nullptr) {
+    for(int a = 0; a < 3; ++a) force[a] = nullptr;  // buffers stay owned until destruction
+    return;
+  }
+  if(cfg.flow_case != OF_CASE_TAYLOR_GREEN) throw InvalidArgument("a body force is supported for Taylor-Green only");
+  for(int a = 0; a < 3; ++a) {
+    if(f[a] == nullptr) throw InvalidArgument("forcing needs all three components");
+    const size_t n = make_lay(g, OF_FACE_X + a).size();
+    if(force_buf[a] == nullptr) {
+      OFH_CHECK(hipMalloc(&force_buf[a], n * sizeof(double)));
+      owned.push_back(force_buf[a]);
+    }
+    OFH_CHECK(hipMemcpyAsync(force_buf[a], f[a], n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    force[a] = force_buf[a];
+  }
+}
+
 void Stepper::initialize(int convention) {
   const int three_d = g.nz > 1;
   for(int a = 0; a < 3; ++a) OFH_CHECK(hipMemsetAsync(adv[a], 0, make_lay(g, OF_FACE_X + a).size() * 8, c.stream));
@@ -247,6 +265,9 @@ void Stepper::step() {
   launch_predictor_rhs(c, g, cfg.flow_case == OF_CASE_TAYLOR_GREEN ? 0 : 1, cfg.scheme, nu, dt, vel[0], vel[1], vel[2],
                        p_total, has_prev ? adv[0] : nullptr, has_prev ? adv[1] : nullptr,
                        has_prev ? adv[2] : nullptr, adv[0], adv[1], adv[2], rhs[0], rhs[1], rhs[2]);
+  // body force, added after the reference's terms: rhs += dt f
+  if(force[0])
+    for(int a = 0; a < 3; ++a) launch_axpy_active(c, make_lay(g, OF_FACE_X + a), dt, force[a], rhs[a]);
   // Helmholtz predictor (exact line solves); output carries BC ghosts
   predictor_adi(c, g, bcs, 0.5 * nu * dt, rhs[0], rhs[1], rhs[2], pred[0], pred[1], pred[2]);
   // projection (solver/projection.cpp:1097-1131); pred already carries its BC ghosts

@@ -25,6 +25,8 @@ struct Stepper {
   double* phi = nullptr;
   double* prhs = nullptr;
   double* diag = nullptr;
+  double* force[3]{};  // optional body force f: rhs += dt f on the active range (null: none)
+  double* force_buf[3]{};
   PoissonSolver* poisson = nullptr;
   int has_prev = 0;
   of_step_metrics metrics{};
@@ -36,6 +38,9 @@ struct Stepper {
   void step();
   double kinetic_energy();
   void total_pressure_bc(const double* const* v);
+  // Copies a device forcing field (three face arrays in the velocity layout) into the stepper;
+  // null clears it.
+  void set_forcing(const double* const* f);
 };
 
 }  // namespace ofh
