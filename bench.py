@@ -50,6 +50,8 @@ def parse():
                     help="adjoint workload timed beside the forward step (c5: 512^3, about 160 GB of HBM)")
     ap.add_argument("--adjoint-fd", type=int, default=0,
                     help="central finite-difference checks of the adjoint gradient along this many seeded directions")
+    ap.add_argument("--adjoint-tl", type=int, default=0,
+                    help="adjoint-versus-tangent-linear checks along this many seeded directions")
     ap.add_argument("--adjoint-only", action="store_true", help="skip the forward benchmark")
     return ap.parse_args()
 
@@ -64,7 +66,7 @@ ADJOINT_CASES = {
 }
 
 
-def run_adjoint(sa, ctx, name, fd_dirs):
+def run_adjoint(sa, ctx, name, fd_dirs, tl_dirs=0):
     """Forward run with checkpoints, then the reverse sweep (window recompute + adjoint steps).
     Initial state: the Taylor-Green field plus the seeded divergence-free perturbation of RMS
     1e-2. Optional central finite differences of the final kinetic energy along seeded
@@ -91,9 +93,11 @@ def run_adjoint(sa, ctx, name, fd_dirs):
     if cs["forced"]:
         adj.set_forcing([torch.from_numpy(f).cuda() for f in inputs.forcing(n, ADJOINT_SEED, 1e-2)])
     torch.cuda.synchronize()
+    print(f"# adjoint {name}: inputs ready, forward pass of {steps} steps", flush=True)
     t0 = time.perf_counter()
     ke, m = adj.forward(u0, p0, steps)
     t1 = time.perf_counter()
+    print(f"# adjoint {name}: forward {t1 - t0:.2f} s, reverse sweep", flush=True)
     gu, _, gf = adj.backward(want_pressure=False, want_forcing=cs["forced"])
     torch.cuda.synchronize()
     t2 = time.perf_counter()
@@ -106,9 +110,16 @@ def run_adjoint(sa, ctx, name, fd_dirs):
     if cs["forced"]:
         out["forcing_wavenumbers"] = list(inputs.forcing_wavenumbers(ADJOINT_SEED))
     grad = gf if cs["forced"] else gu
+
+    def direction(d):
+        # velocity directions: seeded divergence-free fields of RMS 1; forcing directions: seeded
+        # forcing patterns of unit amplitude
+        return inputs.forcing(n, ADJOINT_SEED + d, 1.0) if cs["forced"] else \
+            inputs.perturbation(n, ADJOINT_SEED + d, 1.0)
+
     errs = []
     for d in range(1, fd_dirs + 1):
-        dirn = [torch.from_numpy(a).cuda() for a in inputs.perturbation(n, ADJOINT_SEED + d, 1.0)]
+        dirn = [torch.from_numpy(a).cuda() for a in direction(d)]
         pred = float(sum((g * x).sum() for g, x in zip(grad, dirn)))
         best = None
         for eps in (1e-5, 1e-6, 1e-7):
@@ -129,6 +140,25 @@ def run_adjoint(sa, ctx, name, fd_dirs):
         out["fd_directions"] = len(errs)
         out["fd_max_rel_error"] = max(errs)
         out["fd_eps"] = [1e-5, 1e-6, 1e-7]
+        # back to the base trajectory for the tangent checks
+        if cs["forced"]:
+            adj.set_forcing([torch.from_numpy(f).cuda() for f in inputs.forcing(n, ADJOINT_SEED, 1e-2)])
+        adj.forward(u0, p0, steps)
+    # tangent-linear consistency: <grad, d> against the device tangent along d
+    tl_errs = []
+    for d in range(1, tl_dirs + 1):
+        dirn = [torch.from_numpy(a).cuda() for a in direction(d)]
+        pred = float(sum((g * x).sum() for g, x in zip(grad, dirn)))
+        t3 = time.perf_counter()
+        _, dke = adj.tangent(dforce=dirn) if cs["forced"] else adj.tangent(dvel=dirn)
+        t4 = time.perf_counter()
+        tl_errs.append(abs(dke - pred) / abs(pred))
+        out["tangent_s"] = t4 - t3
+        print(f"# adjoint {name}: direction {d}: <grad, d> = {pred:.10e}, tangent {dke:.10e}, "
+              f"rel. difference {tl_errs[-1]:.2e}", flush=True)
+    if tl_errs:
+        out["tl_directions"] = len(tl_errs)
+        out["tl_max_rel_difference"] = max(tl_errs)
     adj.close()
     del u0, p0, grad, gu, gf
     torch.cuda.empty_cache()
@@ -296,7 +326,7 @@ def main():
     assert nsteps == 100, nsteps
     ctx = sa.Context(0)
     if a.adjoint_only:
-        r = run_adjoint(sa, ctx, a.adjoint if a.adjoint != "none" else "c4", a.adjoint_fd)
+        r = run_adjoint(sa, ctx, a.adjoint if a.adjoint != "none" else "c4", a.adjoint_fd, a.adjoint_tl)
         print(json.dumps({"metric": "adjoint s/step", "value": r["reverse_s_per_step"], "unit": "s/step",
                           "higher_is_better": False, "adjoint": r}))
         return 0
@@ -361,7 +391,7 @@ def main():
     if a.adjoint != "none":
         for k in list(res):
             res[k].pop("fields", None)
-        r_adj = run_adjoint(sa, ctx, a.adjoint, a.adjoint_fd)
+        r_adj = run_adjoint(sa, ctx, a.adjoint, a.adjoint_fd, a.adjoint_tl)
         print(f"# adjoint {a.adjoint}: forward {r_adj['forward_s']:.2f} s, reverse {r_adj['reverse_s']:.2f} s "
               f"({r_adj['reverse_s_per_step'] * 1e3:.1f} ms/step)", flush=True)
         out["adjoint_s_per_step"] = r_adj["reverse_s_per_step"]

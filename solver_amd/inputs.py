@@ -19,8 +19,11 @@ same value as face 1), ghosts zero.
 from __future__ import annotations
 
 import hashlib
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
+import scipy.fft
 
 GOLDEN_GAMMA = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -44,13 +47,33 @@ def uniform01(seed: int, count: int, offset: int = 0) -> np.ndarray:
     return ((splitmix64(seed, count, offset) >> np.uint64(11)).astype(np.float64) + 0.5) * (1.0 / 9007199254740992.0)
 
 
-def normals(seed: int, count: int) -> np.ndarray:
-    """Standard normal deviates by the Box-Muller transform of consecutive uniform pairs."""
-    m = (count + 1) // 2
-    u = uniform01(seed, 2 * m)
+def _threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _normals_into(out: np.ndarray, seed: int, offset: int) -> None:
+    u = uniform01(seed, out.size, offset)
     r = np.sqrt(-2.0 * np.log(u[0::2]))
     t = 2.0 * np.pi * u[1::2]
-    return np.concatenate([r * np.cos(t), r * np.sin(t)])[:count]
+    out[0::2] = r * np.cos(t)
+    out[1::2] = r * np.sin(t)
+
+
+def normals(seed: int, count: int, offset: int = 0) -> np.ndarray:
+    """Standard normal deviates offset .. offset+count-1 (offset and count even): the Box-Muller
+    transform of uniform pair i gives deviates 2i (cosine) and 2i+1 (sine), so any aligned
+    sub-range of the stream can be generated on its own; large requests are split into aligned
+    chunks generated on several threads (the values do not depend on the split)."""
+    if offset % 2 or count % 2:
+        raise ValueError("normals are generated in pairs: offset and count must be even")
+    out = np.empty(count)
+    chunk = 1 << 22
+    if count <= chunk:
+        _normals_into(out, seed, offset)
+        return out
+    with ThreadPoolExecutor(_threads()) as ex:
+        list(ex.map(lambda b: _normals_into(out[b:b + chunk], seed, offset + b), range(0, count, chunk)))
+    return out
 
 
 def _storage(n, loc):
@@ -64,24 +87,37 @@ def perturbation(n: int, seed: int, rms: float):
     """Discretely divergence-free random MAC velocity (u, v, w) on the periodic n^3 grid."""
     h = 2.0 * np.pi / n
     kk = np.fft.fftfreq(n, 1.0 / n)  # integer wavenumbers
-    kz, ky, kx = np.meshgrid(kk, kk, kk, indexing="ij")
-    kmag2 = kx * kx + ky * ky + kz * kz
-    amp = np.zeros_like(kmag2)
+    # broadcastable wavenumbers along (z, y, x)
+    k3 = [kk[None, None, :], kk[None, :, None], kk[:, None, None]]  # x, y, z
+    kmag2 = k3[0] ** 2 + k3[1] ** 2 + k3[2] ** 2
     nz = kmag2 > 0
+    amp = np.zeros(kmag2.shape)
     amp[nz] = np.exp(-kmag2[nz] / 64.0) / kmag2[nz]
-    g = normals(seed, 6 * n * n * n).reshape(3, 2, n, n, n)
-    coef = [(g[c, 0] + 1j * g[c, 1]) * np.sqrt(0.5) * amp for c in range(3)]
+    # component c uses deviates [2 c n^3, 2 (c+1) n^3): real parts, then imaginary parts
+    n3 = n * n * n
+    coef = []
+    for c in range(3):
+        g = normals(seed, 2 * n3, offset=2 * n3 * c).reshape(2, n, n, n)
+        coef.append((g[0] + 1j * g[1]) * (np.sqrt(0.5) * amp))
+        del g
     # MAC operators in Fourier space: D_a = (e^{i th} - 1)/h (faces -> cells), G_a =
     # (1 - e^{-i th})/h (cells -> faces), D.G = sum (2 cos th - 2)/h^2
-    th = [2.0 * np.pi * kx / n, 2.0 * np.pi * ky / n, 2.0 * np.pi * kz / n]
+    th = [2.0 * np.pi * k / n for k in k3]
     D = [(np.exp(1j * t) - 1.0) / h for t in th]
     G = [(1.0 - np.exp(-1j * t)) / h for t in th]
-    lap = sum((2.0 * np.cos(t) - 2.0) / (h * h) for t in th)
-    div = sum(D[a] * coef[a] for a in range(3))
-    phi = np.zeros_like(div)
-    phi[nz] = div[nz] / lap[nz]
-    coef = [coef[a] - G[a] * phi for a in range(3)]
-    comps = [np.real(np.fft.ifftn(c)) for c in coef]
+    lap = (2.0 * np.cos(th[0]) - 2.0) / (h * h) + (2.0 * np.cos(th[1]) - 2.0) / (h * h) + \
+        (2.0 * np.cos(th[2]) - 2.0) / (h * h)
+    phi = D[0] * coef[0] + D[1] * coef[1] + D[2] * coef[2]
+    phi[nz] = phi[nz] / lap[nz]
+    phi[~nz] = 0.0
+    comps = []
+    for a in range(3):
+        c = coef[a]
+        c -= G[a] * phi
+        coef[a] = None
+        comps.append(np.real(scipy.fft.ifftn(c, workers=_threads())))
+        del c
+    del phi
     scale = rms / np.sqrt(sum(float(np.sum(c * c)) for c in comps) / (3.0 * n ** 3))
     out = []
     for a in range(3):

@@ -13,7 +13,7 @@ from solver_amd import inputs
 
 C4_SEED = 20261203
 # SHA-256 of perturbation(128, C4_SEED, 1e-2) (little-endian float64 storage of u, v, w)
-C4_PERTURBATION_SHA256 = "0e501286cab992b5f9c2545f526bc1dc9e81f5d284ecc5f9e021f56fd46caf55"
+C4_PERTURBATION_SHA256 = "d53bac0ab83f372220055d630efcaba8743af3c780a1fbba957d4ac931aa1a66"
 
 
 def test_splitmix64_published_vector():
@@ -31,6 +31,10 @@ def test_normals_moments():
     assert abs(x.var() - 1.0) < 1e-2
     u = inputs.uniform01(5, 1000)
     assert u.min() > 0.0 and u.max() < 1.0
+    # aligned sub-ranges of the deviate stream are generated independently
+    assert np.array_equal(inputs.normals(5, 40)[10:30], inputs.normals(5, 20, offset=10))
+    with pytest.raises(ValueError):
+        inputs.normals(5, 3)
 
 
 @pytest.mark.parametrize("n", [8, 16, 24])
