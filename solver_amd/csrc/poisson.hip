@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace ofh {
@@ -727,6 +728,322 @@ __global__ void __launch_bounds__(256) k_smooth_last(const T* __restrict__ b, co
   store_partials<2>(v, ops, partials);
 }
 
+// ---------------------------------------------------------------- fused all-periodic levels
+// On a level that is periodic on every axis and halves every axis, the down leg (two
+// pre-smoothing sweeps, residual, restriction) and the up leg (prolongation, two post-smoothing
+// sweeps) each run as one kernel by temporal blocking in LDS: a 64x8 tile marches KZ planes
+// along k through three-plane rings, producing each intermediate on the tile plus the halo its
+// consumer needs (x1 on halo 2, x2 on halo 1, residual on the tile; x3 on halo 2, x4 on halo 1,
+// x5 on the tile). Every value is computed by the same per-cell expression as the unfused
+// kernels. The one difference: the periodic operator annihilates constants exactly, so the
+// residual reads x2 and the last sweep reads x4 without their means subtracted (the means of
+// intermediates are not known inside the kernel); those values differ from the unfused path by
+// rounding only. The down kernel reads b once and writes x2 and the coarse rhs (17 B per cell in
+// FP64, against 33 B for the two unfused kernels); the up kernel reads x2, b and the coarse
+// correction and writes x5 (25 B against 49 B).
+constexpr int kFX = 64, kFY = 8, kFThreads = 256;
+
+__device__ inline int wrap_index(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+__device__ inline int ring3(int k) { return (k + 6) % 3; }  // k >= -6
+
+struct FusedTile {
+  int i0, j0, k0, k1;
+};
+__device__ inline FusedTile fused_tile(const PLevel& L, int kz) {
+  const int gx = (L.nx + kFX - 1) / kFX, gy = (L.ny + kFY - 1) / kFY;
+  unsigned r = xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = (int)(r % (unsigned)gx);
+  r /= (unsigned)gx;
+  const int by = (int)(r % (unsigned)gy);
+  const int bz = (int)(r / (unsigned)gy);
+  FusedTile t;
+  t.i0 = bx * kFX;
+  t.j0 = by * kFY;
+  t.k0 = bz * kz;
+  t.k1 = min(t.k0 + kz, L.nz);
+  return t;
+}
+
+// Global loads of the plane after next are issued while the current plane is computed (two
+// register buffers alternate, so the loop is unrolled by two); each thread also carries the
+// right-hand side of its own two tile cells in a three-plane register queue, so the LDS rings
+// need only three planes. Two barriers per plane: after the LDS stores, and between the stage
+// on the halo and the stage on the tile.
+template <typename T>
+__global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ b, T* __restrict__ x2g, PLevel L,
+                                                          PLevel Cl, T* __restrict__ bc, double* partials,
+                                                          const double* ps, int lslot, int zm, int kz) {
+  if(stopped(ps)) return;
+  constexpr int BW = kFX + 4, BH = kFY + 4, BP = BW * BH;  // b - mb, halo 2
+  constexpr int XW = kFX + 2, XH = kFY + 2, XP = XW * XH;  // x2, halo 1
+  constexpr int NB = (BP + kFThreads - 1) / kFThreads;
+  __shared__ T sb[3][BP];
+  __shared__ T sx[3][XP];
+  const int tid = threadIdx.x;
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const size_t plane = (size_t)nx * ny;
+  const size_t cplane = (size_t)Cl.nx * Cl.ny;
+  const T mb = T(ps[lslot + L_MB]);
+  const T m1 = zm ? T((ps[lslot + L_SBW] - ps[lslot + L_MB] * L.winv_sum) / (double)L.n) : T(0);
+  const T w = T(L.winv[0]);
+  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+  const T inv_children = T(1.0 / 8.0);
+  const FusedTile t = fused_tile(L, kz);
+  const int tx = tid & 63, ty = tid >> 6;
+  const int gi = t.i0 + tx, gj0 = t.j0 + 2 * ty;
+  const bool own0 = gi < nx && gj0 < ny, own1 = gi < nx && gj0 + 1 < ny;
+  // per-thread element offsets of the halo-2 plane
+  size_t off[NB];
+  bool okb[NB];
+#pragma unroll
+  for(int m = 0; m < NB; ++m) {
+    const int e = tid + m * kFThreads;
+    const int gx = t.i0 + (e % BW) - 2, gy = t.j0 + (e / BW) - 2;
+    okb[m] = e < BP && gx < nx + 2 && gy < ny + 2;
+    off[m] = okb[m] ? (size_t)wrap_index(gx, nx) + (size_t)nx * wrap_index(gy, ny) : 0;
+  }
+  const size_t own_off0 = own0 ? (size_t)gi + (size_t)nx * gj0 : 0, own_off1 = own1 ? own_off0 + nx : 0;
+  // Loads are unconditional (masked-out elements read offset 0) and masked afterwards: a load
+  // under a lane-divergent condition is compiled with a wait for its completion before the
+  // branches rejoin, which would serialise every load of the plane.
+  // The raw values stay in registers until the next plane's LDS store, so the loads are only
+  // waited for after the current plane's compute.
+  auto fetch = [&](int kp, T (&pre)[NB], T (&tb)[2]) {
+    const size_t kz = plane * (size_t)wrap_index(kp, nz);
+#pragma unroll
+    for(int m = 0; m < NB; ++m) pre[m] = b[off[m] + kz];
+    tb[0] = b[own_off0 + kz];
+    tb[1] = b[own_off1 + kz];
+  };
+  auto X1 = [&](int slot, int e) { return sb[slot][e] * w - m1; };
+  double v[3] = {0.0, 0.0, 0.0};
+  T acc = T(0);
+  T q1[2] = {T(0), T(0)}, q2[2] = {T(0), T(0)};  // tile rhs of planes kp - 1, kp - 2
+  const int last = (t.k1 - t.k0) + 1;
+  auto step = [&](int s, T (&cur)[NB], T (&ctb)[2], T (&nxt)[NB], T (&ntb)[2]) {
+    const int kp = t.k0 + s;
+    {
+      T* dst = sb[ring3(kp)];
+#pragma unroll
+      for(int m = 0; m < NB; ++m) {
+        const int e = tid + m * kFThreads;
+        if(e < BP) dst[e] = okb[m] ? cur[m] - mb : T(0);
+      }
+    }
+    const T tb0 = own0 ? ctb[0] - mb : T(0), tb1 = own1 ? ctb[1] - mb : T(0);
+    __syncthreads();
+    if(s < last) fetch(kp + 1, nxt, ntb);
+    if(s >= 0) {  // x2 on plane kq = kp - 1, tile + halo 1
+      const int kq = kp - 1;
+      const int sm = ring3(kq - 1), sc = ring3(kq), sp = ring3(kq + 1);
+      T* dst = sx[ring3(kq)];
+      const bool own = kq >= t.k0 && kq < t.k1;
+      for(int e = tid; e < XP; e += kFThreads) {
+        const int xi = e % XW, xj = e / XW;
+        const int be = (xi + 1) + BW * (xj + 1);
+        const T bt = sb[sc][be];
+        const T c = bt * w - m1;
+        const T ax = lap_neg<T>(c, X1(sc, be - 1), X1(sc, be + 1), X1(sc, be - BW), X1(sc, be + BW), X1(sm, be),
+                                X1(sp, be), ihx2, ihy2, ihz2);
+        const T out = c + w * (bt + T(-1) * ax);
+        dst[e] = out;
+        const int ci = t.i0 + xi - 1, cj = t.j0 + xj - 1;
+        if(own && xi >= 1 && xi <= kFX && xj >= 1 && xj <= kFY && ci < nx && cj < ny) {
+          x2g[(size_t)ci + (size_t)nx * cj + plane * kq] = out;
+          v[0] += (double)out;
+        }
+      }
+    }
+    __syncthreads();
+    if(s >= 2) {  // residual on plane kr = kp - 2 (tile), restricted 2x2x2
+      const int kr = kp - 2;
+      const int sm = ring3(kr - 1), sc = ring3(kr), sp = ring3(kr + 1);
+      const bool second = ((kr - t.k0) & 1) != 0;
+      T xs[2];
+#pragma unroll
+      for(int rr = 0; rr < 2; ++rr) {
+        const int xe = (tx + 1) + XW * (2 * ty + rr + 1);
+        const T c = sx[sc][xe];
+        const T ax = lap_neg<T>(c, sx[sc][xe - 1], sx[sc][xe + 1], sx[sc][xe - XW], sx[sc][xe + XW], sx[sm][xe],
+                                sx[sp][xe], ihx2, ihy2, ihz2);
+        T r = q2[rr] + T(-1) * ax;
+        r += __shfl_xor(r, 1, 64);  // x pair
+        xs[rr] = r;
+      }
+      const T ysum = xs[0] + xs[1];  // y pair
+      const T tot = second ? acc + ysum : ysum;
+      acc = tot;
+      if(second && (tx & 1) == 0 && own0) {
+        const int ic = gi >> 1, jc = gj0 >> 1, kc = kr >> 1;
+        const T val = inv_children * tot;
+        bc[(size_t)ic + (size_t)Cl.nx * jc + cplane * kc] = val;
+        v[1] += (double)val;
+        v[2] += (double)val * Cl.winv[0];
+      }
+    }
+    q2[0] = q1[0];
+    q2[1] = q1[1];
+    q1[0] = tb0;
+    q1[1] = tb1;
+  };
+  T pa[NB], pb[NB], ta[2], tb2[2];
+  fetch(t.k0 - 2, pa, ta);
+  for(int s = -2; s <= last; s += 2) {
+    step(s, pa, ta, pb, tb2);
+    step(s + 1, pb, tb2, pa, ta);
+  }
+  const int ops[3] = {kSum, kSum, kSum};
+  store_partials<3>(v, ops, partials);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b, const T* __restrict__ x2,
+                                                        const T* __restrict__ xc, PLevel L, PLevel Cl,
+                                                        T* __restrict__ x5g, double* partials, const double* ps,
+                                                        int lslot, int cslot, int kz) {
+  if(stopped(ps)) return;
+  constexpr int AW = kFX + 4, AH = kFY + 4, AP = AW * AH;  // x3, halo 2
+  constexpr int XW = kFX + 2, XH = kFY + 2, XP = XW * XH;  // x4 and b - mb, halo 1
+  constexpr int NA = (AP + kFThreads - 1) / kFThreads, NX = (XP + kFThreads - 1) / kFThreads;
+  __shared__ T s3[3][AP];
+  __shared__ T s4[3][XP];
+  __shared__ T sbt[2][XP];
+  const int tid = threadIdx.x;
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const size_t plane = (size_t)nx * ny;
+  const size_t cplane = (size_t)Cl.nx * Cl.ny;
+  const T mb = T(ps[lslot + L_MB]);
+  const T m2 = T(ps[lslot + L_MX2]);
+  const T mc = T(ps[cslot + L_MX5]);
+  const T w = T(L.winv[0]);
+  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+  const FusedTile t = fused_tile(L, kz);
+  const int tx = tid & 63, ty = tid >> 6;
+  const int gi = t.i0 + tx, gj0 = t.j0 + 2 * ty;
+  const bool own0 = gi < nx && gj0 < ny, own1 = gi < nx && gj0 + 1 < ny;
+  size_t offa[NA], offc[NA], offx[NX];
+  bool oka[NA], okx[NX];
+#pragma unroll
+  for(int m = 0; m < NA; ++m) {
+    const int e = tid + m * kFThreads;
+    const int gx = t.i0 + (e % AW) - 2, gy = t.j0 + (e / AW) - 2;
+    oka[m] = e < AP && gx < nx + 2 && gy < ny + 2;
+    const int ii = oka[m] ? wrap_index(gx, nx) : 0, jj = oka[m] ? wrap_index(gy, ny) : 0;
+    offa[m] = (size_t)ii + (size_t)nx * jj;
+    offc[m] = (size_t)(ii >> 1) + (size_t)Cl.nx * (jj >> 1);
+  }
+#pragma unroll
+  for(int m = 0; m < NX; ++m) {
+    const int e = tid + m * kFThreads;
+    const int gx = t.i0 + (e % XW) - 1, gy = t.j0 + (e / XW) - 1;
+    okx[m] = e < XP && gx < nx + 1 && gy < ny + 1;
+    offx[m] = okx[m] ? (size_t)wrap_index(gx, nx) + (size_t)nx * wrap_index(gy, ny) : 0;
+  }
+  const size_t own_off0 = own0 ? (size_t)gi + (size_t)nx * gj0 : 0, own_off1 = own1 ? own_off0 + nx : 0;
+  struct Pre {
+    T a[NA];  // x2
+    T c[NA];  // parent of x2 in the coarse correction
+    T x[NX];  // b
+    T tb[2];  // b of the thread's two tile cells
+  };
+  auto fetch = [&](int kp, Pre& p) {  // unconditional raw loads (see k_down_fused)
+    const int kw = wrap_index(kp, nz);
+    const size_t kz = plane * (size_t)kw, cz = cplane * (size_t)(kw >> 1);
+#pragma unroll
+    for(int m = 0; m < NA; ++m) {
+      p.a[m] = x2[offa[m] + kz];
+      p.c[m] = xc[offc[m] + cz];
+    }
+#pragma unroll
+    for(int m = 0; m < NX; ++m) p.x[m] = b[offx[m] + kz];
+    p.tb[0] = b[own_off0 + kz];
+    p.tb[1] = b[own_off1 + kz];
+  };
+  double v[2] = {0.0, 0.0};
+  T q1[2] = {T(0), T(0)}, q2[2] = {T(0), T(0)};  // tile rhs of planes kp - 1, kp - 2
+  const int last = (t.k1 - t.k0) + 1;
+  auto step = [&](int s, Pre& cur, Pre& nxt) {
+    const int kp = t.k0 + s;
+    {
+      T* d3 = s3[ring3(kp)];
+#pragma unroll
+      for(int m = 0; m < NA; ++m) {  // x3 = (x2 - m2) + (xc - mc)[parent]
+        const int e = tid + m * kFThreads;
+        if(e < AP) d3[e] = oka[m] ? (cur.a[m] - m2) + (cur.c[m] - mc) : T(0);
+      }
+      T* db = sbt[kp & 1];
+#pragma unroll
+      for(int m = 0; m < NX; ++m) {
+        const int e = tid + m * kFThreads;
+        if(e < XP) db[e] = okx[m] ? cur.x[m] - mb : T(0);
+      }
+    }
+    const T tb0 = own0 ? cur.tb[0] - mb : T(0), tb1 = own1 ? cur.tb[1] - mb : T(0);
+    __syncthreads();
+    if(s < last) fetch(kp + 1, nxt);
+    if(s >= 0) {  // x4 on plane kq = kp - 1, tile + halo 1
+      const int kq = kp - 1;
+      const int sm = ring3(kq - 1), sc = ring3(kq), sp = ring3(kq + 1);
+      T* dst = s4[ring3(kq)];
+      const T* bq = sbt[kq & 1];
+      for(int e = tid; e < XP; e += kFThreads) {
+        const int ae = (e % XW + 1) + AW * (e / XW + 1);
+        const T c = s3[sc][ae];
+        const T ax = lap_neg<T>(c, s3[sc][ae - 1], s3[sc][ae + 1], s3[sc][ae - AW], s3[sc][ae + AW], s3[sm][ae],
+                                s3[sp][ae], ihx2, ihy2, ihz2);
+        dst[e] = c + w * (bq[e] + T(-1) * ax);
+      }
+    }
+    __syncthreads();
+    if(s >= 2) {  // x5 on plane kr = kp - 2 (tile)
+      const int kr = kp - 2;
+      const int sm = ring3(kr - 1), sc = ring3(kr), sp = ring3(kr + 1);
+#pragma unroll
+      for(int rr = 0; rr < 2; ++rr) {
+        const int xe = (tx + 1) + XW * (2 * ty + rr + 1);
+        const T c = s4[sc][xe];
+        const T ax = lap_neg<T>(c, s4[sc][xe - 1], s4[sc][xe + 1], s4[sc][xe - XW], s4[sc][xe + XW], s4[sm][xe],
+                                s4[sp][xe], ihx2, ihy2, ihz2);
+        const T bt = q2[rr];
+        const T out = c + w * (bt + T(-1) * ax);
+        if(rr == 0 ? own0 : own1) {
+          x5g[(rr == 0 ? own_off0 : own_off1) + plane * kr] = out;
+          v[0] += (double)out;
+          v[1] += (double)bt * (double)out;
+        }
+      }
+    }
+    q2[0] = q1[0];
+    q2[1] = q1[1];
+    q1[0] = tb0;
+    q1[1] = tb1;
+  };
+  Pre pa, pb;
+  fetch(t.k0 - 2, pa);
+  for(int s = -2; s <= last; s += 2) {
+    step(s, pa, pb);
+    step(s + 1, pb, pa);
+  }
+  const int ops[2] = {kSum, kSum};
+  store_partials<2>(v, ops, partials);
+}
+
+struct EpiDown {  // mean of x2; coarse rhs mean, sum of b w/D, sum
+  static constexpr int NV = 3;
+  double* ps;
+  int lslot, cslot;
+  double n, nc;
+  int zm;
+  __device__ int op(int) const { return kSum; }
+  __device__ bool active() const { return !stopped(ps); }
+  __device__ void operator()(const double (&r)[3]) const {
+    ps[lslot + L_MX2] = zm ? r[0] / n : 0.0;
+    ps[cslot + L_MB] = zm ? r[1] / nc : 0.0;
+    ps[cslot + L_SBW] = r[2];
+    ps[cslot + L_SUMB] = r[1];
+  }
+};
+
 // ---------------------------------------------------------------- one-workgroup tail
 // Levels l0..nl-1 of the V-cycle (the small ones) in a single workgroup: the same per-cell
 // formulas, block-wide sums for the deferred means, and the coarsest-level elimination with the
@@ -1157,6 +1474,29 @@ HotLaunch hot_launch(const PLevel& L) {
   return h;
 }
 
+// Launch shape of the fused all-periodic level kernels: 64x8 tiles, kz planes per block (even,
+// so that z pairs of the restriction stay inside a block). Each block re-reads the 4 planes
+// around its range, so long ranges are cheaper, but a level must still have enough blocks to
+// fill the chip: the longest range in {16, 8, 4, 2} that gives at least 1024 blocks.
+struct FusedLaunch {
+  dim3 grid;
+  int kz;
+};
+FusedLaunch fused_launch(const PLevel& L) {
+  FusedLaunch f;
+  const size_t tiles = (size_t)((L.nx + kFX - 1) / kFX) * (size_t)((L.ny + kFY - 1) / kFY);
+  f.kz = 2;
+  for(int kz : {16, 8, 4})
+    if(tiles * (size_t)((L.nz + kz - 1) / kz) >= 1024) {
+      f.kz = kz;
+      break;
+    }
+  const size_t n = tiles * (size_t)((L.nz + f.kz - 1) / f.kz);
+  if(n > (size_t)kMaxPartials) throw InvalidArgument("grid too large for the reduction buffer");
+  f.grid = dim3((unsigned)n);
+  return f;
+}
+
 // Instantiates KERN<pre..., KZ, UNI> for the launch shape h.
 #define OFH_HOT_LAUNCH(KERN, h, stream, ...)                                                  \
   do {                                                                                        \
@@ -1195,6 +1535,7 @@ class PoissonSolver {
   hipEvent_t ev[2]{};
   PLevel* dlv = nullptr;      // level descriptors for the tail kernel
   int l0 = 0;                 // first level handled by the one-workgroup tail
+  std::vector<char> fused;    // per level: the fused all-periodic kernels run it
   // coarse solve data
   int nc = 0;
   int* perm = nullptr;
@@ -1209,9 +1550,19 @@ class PoissonSolver {
   void vcycle();
   template <typename T>
   PoissonResult solve_t(const double* rhs, double* p, int maxit, double tol, bool want_history);
+  // the array holding a level's smoothed correction after the up leg: the fused up kernel
+  // writes it beside x2 (its halo is still being read by neighbouring tiles)
+  void* x_final(int l) const { return l < l0 && fused[l] ? lxb[l] : lxa[l]; }
 };
 
 namespace {
+
+// SOLVER_AMD_MG_UNFUSED=1 selects the unfused level kernels everywhere (they differ from the
+// fused ones only in rounding); read when a solver is built.
+bool fused_disabled() {
+  const char* e = std::getenv("SOLVER_AMD_MG_UNFUSED");
+  return e != nullptr && e[0] == '1';
+}
 
 bool p_axis_periodic(const of_pressure_bc_set& b, int axis) {
   const bool lo = b.faces[2 * axis].type == OF_PRESSURE_PERIODIC, hi = b.faces[2 * axis + 1].type == OF_PRESSURE_PERIODIC;
@@ -1395,6 +1746,12 @@ void PoissonSolver::build() {
       l0 = l;
       break;
     }
+  fused.assign(lv.size(), 0);
+  for(int l = 0; l < l0; ++l) {
+    const PLevel& P = lv[l];
+    const bool uni = P.per[0] && P.per[1] && P.per[2];
+    fused[l] = uni && P.rx == 2 && P.ry == 2 && P.rz == 2 && P.nx >= 4 && P.ny >= 4 && P.nz >= 4 && !fused_disabled();
+  }
   const size_t n0 = (size_t)lv[0].n;
   OFH_CHECK(hipMalloc(&pc, n0 * sizeof(double)));
   OFH_CHECK(hipMalloc(&r, n0 * sizeof(double)));
@@ -1447,6 +1804,17 @@ void PoissonSolver::vcycle() {
   for(int l = 0; l < l0; ++l) {
     const PLevel& P = lv[l];
     const PLevel& C = lv[l + 1];
+    if(fused[l]) {
+      const FusedLaunch f = fused_launch(P);
+      {
+        ProfScope prof_(c, "k_down_fused", l == 0);
+        hipLaunchKernelGGL(k_down_fused<T>, f.grid, dim3(kFThreads), 0, c.stream, bptr(l), static_cast<T*>(lxa[l]), P,
+                           C, static_cast<T*>(lb[l + 1]), c.partials, ps, slot(l), zmi, f.kz);
+      }
+      launch_finalize(c.stream, c.partials, f.grid,
+                      EpiDown{ps, slot(l), slot(l + 1), (double)P.n, (double)C.n, zmi});
+      continue;
+    }
     const HotLaunch h = hot_launch(P);
     {
       ProfScope prof_(c, "k_smooth12", l == 0);
@@ -1486,12 +1854,23 @@ void PoissonSolver::vcycle() {
   for(int l = l0 - 1; l >= 0; --l) {
     const PLevel& P = lv[l];
     const PLevel& C = lv[l + 1];
+    const T* xc = static_cast<const T*>(x_final(l + 1));
+    if(fused[l]) {
+      const FusedLaunch f = fused_launch(P);
+      {
+        ProfScope prof_(c, "k_up_fused", l == 0);
+        hipLaunchKernelGGL(k_up_fused<T>, f.grid, dim3(kFThreads), 0, c.stream, bptr(l),
+                           static_cast<const T*>(lxa[l]), xc, P, C, static_cast<T*>(lxb[l]), c.partials, ps, slot(l),
+                           slot(l + 1), f.kz);
+      }
+      launch_finalize(c.stream, c.partials, f.grid, EpiSmoothLast{ps, slot(l), (double)P.n, zmi, l == 0 ? 1 : 0});
+      continue;
+    }
     const HotLaunch h = hot_launch(P);
     {
       ProfScope prof_(c, "k_prolong_smooth", l == 0);
-      OFH_HOT_LAUNCH(k_prolong_smooth<T, h, c.stream, bptr(l), static_cast<const T*>(lxa[l]),
-                     static_cast<const T*>(lxa[l + 1]), P, C, static_cast<T*>(lxb[l]), c.partials, ps, slot(l),
-                     slot(l + 1));
+      OFH_HOT_LAUNCH(k_prolong_smooth<T, h, c.stream, bptr(l), static_cast<const T*>(lxa[l]), xc, P, C,
+                     static_cast<T*>(lxb[l]), c.partials, ps, slot(l), slot(l + 1));
     }
     launch_finalize(c.stream, c.partials, h.grid, EpiMean{ps, slot(l) + L_MX4, (double)P.n, zmi});
     {
@@ -1530,7 +1909,7 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   for(; it < maxit; ++it) {
     {
       ProfScope prof_(c, "k_direction");
-      hipLaunchKernelGGL(k_direction<T>, dim3(pgrid), dim3(256), 0, c.stream, static_cast<const T*>(lxa[0]),
+      hipLaunchKernelGGL(k_direction<T>, dim3(pgrid), dim3(256), 0, c.stream, static_cast<const T*>(x_final(0)),
                          d[(it + 1) % 2], d[it % 2], pc, F.n, ps, it == 0 ? 1 : 0);
     }
     {

@@ -1,0 +1,79 @@
+"""GPU checks of the fused all-periodic multigrid level kernels against the unfused ones.
+
+The fused down/up kernels evaluate the same per-cell expressions as the unfused level kernels
+except that two intermediate means are not subtracted (the periodic operator annihilates
+constants), so a solve must take the same number of iterations and agree to rounding. The
+unfused path is selected by SOLVER_AMD_MG_UNFUSED=1 when a solver is built.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+import solver_amd as sa
+from tests.gpu_util import dev, host, rel_l2_active, zeros
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return sa.Context(0)
+
+
+def solve(ctx, g, rhs, precision, tol, unfused):
+    os.environ["SOLVER_AMD_MG_UNFUSED"] = "1" if unfused else "0"
+    try:
+        o = sa.ProjectionOptions(1.0, 1.0, 400, tol)
+        pb = sa.pressure_bc_set([sa.P_PERIODIC] * 6)
+        sol = zeros(g, sa.CELL)
+        d = sa.PoissonDiagnostics()
+        ctx.run("of_solve_pressure_poisson", C.byref(g), C.byref(pb), C.byref(o), precision, sa.ptr(dev(rhs)),
+                sa.ptr(sol), C.byref(d), None, 0)
+    finally:
+        os.environ.pop("SOLVER_AMD_MG_UNFUSED", None)
+    return host(sol), d
+
+
+@pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (96, 48, 32), (40, 24, 72)])
+def test_fused_solve_matches_unfused(ctx, precision, shape):
+    nx, ny, nz = shape
+    g = sa.grid(nx, ny, nz, 2 * math.pi / nx, 2 * math.pi / ny, 2 * math.pi / nz)
+    rng = np.random.default_rng(nx + ny + nz)
+    rhs = rng.standard_normal(sa.storage_shape(nx, ny, nz, sa.CELL))
+    tol = 1e-12 if precision == sa.FP64 else 1e-10
+    a, da = solve(ctx, g, rhs, precision, tol, unfused=False)
+    b, db = solve(ctx, g, rhs, precision, tol, unfused=True)
+    assert da.converged == db.converged == 1
+    assert abs(da.iterations - db.iterations) <= (0 if precision == sa.FP64 else 1), (da.iterations, db.iterations)
+    # both solutions are within the tolerance of the exact one; they agree far better than that
+    assert rel_l2_active(a, b, mean_removed=True) <= 10 * tol
+
+
+@pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
+def test_fused_stepper_matches_unfused(ctx, precision):
+    n, steps = 48, 4
+    L = 2 * math.pi
+    dt, ns = sa.taylor_green_dt(n, n, n, 0.5, steps * 0.5 / (3 * n / L) * (1 - 1e-14))
+    out = []
+    for unfused in (False, True):
+        os.environ["SOLVER_AMD_MG_UNFUSED"] = "1" if unfused else "0"
+        try:
+            st = sa.Stepper(ctx, sa.TAYLOR_GREEN, n, n, n, 1 / 1600, dt, 400, 1e-12, precision)
+        finally:
+            os.environ.pop("SOLVER_AMD_MG_UNFUSED", None)
+        st.initialize(convention=1)
+        its = [st.run(1).pressure_iterations for _ in range(ns)]
+        vel, p, _, _, m = st.download()
+        out.append((vel, p, its, m))
+        st.close()
+    (va, pa, ia, ma), (vb, pb, ib, mb) = out
+    if precision == sa.FP64:
+        assert ia == ib
+    for x, y in zip(va, vb):
+        assert rel_l2_active(x, y) <= 1e-11
+    assert rel_l2_active(pa, pb, mean_removed=True) <= 1e-10
+    assert ma.max_divergence_linf <= 1e-10
