@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 namespace ofh {
@@ -764,8 +765,8 @@ __device__ inline FusedTile fused_tile(const PLevel& L, int kz) {
   return t;
 }
 
-// Global loads of the plane after next are issued while the current plane is computed (two
-// register buffers alternate, so the loop is unrolled by two); each thread also carries the
+// Global loads two planes ahead are issued while the current plane is computed (three register
+// buffers rotate, so the loop is unrolled by three); each thread also carries the
 // right-hand side of its own two tile cells in a three-plane register queue, so the LDS rings
 // need only three planes. Two barriers per plane: after the LDS stores, and between the stage
 // on the halo and the stage on the tile.
@@ -774,10 +775,12 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
                                                           PLevel Cl, T* __restrict__ bc, double* partials,
                                                           const double* ps, int lslot, int zm, int kz) {
   if(stopped(ps)) return;
-  constexpr int BW = kFX + 4, BH = kFY + 4, BP = BW * BH;  // b - mb, halo 2
-  constexpr int XW = kFX + 2, XH = kFY + 2, XP = XW * XH;  // x2, halo 1
+  constexpr int BW = kFX + 4, BH = kFY + 4, BP = BW * BH;  // x1, halo 2
+  constexpr int XW = kFX + 2, XH = kFY + 2, XP = XW * XH;  // b - mb and x2, halo 1
   constexpr int NB = (BP + kFThreads - 1) / kFThreads;
-  __shared__ T sb[3][BP];
+  constexpr int NH = (XP + kFThreads - 1) / kFThreads;
+  __shared__ T sb[3][BP];   // x1 = (b - mb) w - m1, computed once per point
+  __shared__ T sbt[3][XP];  // b - mb
   __shared__ T sx[3][XP];
   const int tid = threadIdx.x;
   const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -792,15 +795,32 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
   const int tx = tid & 63, ty = tid >> 6;
   const int gi = t.i0 + tx, gj0 = t.j0 + 2 * ty;
   const bool own0 = gi < nx && gj0 < ny, own1 = gi < nx && gj0 + 1 < ny;
-  // per-thread element offsets of the halo-2 plane
+  // per-thread element offsets of the halo-2 plane (and the matching halo-1 index, or -1)
   size_t off[NB];
   bool okb[NB];
+  int e1[NB];
 #pragma unroll
   for(int m = 0; m < NB; ++m) {
     const int e = tid + m * kFThreads;
-    const int gx = t.i0 + (e % BW) - 2, gy = t.j0 + (e / BW) - 2;
+    const int ex = e % BW, ey = e / BW;
+    const int gx = t.i0 + ex - 2, gy = t.j0 + ey - 2;
     okb[m] = e < BP && gx < nx + 2 && gy < ny + 2;
     off[m] = okb[m] ? (size_t)wrap_index(gx, nx) + (size_t)nx * wrap_index(gy, ny) : 0;
+    e1[m] = e < BP && ex >= 1 && ex <= XW && ey >= 1 && ey <= XH ? (ex - 1) + XW * (ey - 1) : -1;
+  }
+  // per-thread points of the halo-1 stage: LDS index in the halo-2 plane, and the global
+  // offset when the point is a tile cell inside the level (else -1)
+  int hb[NH];
+  long long hg[NH];
+#pragma unroll
+  for(int m = 0; m < NH; ++m) {
+    const int e = tid + m * kFThreads;
+    const int xi = e % XW, xj = e / XW;
+    hb[m] = (xi + 1) + BW * (xj + 1);
+    const int ci = t.i0 + xi - 1, cj = t.j0 + xj - 1;
+    hg[m] = e < XP && xi >= 1 && xi <= kFX && xj >= 1 && xj <= kFY && ci < nx && cj < ny
+                ? (long long)ci + (long long)nx * cj
+                : -1;
   }
   const size_t own_off0 = own0 ? (size_t)gi + (size_t)nx * gj0 : 0, own_off1 = own1 ? own_off0 + nx : 0;
   // Loads are unconditional (masked-out elements read offset 0) and masked afterwards: a load
@@ -815,7 +835,6 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
     tb[0] = b[own_off0 + kz];
     tb[1] = b[own_off1 + kz];
   };
-  auto X1 = [&](int slot, int e) { return sb[slot][e] * w - m1; };
   double v[3] = {0.0, 0.0, 0.0};
   T acc = T(0);
   T q1[2] = {T(0), T(0)}, q2[2] = {T(0), T(0)};  // tile rhs of planes kp - 1, kp - 2
@@ -823,34 +842,42 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
   auto step = [&](int s, T (&cur)[NB], T (&ctb)[2], T (&nxt)[NB], T (&ntb)[2]) {
     const int kp = t.k0 + s;
     {
-      T* dst = sb[ring3(kp)];
+      T* d1 = sb[ring3(kp)];
+      T* dt = sbt[ring3(kp)];
 #pragma unroll
       for(int m = 0; m < NB; ++m) {
         const int e = tid + m * kFThreads;
-        if(e < BP) dst[e] = okb[m] ? cur[m] - mb : T(0);
+        const T bt = okb[m] ? cur[m] - mb : T(0);
+        if(e < BP) d1[e] = bt * w - m1;
+        if(e1[m] >= 0) dt[e1[m]] = bt;
       }
     }
     const T tb0 = own0 ? ctb[0] - mb : T(0), tb1 = own1 ? ctb[1] - mb : T(0);
     __syncthreads();
-    if(s < last) fetch(kp + 1, nxt, ntb);
+    if(s + 2 <= last) fetch(kp + 2, nxt, ntb);
     if(s >= 0) {  // x2 on plane kq = kp - 1, tile + halo 1
       const int kq = kp - 1;
-      const int sm = ring3(kq - 1), sc = ring3(kq), sp = ring3(kq + 1);
+      const T* x1m = sb[ring3(kq - 1)];
+      const T* x1c = sb[ring3(kq)];
+      const T* x1p = sb[ring3(kq + 1)];
+      const T* btc = sbt[ring3(kq)];
       T* dst = sx[ring3(kq)];
       const bool own = kq >= t.k0 && kq < t.k1;
-      for(int e = tid; e < XP; e += kFThreads) {
-        const int xi = e % XW, xj = e / XW;
-        const int be = (xi + 1) + BW * (xj + 1);
-        const T bt = sb[sc][be];
-        const T c = bt * w - m1;
-        const T ax = lap_neg<T>(c, X1(sc, be - 1), X1(sc, be + 1), X1(sc, be - BW), X1(sc, be + BW), X1(sm, be),
-                                X1(sp, be), ihx2, ihy2, ihz2);
-        const T out = c + w * (bt + T(-1) * ax);
-        dst[e] = out;
-        const int ci = t.i0 + xi - 1, cj = t.j0 + xj - 1;
-        if(own && xi >= 1 && xi <= kFX && xj >= 1 && xj <= kFY && ci < nx && cj < ny) {
-          x2g[(size_t)ci + (size_t)nx * cj + plane * kq] = out;
-          v[0] += (double)out;
+      const size_t pq = plane * (size_t)kq;
+#pragma unroll
+      for(int m = 0; m < NH; ++m) {
+        const int e = tid + m * kFThreads;
+        if(e < XP) {
+          const int be = hb[m];
+          const T c = x1c[be];
+          const T ax = lap_neg<T>(c, x1c[be - 1], x1c[be + 1], x1c[be - BW], x1c[be + BW], x1m[be], x1p[be], ihx2,
+                                  ihy2, ihz2);
+          const T out = c + w * (btc[e] + T(-1) * ax);
+          dst[e] = out;
+          if(own && hg[m] >= 0) {
+            x2g[(size_t)hg[m] + pq] = out;
+            v[0] += (double)out;
+          }
         }
       }
     }
@@ -886,12 +913,18 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
     q1[0] = tb0;
     q1[1] = tb1;
   };
-  T pa[NB], pb[NB], ta[2], tb2[2];
+  // three register buffers rotate: a step stores plane kp and fetches plane kp + 2
+  T pa[NB], pb[NB], pc[NB], ta[2], tb[2], tc[2];
   fetch(t.k0 - 2, pa, ta);
-  for(int s = -2; s <= last; s += 2) {
-    step(s, pa, ta, pb, tb2);
-    step(s + 1, pb, tb2, pa, ta);
+  fetch(t.k0 - 1, pb, tb);
+  int s = -2;
+  for(; s + 2 <= last; s += 3) {
+    step(s, pa, ta, pc, tc);
+    step(s + 1, pb, tb, pa, ta);
+    step(s + 2, pc, tc, pb, tb);
   }
+  if(s <= last) step(s, pa, ta, pc, tc);
+  if(s + 1 <= last) step(s + 1, pb, tb, pa, ta);
   const int ops[3] = {kSum, kSum, kSum};
   store_partials<3>(v, ops, partials);
 }
@@ -923,6 +956,7 @@ __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b,
   const bool own0 = gi < nx && gj0 < ny, own1 = gi < nx && gj0 + 1 < ny;
   size_t offa[NA], offc[NA], offx[NX];
   bool oka[NA], okx[NX];
+  int ha[NX];  // halo-1 stage points: index in the halo-2 plane
 #pragma unroll
   for(int m = 0; m < NA; ++m) {
     const int e = tid + m * kFThreads;
@@ -938,6 +972,7 @@ __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b,
     const int gx = t.i0 + (e % XW) - 1, gy = t.j0 + (e / XW) - 1;
     okx[m] = e < XP && gx < nx + 1 && gy < ny + 1;
     offx[m] = okx[m] ? (size_t)wrap_index(gx, nx) + (size_t)nx * wrap_index(gy, ny) : 0;
+    ha[m] = (e % XW + 1) + AW * (e / XW + 1);
   }
   const size_t own_off0 = own0 ? (size_t)gi + (size_t)nx * gj0 : 0, own_off1 = own1 ? own_off0 + nx : 0;
   struct Pre {
@@ -980,18 +1015,25 @@ __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b,
     }
     const T tb0 = own0 ? cur.tb[0] - mb : T(0), tb1 = own1 ? cur.tb[1] - mb : T(0);
     __syncthreads();
-    if(s < last) fetch(kp + 1, nxt);
+    if(s + 2 <= last) fetch(kp + 2, nxt);
     if(s >= 0) {  // x4 on plane kq = kp - 1, tile + halo 1
       const int kq = kp - 1;
       const int sm = ring3(kq - 1), sc = ring3(kq), sp = ring3(kq + 1);
       T* dst = s4[ring3(kq)];
       const T* bq = sbt[kq & 1];
-      for(int e = tid; e < XP; e += kFThreads) {
-        const int ae = (e % XW + 1) + AW * (e / XW + 1);
-        const T c = s3[sc][ae];
-        const T ax = lap_neg<T>(c, s3[sc][ae - 1], s3[sc][ae + 1], s3[sc][ae - AW], s3[sc][ae + AW], s3[sm][ae],
-                                s3[sp][ae], ihx2, ihy2, ihz2);
-        dst[e] = c + w * (bq[e] + T(-1) * ax);
+      const T* x3m = s3[sm];
+      const T* x3c = s3[sc];
+      const T* x3p = s3[sp];
+#pragma unroll
+      for(int m = 0; m < NX; ++m) {
+        const int e = tid + m * kFThreads;
+        if(e < XP) {
+          const int ae = ha[m];
+          const T c = x3c[ae];
+          const T ax =
+              lap_neg<T>(c, x3c[ae - 1], x3c[ae + 1], x3c[ae - AW], x3c[ae + AW], x3m[ae], x3p[ae], ihx2, ihy2, ihz2);
+          dst[e] = c + w * (bq[e] + T(-1) * ax);
+        }
       }
     }
     __syncthreads();
@@ -1018,12 +1060,17 @@ __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b,
     q1[0] = tb0;
     q1[1] = tb1;
   };
-  Pre pa, pb;
+  Pre pa, pb, pc;  // rotate as in k_down_fused
   fetch(t.k0 - 2, pa);
-  for(int s = -2; s <= last; s += 2) {
-    step(s, pa, pb);
+  fetch(t.k0 - 1, pb);
+  int s = -2;
+  for(; s + 2 <= last; s += 3) {
+    step(s, pa, pc);
     step(s + 1, pb, pa);
+    step(s + 2, pc, pb);
   }
+  if(s <= last) step(s, pa, pc);
+  if(s + 1 <= last) step(s + 1, pb, pa);
   const int ops[2] = {kSum, kSum};
   store_partials<2>(v, ops, partials);
 }
@@ -1284,6 +1331,25 @@ __global__ void __launch_bounds__(256) k_direction(const T* __restrict__ z, cons
   const int stride = gridDim.x * 256 * kPointIlp;
   for(int base = blockIdx.x * 256 * kPointIlp + threadIdx.x; base < n; base += stride) {
     double zt[kPointIlp], dv[kPointIlp], pv[kPointIlp];
+    if(base + (kPointIlp - 1) * 256 < n) {
+      // whole chunk in range: unconditional loads (d_old and p are valid memory even when
+      // `first` or `pend` leaves them unused), so they are issued together
+#pragma unroll
+      for(int e = 0; e < kPointIlp; ++e) {
+        const int id = base + e * 256;
+        zt[e] = (double)z[id];
+        dv[e] = d_old[id];
+        pv[e] = p[id];
+      }
+#pragma unroll
+      for(int e = 0; e < kPointIlp; ++e) {
+        const int id = base + e * 256;
+        const double zc = zt[e] - mz;
+        d_new[id] = first ? zc : zc + beta * (dv[e] - md);
+        if(pend) p[id] = pv[e] + alpha_prev * (dv[e] - md);
+      }
+      continue;
+    }
 #pragma unroll
     for(int e = 0; e < kPointIlp; ++e) {
       const int id = base + e * 256;
@@ -1304,6 +1370,57 @@ __global__ void __launch_bounds__(256) k_direction(const T* __restrict__ z, cons
   }
 }
 
+// The k march of k_apply_dot. FULL: the block owns KZ planes, so there is no per-plane bound
+// test (loads under such a test are waited for plane by plane). The pointers are restrict
+// parameters of this function, so the next planes' loads may be scheduled above the current
+// store.
+template <int KZ, bool UNI, bool FULL>
+__device__ __forceinline__ void apply_march(const double* __restrict__ d, double* __restrict__ q, const PLevel& L,
+                                            const HotTile& t, double (&v)[3]) {
+  const ZWin<UNI> zw(L);
+  const size_t plane = (size_t)L.nx * L.ny;
+  const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+  const Nb x = nb_x(L, t.i), y = nb_y(L, t.j);
+  const size_t oxm = (size_t)x.cm + (size_t)L.nx * t.j, oxp = (size_t)x.cp + (size_t)L.nx * t.j;
+  const size_t oym = (size_t)t.i + (size_t)L.nx * y.cm, oyp = (size_t)t.i + (size_t)L.nx * y.cp;
+  double prev = d[col + plane * zw.kprev(t.k0)];
+  double cur = d[col + plane * t.k0];
+  // software pipeline: the lateral neighbours of plane k + 1 and the centre of plane k + 2 are
+  // loaded before plane k is computed
+  double lw = d[oxm + plane * t.k0], le = d[oxp + plane * t.k0], ls = d[oym + plane * t.k0],
+         ln = d[oyp + plane * t.k0];
+  double next = d[col + plane * zw.knext(t.k0)];
+#pragma unroll
+  for(int kk = 0; kk < KZ; ++kk) {
+    const int k = t.k0 + kk;
+    if(FULL || k < t.k1) {
+      double nlw = 0.0, nle = 0.0, nls = 0.0, nln = 0.0, nnext = 0.0;
+      if(kk + 1 < KZ && (FULL || k + 1 < t.k1)) {
+        const size_t pk1 = plane * (size_t)(k + 1);
+        nlw = d[oxm + pk1];
+        nle = d[oxp + pk1];
+        nls = d[oym + pk1];
+        nln = d[oyp + pk1];
+        nnext = d[col + plane * zw.knext(k + 1)];
+      }
+      const size_t pk = plane * k;
+      const double qv = lap_neg<double>(cur, sgn(x.sm, lw), sgn(x.sp, le), sgn(y.sm, ls), sgn(y.sp, ln),
+                                        zw.below(k, prev, cur), zw.above(k, next, cur), L.ihx2, L.ihy2, L.ihz2);
+      q[col + pk] = qv;
+      v[0] += cur;
+      v[1] += qv;
+      v[2] += cur * qv;
+      prev = cur;
+      cur = next;
+      next = nnext;
+      lw = nlw;
+      le = nle;
+      ls = nls;
+      ln = nln;
+    }
+  }
+}
+
 // q = A d; sums of d, q, d.q.
 template <int KZ, bool UNI>
 __global__ void __launch_bounds__(256) k_apply_dot(const double* __restrict__ d, double* __restrict__ q, PLevel L,
@@ -1312,31 +1429,8 @@ __global__ void __launch_bounds__(256) k_apply_dot(const double* __restrict__ d,
   const HotTile t = hot_tile<KZ>(L);
   double v[3] = {0.0, 0.0, 0.0};
   if(t.in) {
-    const ZWin<UNI> zw(L);
-    const size_t plane = (size_t)L.nx * L.ny;
-    const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
-    const Nb x = nb_x(L, t.i), y = nb_y(L, t.j);
-    const size_t oxm = (size_t)x.cm + (size_t)L.nx * t.j, oxp = (size_t)x.cp + (size_t)L.nx * t.j;
-    const size_t oym = (size_t)t.i + (size_t)L.nx * y.cm, oyp = (size_t)t.i + (size_t)L.nx * y.cp;
-    double prev = d[col + plane * zw.kprev(t.k0)];
-    double cur = d[col + plane * t.k0];
-#pragma unroll
-    for(int kk = 0; kk < KZ; ++kk) {
-      const int k = t.k0 + kk;
-      if(k < t.k1) {
-        const double next = d[col + plane * zw.knext(k)];
-        const size_t pk = plane * k;
-        const double qv = lap_neg<double>(cur, sgn(x.sm, d[oxm + pk]), sgn(x.sp, d[oxp + pk]), sgn(y.sm, d[oym + pk]),
-                                          sgn(y.sp, d[oyp + pk]), zw.below(k, prev, cur), zw.above(k, next, cur),
-                                          L.ihx2, L.ihy2, L.ihz2);
-        q[col + pk] = qv;
-        v[0] += cur;
-        v[1] += qv;
-        v[2] += cur * qv;
-        prev = cur;
-        cur = next;
-      }
-    }
+    if(t.k1 - t.k0 == KZ) apply_march<KZ, UNI, true>(d, q, L, t, v);
+    else apply_march<KZ, UNI, false>(d, q, L, t, v);
   }
   const int ops[3] = {kSum, kSum, kSum};
   store_partials<3>(v, ops, partials);
@@ -1356,19 +1450,24 @@ __global__ void __launch_bounds__(256) k_update(double* __restrict__ r, const do
     const int cxy = bclass(t.i, L.nx) + 4 * bclass(t.j, L.ny);
     const size_t plane = (size_t)L.nx * L.ny;
     const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+    auto march = [&](auto full) {  // branch-free planes for a full k range (see k_apply_dot)
+      constexpr bool FULL = decltype(full)::value;
 #pragma unroll
-    for(int kk = 0; kk < KZ; ++kk) {
-      const int k = t.k0 + kk;
-      if(k < t.k1) {
-        const size_t id = col + plane * k;
-        const double rn = (r[id] - mr) + -alpha * (q[id] - mq);
-        r[id] = rn;
-        if(write_lo) b_lo[id] = T(rn);
-        v[0] += rn;
-        v[1] += rn * rn;
-        v[2] += rn * (UNI ? L.winv[0] : L.winv[cxy + 16 * bclass(k, L.nz)]);
+      for(int kk = 0; kk < KZ; ++kk) {
+        const int k = t.k0 + kk;
+        if(FULL || k < t.k1) {
+          const size_t id = col + plane * k;
+          const double rn = (r[id] - mr) + -alpha * (q[id] - mq);
+          r[id] = rn;
+          if(write_lo) b_lo[id] = T(rn);
+          v[0] += rn;
+          v[1] += rn * rn;
+          v[2] += rn * (UNI ? L.winv[0] : L.winv[cxy + 16 * bclass(k, L.nz)]);
+        }
       }
-    }
+    };
+    if(t.k1 - t.k0 == KZ) march(std::true_type{});
+    else march(std::false_type{});
   }
   const int ops[3] = {kSum, kSum, kSum};
   store_partials<3>(v, ops, partials);
