@@ -173,13 +173,15 @@ def kernel_bytes(name, n_cells, mixed):
         "k_apply_dot": 2 * f64,                         # d read; q written
         "k_direction": t + 2 * f64 + 2 * f64,           # z, d_old, p read; d_new, p written
         "k_update": 3 * f64 + (4 if mixed else 0),      # r, q read; r (+ FP32 copy) written
-        "k_smooth12": 2 * t,                            # b read; x2 written
+        "k_down_fused": 2 * t + t / 8,                  # b read; x2, coarse b written
+        "k_up_fused": 3 * t + t / 8,                    # b, x2, coarse x read; x5 written
+        "k_smooth12": 2 * t,                            # b read; x2 written (unfused levels)
         "k_residual_restrict": 2 * t + t / 8,           # b, x2 read; coarse b written
         "k_prolong_smooth": 3 * t + t / 8,              # b, x2, coarse x read; x4 written
         "k_smooth_last": 3 * t,                         # b, x4 read; x5 written
         "k_predictor_rhs": 7 * f64,                     # q, 2 other components, p, A_prev; A, rhs
-        "k_line_solve_x": 2 * f64,                      # line read and written once
-        "k_line_solve_s": 2 * f64,
+        "k_line_solve_x": 3 * 2 * f64,                  # 3 components: line read + written once
+        "k_line_solve_s": 3 * 2 * f64,
     }
     return table[name] * n_cells
 

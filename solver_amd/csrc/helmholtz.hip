@@ -66,259 +66,335 @@ std::vector<double> thomas_apply(const std::vector<double>& lo, const std::vecto
   return x;
 }
 
-// One thread per line. Forward elimination stores the modified right-hand side in place,
-// back substitution overwrites it with the solution; the cyclic case then applies the
-// rank-one correction sol = x - fac*z with fac from x[0], x[n-1].
-__global__ void k_line_solve(double* f, Lay L, int axis, LineSys S, const double* __restrict__ piv,
-                             const double* __restrict__ mu, const double* __restrict__ z) {
-  const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
-  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
-  const int t1 = blockIdx.y * blockDim.y + threadIdx.y;
-  if(t0 >= L.a(o0) || t1 >= L.a(o1)) return;
-  int c[3];
-  c[axis] = 0;
-  c[o0] = 1 + t0;
-  c[o1] = 1 + t1;
-  double* base = f + L.at(c[0], c[1], c[2]);
-  const ptrdiff_t st = axis == 0 ? 1 : axis == 1 ? (ptrdiff_t)L.sx : (ptrdiff_t)L.sx * L.sy;
-  const int n = S.n;
-  if(S.cyclic && n == 1) {
-    base[S.b * st] = base[S.b * st] / S.single_pivot;
-  } else {
-    // forward elimination
-    double r0 = base[S.b * st];
-    if(n == 1) {
-      r0 = r0 + S.add_lo;
-      r0 = r0 + S.add_hi;
+// ------------------------------------------------------------------ batched line solves
+// One sweep direction of up to three velocity components in one launch. Each thread owns one
+// line and solves it in two passes over HBM instead of three:
+//  * pass 1 reads the right-hand side top-down and runs the forward elimination
+//    mr_t = (r_t - lo*mr_{t-1}) / piv_t, keeping only mr at the end of every kSeg-element
+//    segment (checkpoints, 1/kSeg of the line); for a cyclic line it also accumulates
+//    x_0 = sum_t c_t mr_t with c_t = prod_{s<t} (-mu_s), the first element of the back
+//    substitution, so the Sherman-Morrison factor fac = (x_0 + uc*x_{n-1}/gamma) / den is
+//    known after this pass;
+//  * pass 2 walks the segments bottom-up: it re-reads the segment's right-hand side,
+//    recomputes its mr from the checkpoint (the same operations in the same order, so the
+//    same values), runs the back substitution x_t = mr_t - mu_t*x_{t+1} and stores the
+//    corrected solution x_t - fac*z_t once.
+// HBM traffic is about 25 bytes per element (read, read, write, checkpoints) against 48 for
+// separate forward, backward and correction passes. Only x_0 differs from the reference's
+// recurrences (solver/projection.cpp:522-617): it is summed in the forward pass rather than
+// produced by the back substitution, which changes its rounding, not its value.
+constexpr int kSeg = 16;      // checkpoint spacing = register chunk of one pass
+constexpr int kPlanVecs = 4;  // piv, mu, z, c per (component, axis) in the plan buffer
+constexpr int kAdiCheckpointSlot = 20;  // Context scratch slot of the checkpoints
+
+struct LineTask {
+  double* f;
+  double* ck;  // checkpoints [segment][line] of this task
+  Lay L;
+  LineSys S;
+  const double* coef;  // piv | mu | z | c, nmax apart
+  unsigned blocks;     // thread blocks of this task
+  int nlines;
+};
+struct LineBatch {
+  LineTask t[3];
+  int ntasks;
+  int nmax;
+};
+// Line coefficients are read-only during a sweep and indexed by the (uniform) element index:
+// reading them through the constant address space lets the compiler use scalar loads.
+typedef const double __attribute__((address_space(4)))* const_dptr;
+__device__ inline const_dptr as_constant(const double* p) { return (const_dptr)p; }
+struct LineCoef {
+  const_dptr piv;
+  const_dptr mu;
+  const_dptr zv;
+  const_dptr cx;
+  __device__ LineCoef(const double* c, int nmax)
+      : piv(as_constant(c)), mu(as_constant(c + nmax)), zv(as_constant(c + 2 * nmax)), cx(as_constant(c + 3 * nmax)) {}
+};
+
+// Element at a block-uniform base plus a per-lane byte offset below 2^32 (scalar base plus
+// 32-bit vector offset addressing: no per-element vector address arithmetic).
+__device__ inline double& at_bytes(double* base, unsigned boff) {
+  return *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + boff);
+}
+
+// Task of a block (tasks occupy consecutive block ranges); b becomes the block within it.
+// The task is assembled field by field from block-uniform selects, so it stays in scalar
+// registers (a copy of a whole indexed element would be placed in per-thread memory).
+__device__ inline LineTask line_task(const LineBatch& B, unsigned& b) {
+  int ti = 0;
+  if(B.ntasks > 1 && b >= B.t[0].blocks) {
+    b -= B.t[0].blocks;
+    ti = 1;
+    if(B.ntasks > 2 && b >= B.t[1].blocks) {
+      b -= B.t[1].blocks;
+      ti = 2;
+    }
+  }
+  LineTask T;
+#define OFH_PICK(fld) T.fld = ti == 0 ? B.t[0].fld : ti == 1 ? B.t[1].fld : B.t[2].fld
+  OFH_PICK(f);
+  OFH_PICK(ck);
+  OFH_PICK(L.ax);
+  OFH_PICK(L.ay);
+  OFH_PICK(L.az);
+  OFH_PICK(L.sx);
+  OFH_PICK(L.sy);
+  OFH_PICK(L.sz);
+  OFH_PICK(S.b);
+  OFH_PICK(S.e);
+  OFH_PICK(S.n);
+  OFH_PICK(S.cyclic);
+  OFH_PICK(S.copy_dup);
+  OFH_PICK(S.lo);
+  OFH_PICK(S.add_lo);
+  OFH_PICK(S.add_hi);
+  OFH_PICK(S.uc);
+  OFH_PICK(S.gamma);
+  OFH_PICK(S.den);
+  OFH_PICK(S.single_pivot);
+  OFH_PICK(coef);
+  OFH_PICK(blocks);
+  OFH_PICK(nlines);
+#undef OFH_PICK
+  return T;
+}
+
+// Forward elimination of elements [s0, s0 + kSeg) held in v (overwritten with mr); mr
+// carries mr_{s0-1} in and mr of the segment's last element out. EDGE: the segment holds
+// the first or the last element (boundary constants, end of line). ACC: accumulate x_0.
+template <bool EDGE, bool ACC>
+__device__ inline void line_forward(double (&v)[kSeg], int s0, int n, const LineSys& S, const LineCoef& C,
+                                    double& mr, double& x0) {
+#pragma unroll
+  for(int e = 0; e < kSeg; ++e) {
+    const int t = s0 + e;
+    if(EDGE && t >= n) break;
+    double r = v[e];
+    if(EDGE && t == 0) {
+      r = r + S.add_lo;
+      if(n == 1) r = r + S.add_hi;
+      mr = r / C.piv[0];
     } else {
-      r0 = r0 + S.add_lo;
+      if(EDGE && t == n - 1) r = r + S.add_hi;
+      mr = (r - S.lo * mr) / C.piv[t];
     }
-    double mr = r0 / piv[0];
-    base[S.b * st] = mr;
-    for(int t = 1; t < n; ++t) {
-      double r = base[(S.b + t) * st];
-      if(t == n - 1) r = r + S.add_hi;
-      mr = (r - S.lo * mr) / piv[t];
-      base[(S.b + t) * st] = mr;
-    }
-    // back substitution
-    double x = mr;
-    const double x_last = x;
-    for(int t = n - 2; t >= 0; --t) {
-      x = base[(S.b + t) * st] - mu[t] * x;
-      base[(S.b + t) * st] = x;
-    }
-    if(S.cyclic) {
-      const double fac = (x + S.uc * x_last / S.gamma) / S.den;
-      for(int t = 0; t < n; ++t) base[(S.b + t) * st] = base[(S.b + t) * st] - fac * z[t];
-    }
-  }
-  if(S.copy_dup) base[S.e * st] = base[S.b * st];
-}
-
-// x-lines (unit stride along the line): one wave owns 64 lines; the line data moves through
-// LDS in chunks of 64 lines x kLX elements so that every global access is a coalesced row
-// segment (loads issued in batches of 16), and each lane then runs the recurrence of its own
-// line out of LDS. Same arithmetic as k_line_solve.
-constexpr int kLT = 64;
-constexpr int kLX = 32;
-__global__ void __launch_bounds__(kLT) k_line_solve_x(double* f, Lay L, LineSys S, const double* __restrict__ piv,
-                                                      const double* __restrict__ mu, const double* __restrict__ z) {
-  __shared__ double T[kLT][kLX + 1];
-  const int lane = threadIdx.x;
-  const int nl1 = L.ay, nlines = L.ay * L.az;
-  const int line0 = blockIdx.x * kLT;
-  const int nr = min(kLT, nlines - line0);
-  const int n = S.n;
-  const bool mine = lane < nr;
-  // storage offset of this lane's line (32-bit: storage sizes stay below 2^31 elements)
-  const int my_off = mine ? (int)L.at(0, 1 + (line0 + lane) % nl1, 1 + (line0 + lane) / nl1) : 0;
-  if(S.cyclic && n == 1) {
-    if(mine) {
-      f[my_off + S.b] = f[my_off + S.b] / S.single_pivot;
-      if(S.copy_dup) f[my_off + S.e] = f[my_off + S.b];
-    }
-    return;
-  }
-  const int col = lane % kLX, half = lane / kLX;  // two lanes per row segment of kLX elements
-  auto load = [&](int c0, int cn) {
-    for(int r0 = 0; r0 < kLT; r0 += 32) {
-      double v[16];
-#pragma unroll
-      for(int e = 0; e < 16; ++e) {
-        const int r = r0 + 2 * e + half;
-        const int off = __shfl(my_off, r, 64);
-        v[e] = (r < nr && col < cn) ? f[off + S.b + c0 + col] : 0.0;
-      }
-#pragma unroll
-      for(int e = 0; e < 16; ++e) T[r0 + 2 * e + half][col] = v[e];
-    }
-    __syncthreads();
-  };
-  auto store = [&](int c0, int cn) {
-    __syncthreads();
-    for(int r0 = 0; r0 < kLT; r0 += 32) {
-#pragma unroll
-      for(int e = 0; e < 16; ++e) {
-        const int r = r0 + 2 * e + half;
-        const int off = __shfl(my_off, r, 64);
-        if(r < nr && col < cn) f[off + S.b + c0 + col] = T[r][col];
-      }
-    }
-    __syncthreads();
-  };
-  double mr = 0.0;
-  for(int c0 = 0; c0 < n; c0 += kLX) {  // forward elimination
-    const int cn = min(kLX, n - c0);
-    load(c0, cn);
-    if(mine) {
-      for(int t = 0; t < cn; ++t) {
-        const int tg = c0 + t;
-        double r = T[lane][t];
-        if(tg == 0) {
-          if(n == 1) {
-            r = r + S.add_lo;
-            r = r + S.add_hi;
-          } else {
-            r = r + S.add_lo;
-          }
-          mr = r / piv[0];
-        } else {
-          if(tg == n - 1) r = r + S.add_hi;
-          mr = (r - S.lo * mr) / piv[tg];
-        }
-        T[lane][t] = mr;
-      }
-    }
-    store(c0, cn);
-  }
-  double x = mr;  // back substitution; x[n-1] = mr[n-1] is stored already
-  const double x_last = x;
-  for(int c0 = ((n - 1) / kLX) * kLX; c0 >= 0; c0 -= kLX) {
-    const int cn = min(kLX, n - c0);
-    load(c0, cn);
-    if(mine) {
-      for(int t = cn - 1; t >= 0; --t) {
-        const int tg = c0 + t;
-        if(tg == n - 1) continue;
-        x = T[lane][t] - mu[tg] * x;
-        T[lane][t] = x;
-      }
-    }
-    store(c0, cn);
-  }
-  if(S.cyclic) {  // Sherman-Morrison correction
-    const double fac = (x + S.uc * x_last / S.gamma) / S.den;
-    double first = 0.0;
-    for(int c0 = 0; c0 < n; c0 += kLX) {
-      const int cn = min(kLX, n - c0);
-      load(c0, cn);
-      if(mine) {
-        for(int t = 0; t < cn; ++t) T[lane][t] = T[lane][t] - fac * z[c0 + t];
-        if(c0 == 0) first = T[lane][0];
-      }
-      store(c0, cn);
-    }
-    if(S.copy_dup && mine) f[my_off + S.e] = first;
+    v[e] = mr;
+    if(ACC) x0 = fma(C.cx[t], mr, x0);
   }
 }
 
-// y- and z-lines (lanes along i: coalesced): one thread per line as in k_line_solve, with the
-// line streamed through double-buffered register chunks so that the loads of the next chunk
-// overlap the recurrence of the current one.
-constexpr int kLC = 16;
-__global__ void __launch_bounds__(256) k_line_solve_s(double* f, Lay L, int axis, LineSys S,
-                                                      const double* __restrict__ piv, const double* __restrict__ mu,
-                                                      const double* __restrict__ z) {
+// Back substitution and correction of the segment: v holds mr in, the solution out; x
+// carries x_{s0+kSeg} in and x_{s0} out. EDGE: the segment holds the last element.
+template <bool EDGE>
+__device__ inline void line_backward(double (&v)[kSeg], int s0, int n, const LineCoef& C, double fac, double& x) {
+#pragma unroll
+  for(int e = kSeg - 1; e >= 0; --e) {
+    const int t = s0 + e;
+    if(EDGE && t >= n) continue;
+    if(EDGE && t == n - 1) x = v[e];
+    else x = v[e] - C.mu[t] * x;
+    v[e] = x - fac * C.zv[t];
+  }
+}
+
+// Pass-1 step for one segment (forward elimination; mr leaves as the checkpoint).
+__device__ inline void line_pass1(double (&v)[kSeg], int s0, int n, const LineSys& S, const LineCoef& C, double& mr,
+                                  double& x0) {
+  if(s0 > 0 && s0 + kSeg < n) {
+    if(S.cyclic) line_forward<false, true>(v, s0, n, S, C, mr, x0);
+    else line_forward<false, false>(v, s0, n, S, C, mr, x0);
+  } else {
+    if(S.cyclic) line_forward<true, true>(v, s0, n, S, C, mr, x0);
+    else line_forward<true, false>(v, s0, n, S, C, mr, x0);
+  }
+}
+
+// Pass-2 step for one segment: recompute mr from the checkpoint mr_{s0-1}, then back-substitute.
+__device__ inline void line_pass2(double (&v)[kSeg], int s0, int n, const LineSys& S, const LineCoef& C, double mr_in,
+                                  double fac, double& x) {
+  double mr = mr_in, unused = 0.0;
+  if(s0 > 0 && s0 + kSeg < n) {
+    line_forward<false, false>(v, s0, n, S, C, mr, unused);
+    line_backward<false>(v, s0, n, C, fac, x);
+  } else {
+    line_forward<true, false>(v, s0, n, S, C, mr, unused);
+    if(s0 + kSeg >= n) line_backward<true>(v, s0, n, C, fac, x);
+    else line_backward<false>(v, s0, n, C, fac, x);
+  }
+}
+
+__device__ inline double sherman_morrison_factor(const LineSys& S, double x0, double x_last) {
+  return S.cyclic ? (x0 + S.uc * x_last / S.gamma) / S.den : 0.0;
+}
+
+// y- and z-lines: lanes along i, so every load and store of a wave is a coalesced 512-byte
+// row segment. The element address is a block-uniform pointer plus the lane's 32-bit offset,
+// so the strided walk costs no vector registers. A pass double-buffers its segments in
+// registers: a segment's loads overlap the recurrence of the previous one.
+constexpr int kLSY = 4;
+__global__ void __launch_bounds__(64 * kLSY) k_line_solve_s(LineBatch B, int axis) {
+  unsigned b = blockIdx.x;
+  const LineTask T = line_task(B, b);
+  const Lay L = T.L;
+  const LineSys S = T.S;
   const int o1 = axis == 2 ? 1 : 2;
-  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
-  const int t1 = blockIdx.y * blockDim.y + threadIdx.y;
-  if(t0 >= L.ax || t1 >= L.a(o1)) return;
-  int cc[3];
-  cc[axis] = 0;
-  cc[0] = 1 + t0;
-  cc[o1] = 1 + t1;
-  double* base = f + L.at(cc[0], cc[1], cc[2]);
+  const unsigned gx = (unsigned)(L.ax + 63) / 64u;
+  const int bx0 = (int)(b % gx) * 64, b10 = (int)(b / gx) * kLSY;
+  const int t0 = bx0 + threadIdx.x, t1 = b10 + threadIdx.y;
+  if(t0 >= L.ax || t1 >= L.a(o1)) return;  // no block-level synchronisation below
   const ptrdiff_t st = axis == 1 ? (ptrdiff_t)L.sx : (ptrdiff_t)L.sx * L.sy;
+  const ptrdiff_t so1 = o1 == 1 ? (ptrdiff_t)L.sx : (ptrdiff_t)L.sx * L.sy;
+  // block origin (uniform): i = 1 + bx0, the line axis at S.b, the other axis at 1 + b10
+  double* const ub = T.f + L.at(1 + bx0, axis == 1 ? S.b : 1 + b10, axis == 1 ? 1 + b10 : S.b);
+  const unsigned boff = 8u * (threadIdx.x + (unsigned)(threadIdx.y * so1));  // lane byte offset
+  auto at = [&](int t) -> double& { return at_bytes(ub + (ptrdiff_t)t * st, boff); };
   const int n = S.n;
   if(S.cyclic && n == 1) {
-    base[S.b * st] = base[S.b * st] / S.single_pivot;
-    if(S.copy_dup) base[S.e * st] = base[S.b * st];
+    at(0) = at(0) / S.single_pivot;
+    if(S.copy_dup) at(S.e - S.b) = at(0);
     return;
   }
-  auto ld = [&](int c0, double(&v)[kLC]) {
+  const LineCoef C(T.coef, B.nmax);
+  double* const ckb = T.ck + (size_t)b10 * L.ax + bx0;  // checkpoints [segment][line]
+  const unsigned cboff = 8u * (threadIdx.x + threadIdx.y * (unsigned)L.ax);
+  auto ck = [&](int sg) -> double& { return at_bytes(ckb + (size_t)sg * T.nlines, cboff); };
+  const int nseg = (n + kSeg - 1) / kSeg;
+  auto ld = [&](double(&v)[kSeg], int sg) {
+    const int s0 = sg * kSeg;
 #pragma unroll
-    for(int e = 0; e < kLC; ++e)
-      if(c0 + e < n) v[e] = base[(ptrdiff_t)(S.b + c0 + e) * st];
+    for(int e = 0; e < kSeg; ++e) v[e] = at(min(s0 + e, n - 1));
   };
-  auto stv = [&](int c0, const double(&v)[kLC]) {
+  auto sv = [&](const double(&v)[kSeg], int sg) {
+    const int s0 = sg * kSeg;
+    if(s0 + kSeg <= n) {
 #pragma unroll
-    for(int e = 0; e < kLC; ++e)
-      if(c0 + e < n) base[(ptrdiff_t)(S.b + c0 + e) * st] = v[e];
+      for(int e = 0; e < kSeg; ++e) at(s0 + e) = v[e];
+    } else {
+#pragma unroll
+      for(int e = 0; e < kSeg; ++e)
+        if(s0 + e < n) at(s0 + e) = v[e];
+    }
   };
-  double cur[kLC], nxt[kLC];
-  double mr = 0.0;
-  ld(0, cur);
-  for(int c0 = 0; c0 < n; c0 += kLC) {  // forward elimination
-    if(c0 + kLC < n) ld(c0 + kLC, nxt);
-#pragma unroll
-    for(int e = 0; e < kLC; ++e) {
-      const int tg = c0 + e;
-      if(tg < n) {
-        double r = cur[e];
-        if(tg == 0) {
-          if(n == 1) {
-            r = r + S.add_lo;
-            r = r + S.add_hi;
-          } else {
-            r = r + S.add_lo;
-          }
-          mr = r / piv[0];
-        } else {
-          if(tg == n - 1) r = r + S.add_hi;
-          mr = (r - S.lo * mr) / piv[tg];
-        }
-        cur[e] = mr;
-      }
-    }
-    stv(c0, cur);
-#pragma unroll
-    for(int e = 0; e < kLC; ++e) cur[e] = nxt[e];
+  double va[kSeg], vb[kSeg];
+  double mr = 0.0, x0 = 0.0;
+  ld(va, 0);
+  for(int sg = 0; sg < nseg; sg += 2) {  // pass 1
+    if(sg + 1 < nseg) ld(vb, sg + 1);
+    line_pass1(va, sg * kSeg, n, S, C, mr, x0);
+    ck(sg) = mr;
+    if(sg + 1 >= nseg) break;
+    if(sg + 2 < nseg) ld(va, sg + 2);
+    line_pass1(vb, (sg + 1) * kSeg, n, S, C, mr, x0);
+    ck(sg + 1) = mr;
   }
-  double x = mr;  // back substitution
-  const double x_last = x;
-  const int clast = ((n - 1) / kLC) * kLC;
-  ld(clast, cur);
-  for(int c0 = clast; c0 >= 0; c0 -= kLC) {
-    if(c0 > 0) ld(c0 - kLC, nxt);
-#pragma unroll
-    for(int e = kLC - 1; e >= 0; --e) {
-      const int tg = c0 + e;
-      if(tg < n - 1) {
-        x = cur[e] - mu[tg] * x;
-        cur[e] = x;
-      }
+  const double fac = sherman_morrison_factor(S, x0, mr);
+  double x = 0.0, first = 0.0;
+  ld(va, nseg - 1);
+  for(int sg = nseg - 1; sg >= 0; sg -= 2) {  // pass 2
+    if(sg >= 1) ld(vb, sg - 1);
+    line_pass2(va, sg * kSeg, n, S, C, sg > 0 ? ck(sg - 1) : 0.0, fac, x);
+    sv(va, sg);
+    if(sg == 0) {
+      first = va[0];
+      break;
     }
-    stv(c0, cur);
-#pragma unroll
-    for(int e = 0; e < kLC; ++e) cur[e] = nxt[e];
+    if(sg >= 2) ld(va, sg - 2);
+    line_pass2(vb, (sg - 1) * kSeg, n, S, C, sg > 1 ? ck(sg - 2) : 0.0, fac, x);
+    sv(vb, sg - 1);
+    if(sg == 1) first = vb[0];
   }
-  if(S.cyclic) {
-    const double fac = (x + S.uc * x_last / S.gamma) / S.den;
-    double first = 0.0;
-    ld(0, cur);
-    for(int c0 = 0; c0 < n; c0 += kLC) {
-      if(c0 + kLC < n) ld(c0 + kLC, nxt);
-#pragma unroll
-      for(int e = 0; e < kLC; ++e)
-        if(c0 + e < n) cur[e] = cur[e] - fac * z[c0 + e];
-      if(c0 == 0) first = cur[0];
-      stv(c0, cur);
-#pragma unroll
-      for(int e = 0; e < kLC; ++e) cur[e] = nxt[e];
+  if(S.copy_dup) at(S.e - S.b) = first;
+}
+
+// x-lines (unit stride along the line): one wave owns 64 lines. Each segment moves between
+// HBM and a 64 x kSeg LDS tile in row segments of kSeg elements (whole 128-byte lines), and
+// every lane then runs its own line's recurrence on a register copy of its tile row. The
+// next segment's loads are issued before the current segment is computed.
+constexpr int kXR = 64 / kSeg;  // tile rows covered by one load instruction
+__global__ void __launch_bounds__(64) k_line_solve_x(LineBatch B) {
+  __shared__ double tile[64][kSeg + 1];
+  unsigned b = blockIdx.x;
+  const LineTask T = line_task(B, b);
+  const Lay L = T.L;
+  const LineSys S = T.S;
+  const int nl1 = L.ay;
+  const int line0 = (int)b * 64;
+  const int nr = min(64, T.nlines - line0);
+  const int lane = threadIdx.x;
+  const int myl = line0 + min(lane, nr - 1);  // rows past the last line repeat it (loads only)
+  // element offsets relative to the block's first line (the host bounds their span)
+  const size_t base_off = L.at(S.b, 1 + line0 % nl1, 1 + line0 / nl1);
+  const unsigned my_off = (unsigned)(L.at(S.b, 1 + myl % nl1, 1 + myl / nl1) - base_off);
+  double* const f = T.f + base_off;
+  const int n = S.n;
+  if(S.cyclic && n == 1) {
+    if(lane < nr) {
+      f[my_off] = f[my_off] / S.single_pivot;
+      if(S.copy_dup) f[my_off + (S.e - S.b)] = f[my_off];
     }
-    if(S.copy_dup) base[S.e * st] = first;
+    return;
   }
+  const LineCoef C(T.coef, B.nmax);
+  double* const ckb = T.ck + line0;  // checkpoints [segment][line]
+  auto ck = [&](int sg) -> double& { return at_bytes(ckb + (size_t)sg * T.nlines, 8u * lane); };
+  const int nseg = (n + kSeg - 1) / kSeg;
+  const int col = lane % kSeg, sub = lane / kSeg;
+  unsigned roff[kSeg];  // byte offsets of the rows this lane moves
+#pragma unroll
+  for(int e = 0; e < kSeg; ++e) roff[e] = 8u * (unsigned)__shfl((int)my_off, kXR * e + sub, 64);
+  double pre[kSeg];
+  auto ld = [&](int sg) {
+    const int s0 = sg * kSeg;
+    double* const fs = f + s0 + min(col, n - 1 - s0);  // clamped: loads stay inside the line
+#pragma unroll
+    for(int e = 0; e < kSeg; ++e) pre[e] = at_bytes(fs, roff[e]);
+  };
+  auto to_tile = [&]() {
+#pragma unroll
+    for(int e = 0; e < kSeg; ++e) tile[kXR * e + sub][col] = pre[e];
+  };
+  auto store_tile = [&](int sg) {
+    const int s0 = sg * kSeg;
+    const bool cok = s0 + col < n;
+    double* const fs = f + s0 + col;
+#pragma unroll
+    for(int e = 0; e < kSeg; ++e)
+      if(kXR * e + sub < nr && cok) at_bytes(fs, roff[e]) = tile[kXR * e + sub][col];
+  };
+  double v[kSeg];
+  double mr = 0.0, x0 = 0.0;
+  ld(0);
+  for(int sg = 0; sg < nseg; ++sg) {  // pass 1
+    __syncthreads();
+    to_tile();
+    __syncthreads();
+    if(sg + 1 < nseg) ld(sg + 1);
+#pragma unroll
+    for(int e = 0; e < kSeg; ++e) v[e] = tile[lane][e];
+    line_pass1(v, sg * kSeg, n, S, C, mr, x0);
+    if(lane < nr) ck(sg) = mr;
+  }
+  const double fac = sherman_morrison_factor(S, x0, mr);
+  double x = 0.0, first = 0.0;
+  ld(nseg - 1);
+  for(int sg = nseg - 1; sg >= 0; --sg) {  // pass 2
+    __syncthreads();
+    to_tile();
+    __syncthreads();
+    if(sg >= 1) ld(sg - 1);
+#pragma unroll
+    for(int e = 0; e < kSeg; ++e) v[e] = tile[lane][e];
+    line_pass2(v, sg * kSeg, n, S, C, sg > 0 && lane < nr ? ck(sg - 1) : 0.0, fac, x);
+    if(sg == 0) first = v[0];
+#pragma unroll
+    for(int e = 0; e < kSeg; ++e) tile[lane][e] = v[e];
+    __syncthreads();
+    store_tile(sg);
+  }
+  if(S.copy_dup && lane < nr) f[my_off + (S.e - S.b)] = first;
 }
 
 inline bool dirichlet_velocity(const of_bc& c) {  // solver/projection.cpp:45-49
@@ -367,7 +443,7 @@ AdiPlan build_adi_plan(const of_grid& g, const of_bc_set& b, double alpha) {
   AdiPlan P;
   P.nmax = std::max(g.nx, std::max(g.ny, g.nz)) + 2;
   const int nmax = P.nmax;
-  std::vector<double> host(3 * (size_t)nmax * 9, 0.0);
+  std::vector<double> host((size_t)kPlanVecs * nmax * 9, 0.0);
   for(int comp = 0; comp < 3; ++comp) {
     const Lay L = make_lay(g, OF_FACE_X + comp);
     for(int axis = 0; axis < 3; ++axis) {
@@ -457,10 +533,23 @@ AdiPlan build_adi_plan(const of_grid& g, const of_bc_set& b, double alpha) {
         thomas_factors(lo, di, up, piv, mu);
         zv.assign(n, 0.0);
       }
-      const size_t off = (size_t)(comp * 3 + axis) * 3 * nmax;
+      // per-lane byte offsets of the line kernels are 32-bit: a block's rows must span < 4 GiB
+      const double span = 8.0 * L.sx * L.sy * (64.0 / L.ay + 2.0) + 8.0 * L.sx;
+      if(span >= 4294967296.0) throw InvalidArgument("ADI grid planes are too large for the line kernels");
+      // c_t = prod_{s<t} (-mu_s): x_0 of the back substitution is sum_t c_t mr_t
+      std::vector<double> cx(S.n, 0.0);
+      if(S.cyclic) {
+        double cprod = 1.0;
+        for(int t = 0; t < S.n; ++t) {
+          cx[t] = cprod;
+          cprod *= -mu[t];
+        }
+      }
+      const size_t off = (size_t)(comp * 3 + axis) * kPlanVecs * nmax;
       std::copy(piv.begin(), piv.end(), host.begin() + off);
       std::copy(mu.begin(), mu.end(), host.begin() + off + nmax);
       std::copy(zv.begin(), zv.end(), host.begin() + off + 2 * nmax);
+      std::copy(cx.begin(), cx.end(), host.begin() + off + 3 * nmax);
       const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
       P.line_solves += L.a(o0) * L.a(o1);
       P.sweeps.push_back(Pending{comp, axis, S, off});
@@ -524,20 +613,44 @@ AdiPlan plan_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha)
   return g_adi_cache.back()->plan;
 }
 
-void launch_sweep(Context& c, const Lay& L, const Pending& p, const AdiPlan& P, double* f, const LineSys& S) {
-  const int o1 = p.axis == 2 ? 1 : 2;
-  const double* piv = P.dev + p.off;
-  const double* mu = P.dev + p.off + P.nmax;
-  const double* zv = P.dev + p.off + 2 * P.nmax;
-  if(p.axis == 0) {
+// Line solves of every component along `axis` (one launch); returns the mask of the
+// components swept. no_dup drops the duplicated-face copy (the transposed solve folds it
+// back beforehand).
+int launch_axis_sweeps(Context& c, const of_grid& g, const AdiPlan& P, int axis, double* const out[3], bool no_dup) {
+  LineBatch B{};
+  B.nmax = P.nmax;
+  unsigned blocks = 0;
+  size_t ck_total = 0;
+  size_t ck_off[3];
+  int mask = 0;
+  const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
+  for(const Pending& p : P.sweeps) {
+    if(p.axis != axis) continue;
+    LineTask& T = B.t[B.ntasks];
+    T.f = out[p.comp];
+    T.L = make_lay(g, OF_FACE_X + p.comp);
+    T.S = p.S;
+    if(no_dup) T.S.copy_dup = 0;
+    T.coef = P.dev + p.off;
+    T.nlines = T.L.a(o0) * T.L.a(o1);
+    T.blocks = axis == 0 ? div_up((size_t)T.nlines, 64) : div_up((size_t)T.L.ax, 64) * div_up((size_t)T.L.a(o1), kLSY);
+    ck_off[B.ntasks++] = ck_total;
+    ck_total += (size_t)T.nlines * (size_t)((p.S.n + kSeg - 1) / kSeg);
+    blocks += T.blocks;
+    mask |= 1 << p.comp;
+  }
+  if(B.ntasks == 0) return 0;
+  // checkpoint scratch of the context (grow-only; one launch at a time on its stream)
+  double* ck = static_cast<double*>(c.scratch_get(kAdiCheckpointSlot, ck_total * sizeof(double)));
+  for(int t = 0; t < B.ntasks; ++t) B.t[t].ck = ck + ck_off[t];
+  if(axis == 0) {
     ProfScope prof_(c, "k_line_solve_x", true);
-    hipLaunchKernelGGL(k_line_solve_x, dim3(div_up((size_t)L.ay * L.az, kLT)), dim3(kLT), 0, c.stream, f, L, S, piv,
-                       mu, zv);
+    hipLaunchKernelGGL(k_line_solve_x, dim3(blocks), dim3(64), 0, c.stream, B);
   } else {
     ProfScope prof_(c, "k_line_solve_s", true);
-    const dim3 grid((unsigned)((L.ax + 63) / 64), (unsigned)((L.a(o1) + 3) / 4), 1);
-    hipLaunchKernelGGL(k_line_solve_s, grid, dim3(64, 4, 1), 0, c.stream, f, L, p.axis, S, piv, mu, zv);
+    hipLaunchKernelGGL(k_line_solve_s, dim3(blocks), dim3(64, kLSY), 0, c.stream, B, axis);
   }
+  return mask;
 }
 
 // Transpose of the duplicated-face copy of a periodic normal sweep (x[e] := x[b]):
@@ -570,14 +683,13 @@ int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha
     const Lay L = make_lay(g, OF_FACE_X + comp);
     OFH_CHECK(hipMemcpyAsync(out[comp], rhs[comp], L.size() * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
   }
-  for(int comp = 0; comp < 3; ++comp) {
-    launch_component_bc(c, g, b, comp, out[comp]);
-    if(alpha == 0.0) continue;
-    const Lay L = make_lay(g, OF_FACE_X + comp);
-    for(const Pending& p : P.sweeps) {
-      if(p.comp != comp) continue;
-      launch_sweep(c, L, p, P, out[comp], p.S);
-      launch_component_bc(c, g, b, comp, out[comp]);
+  // The components are independent, so the reference's per-component order (ghost fill, then
+  // x, y, z sweeps each followed by a ghost fill) is run axis by axis for all of them.
+  launch_velocity_bc(c, g, b, out[0], out[1], out[2]);
+  if(alpha != 0.0) {
+    for(int axis = 0; axis < 3; ++axis) {
+      const int mask = launch_axis_sweeps(c, g, P, axis, out, false);
+      if(mask) launch_component_bc_mask(c, g, b, out[0], out[1], out[2], mask);
     }
   }
   OFH_CHECK(hipGetLastError());
@@ -592,26 +704,27 @@ void predictor_adi_adjoint(Context& c, const of_grid& g, const of_bc_set& b, dou
   for(int a = 0; a < 3; ++a)
     if(!axis_periodic(b, a)) throw InvalidArgument("the ADI adjoint supports periodic boundaries");
   const AdiPlan P = plan_adi(c, g, b, alpha);
+  double* out[3];
   for(int comp = 0; comp < 3; ++comp) {
     const Lay L = make_lay(g, OF_FACE_X + comp);
+    out[comp] = rbar[comp];
     if(rbar[comp] != ybar[comp])
       OFH_CHECK(hipMemcpyAsync(rbar[comp], ybar[comp], L.size() * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     launch_component_bc_adjoint(c, g, comp, rbar[comp]);
-    if(alpha == 0.0) continue;
-    for(auto it = P.sweeps.rbegin(); it != P.sweeps.rend(); ++it) {
-      const Pending& p = *it;
-      if(p.comp != comp) continue;
-      LineSys S = p.S;
-      if(S.copy_dup) {
-        const int o0 = p.axis == 0 ? 1 : 0, o1 = p.axis == 2 ? 1 : 2;
-        const dim3 block(64, 4, 1);
-        const dim3 grid((unsigned)((L.a(o0) + 63) / 64), (unsigned)((L.a(o1) + 3) / 4), 1);
-        hipLaunchKernelGGL(k_dup_fold, grid, block, 0, c.stream, rbar[comp], L, p.axis, S.b, S.e);
-        S.copy_dup = 0;
-      }
-      launch_sweep(c, L, p, P, rbar[comp], S);
-      launch_component_bc_adjoint(c, g, comp, rbar[comp]);
+  }
+  if(alpha == 0.0) return;
+  for(int axis = 2; axis >= 0; --axis) {
+    for(const Pending& p : P.sweeps) {
+      if(p.axis != axis || !p.S.copy_dup) continue;
+      const Lay L = make_lay(g, OF_FACE_X + p.comp);
+      const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
+      const dim3 block(64, 4, 1);
+      const dim3 grid((unsigned)((L.a(o0) + 63) / 64), (unsigned)((L.a(o1) + 3) / 4), 1);
+      hipLaunchKernelGGL(k_dup_fold, grid, block, 0, c.stream, rbar[p.comp], L, axis, p.S.b, p.S.e);
     }
+    const int mask = launch_axis_sweeps(c, g, P, axis, out, true);
+    for(int comp = 0; comp < 3; ++comp)
+      if(mask & (1 << comp)) launch_component_bc_adjoint(c, g, comp, rbar[comp]);
   }
   OFH_CHECK(hipGetLastError());
 }

@@ -61,6 +61,9 @@ ColumnLaunch column_launch(int ex, int ey, int ez, int min_kz = 1);
 // ------------------------------------------------------------------ fields.hip
 void launch_velocity_bc(Context& c, const of_grid& g, const of_bc_set& b, double* u, double* v, double* w);
 void launch_component_bc(Context& c, const of_grid& g, const of_bc_set& b, int comp, double* f);
+// Ghost fill of the components selected by bit c of `mask` (the others may be null).
+void launch_component_bc_mask(Context& c, const of_grid& g, const of_bc_set& b, double* u, double* v, double* w,
+                              int mask);
 void launch_pressure_bc(Context& c, const of_grid& g, const of_pressure_bc_set& b, double* p);
 void launch_total_pressure_bc(Context& c, const of_grid& g, const of_bc_set& b, const double* su, const double* sv,
                               const double* sw, double* p);
