@@ -401,23 +401,30 @@ constexpr int kRX = 64, kRY = 8, kRKZ = 8;
 constexpr int kRHX = kRX + 4, kRHY = kRY + 4;
 
 // reconstruct() on explicit values: ll, l, r, rr are q at indices r_index-2 .. r_index+1
-// (ll / rr are read only when those indices lie in storage).
+// (ll / rr enter only when those indices lie in storage). Branch-free: both upwind
+// directions share one ratio and one limiter evaluation, selected by the sign of vel, so
+// waves whose face velocities change sign do not run both paths; each selected value is the
+// same expression as in reconstruct().
 __device__ inline double reconstruct4(double ll, double left, double right, double rr, int r_index, int max_index,
                                       double vel, int scheme) {
   if(scheme == 2) return 0.5 * (left + right);
-  if(scheme == 1 || fabs(vel) < kSmall) return vel >= 0.0 ? left : right;
-  if(vel >= 0.0) {
-    if(r_index - 2 < 0) return left;
-    const double dd = right - left;
-    const double ratio = safe_ratio(left - ll, dd);
-    return left + 0.5 * van_leer(ratio) * dd;
-  }
-  if(r_index + 1 > max_index) return right;
+  const bool up = vel >= 0.0;
+  const double upwind = up ? left : right;
+  if(scheme == 1) return upwind;
   const double d = right - left;
-  const double ratio = safe_ratio(rr - right, d);
-  return right - 0.5 * van_leer(ratio) * d;
+  const double ratio = safe_ratio(up ? left - ll : rr - right, d);
+  const double lim = 0.5 * van_leer(ratio);
+  const double tvd = up ? left + lim * d : right - lim * d;
+  const bool plain = fabs(vel) < kSmall || (up ? r_index - 2 < 0 : r_index + 1 > max_index);
+  return plain ? upwind : tvd;
 }
 
+// Loads are issued well before their use: the q plane k+3 is fetched into registers while
+// plane k is computed (and committed to the LDS ring at the top of the next iteration), and
+// the point values of plane k (the other two components, p, A_prev) are requested before the
+// plane's LDS phases. Every load is unconditional at clamped indices (threads outside the
+// active range read valid storage and store nothing); out-of-storage halo values are masked
+// to zero afterwards, so no load waits inside a divergent branch.
 __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, const double* __restrict__ p, Lay Lp,
                                                                    const double* prev, double* adv,
                                                                    double* __restrict__ rhs, RhsParams P) {
@@ -438,24 +445,94 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + kRX * ty;
   const int i = i0 + tx, j = j0 + ty;
   const bool in = i <= L.ax && j <= L.ay;
+  const int ic = min(i, L.ax), jc = min(j, L.ay);  // load coordinates
   const double inv0 = P.fc.inv[0], inv1 = P.fc.inv[1], inv2 = P.fc.inv[2];
   const bool three_d = P.fc.three_d != 0;
   auto slot = [](int kk) { return (kk + 10) % 5; };
-  auto load_plane = [&](int kk) {
-    double* dst = &sq[slot(kk)][0][0];
-    for(int e = tid; e < kRHY * kRHX; e += kRX * kRY) {
-      const int hy = e / kRHX, hx = e - hy * kRHX;
-      const int gi = i0 - 2 + hx, gj = j0 - 2 + hy;
-      double val = 0.0;
-      if(kk >= 0 && kk < L.sz && gi >= 0 && gi < L.sx && gj >= 0 && gj < L.sy) val = q[L.at(gi, gj, kk)];
-      dst[e] = val;
+  // staged plane elements of this thread: e = tid + m * threads
+  constexpr int kPlaneElems = kRHY * kRHX;
+  constexpr int kPE = (kPlaneElems + kRX * kRY - 1) / (kRX * kRY);
+  const size_t plane = (size_t)L.sx * L.sy;
+  size_t qoff[kPE];
+  bool okxy[kPE];
+#pragma unroll
+  for(int m = 0; m < kPE; ++m) {
+    const int e = min(tid + m * kRX * kRY, kPlaneElems - 1);
+    const int hy = e / kRHX, hx = e - hy * kRHX;
+    const int gi = i0 - 2 + hx, gj = j0 - 2 + hy;
+    okxy[m] = gi >= 0 && gi < L.sx && gj >= 0 && gj < L.sy;
+    qoff[m] = L.at(min(max(gi, 0), L.sx - 1), min(max(gj, 0), L.sy - 1), 0);
+  }
+  double qraw[kPE];
+  bool qok[kPE];
+  auto fetch = [&](int kk) {
+    const bool okz = kk >= 0 && kk < L.sz;
+    const size_t kofs = plane * (size_t)min(max(kk, 0), L.sz - 1);
+#pragma unroll
+    for(int m = 0; m < kPE; ++m) {
+      qraw[m] = q[qoff[m] + kofs];
+      qok[m] = okz && okxy[m];
     }
   };
-  for(int kk = kb - 2; kk < kb + 2; ++kk) load_plane(kk);
+  auto commit = [&](int kk) {
+    double* dst = &sq[slot(kk)][0][0];
+#pragma unroll
+    for(int m = 0; m < kPE; ++m) {
+      const int e = tid + m * kRX * kRY;
+      if(e < kPlaneElems) dst[e] = qok[m] ? qraw[m] : 0.0;
+    }
+  };
   const int di = c == 0, dj = c == 1, dk = c == 2;
   const double h = c == 0 ? P.g.dx : c == 1 ? P.g.dy : P.g.dz;
+  const double* u = V.u;
+  const double* v = V.v;
+  const double* w = V.w;
+  const Lay &Lu = V.Lu, &Lv = V.Lv, &Lw = V.Lw;
+  // point values of plane k: the two face-velocity pairs of the other components (a0..a3,
+  // b0..b3, in the order of advect_point), p at the two cells of the gradient, A_prev
+  double a0, a1, a2, a3, b0, b1, b2, b3, p0, p1, pa;
+  auto point_loads = [&](int k) {
+    if(c == 0) {
+      a0 = v[Lv.at(ic - 1, jc, k)];
+      a1 = v[Lv.at(ic, jc, k)];
+      a2 = v[Lv.at(ic - 1, jc + 1, k)];
+      a3 = v[Lv.at(ic, jc + 1, k)];
+      b0 = w[Lw.at(ic - 1, jc, k)];
+      b1 = w[Lw.at(ic, jc, k)];
+      b2 = w[Lw.at(ic - 1, jc, k + 1)];
+      b3 = w[Lw.at(ic, jc, k + 1)];
+    } else if(c == 1) {
+      a0 = u[Lu.at(ic, jc - 1, k)];
+      a1 = u[Lu.at(ic, jc, k)];
+      a2 = u[Lu.at(ic + 1, jc - 1, k)];
+      a3 = u[Lu.at(ic + 1, jc, k)];
+      b0 = w[Lw.at(ic, jc - 1, k)];
+      b1 = w[Lw.at(ic, jc, k)];
+      b2 = w[Lw.at(ic, jc - 1, k + 1)];
+      b3 = w[Lw.at(ic, jc, k + 1)];
+    } else {
+      a0 = u[Lu.at(ic, jc, k - 1)];
+      a1 = u[Lu.at(ic, jc, k)];
+      a2 = u[Lu.at(ic + 1, jc, k - 1)];
+      a3 = u[Lu.at(ic + 1, jc, k)];
+      b0 = v[Lv.at(ic, jc, k - 1)];
+      b1 = v[Lv.at(ic, jc, k)];
+      b2 = v[Lv.at(ic, jc + 1, k - 1)];
+      b3 = v[Lv.at(ic, jc + 1, k)];
+    }
+    p0 = p[Lp.at(ic, jc, k)];
+    p1 = p[Lp.at(ic - di, jc - dj, k - dk)];
+    pa = P.has_prev ? prev[L.at(ic, jc, k)] : 0.0;
+  };
+  for(int kk = kb - 2; kk < kb + 2; ++kk) {
+    fetch(kk);
+    commit(kk);
+  }
+  fetch(kb + 2);
   for(int k = kb; k < ke; ++k) {
-    load_plane(k + 2);
+    commit(k + 2);
+    point_loads(k);
+    if(k + 1 < ke) fetch(k + 3);
     __syncthreads();
     const int s0 = slot(k), sp = slot(k + 1), sm = slot(k - 1);
     // delta_z q over x/y halo 1, delta_y q over x halo 1 (sd_axis)
@@ -492,30 +569,26 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
       const double le = (Q(sp, 0, 0) - 2.0 * qc + Q(sm, 0, 0)) * P.iz;
       const double dif = (la + lb + le) * P.nu;
       // advection (advect_point): face velocities
-      const double* u = V.u;
-      const double* v = V.v;
-      const double* w = V.w;
-      const Lay &Lu = V.Lu, &Lv = V.Lv, &Lw = V.Lw;
       double wv, ev, sv, nv, bv, fv;
       if(c == 0) {
         wv = 0.5 * (Q(s0, 0, -1) + qc);
         ev = 0.5 * (qc + Q(s0, 0, 1));
-        sv = 0.5 * (v[Lv.at(i - 1, j, k)] + v[Lv.at(i, j, k)]);
-        nv = 0.5 * (v[Lv.at(i - 1, j + 1, k)] + v[Lv.at(i, j + 1, k)]);
-        bv = 0.5 * (w[Lw.at(i - 1, j, k)] + w[Lw.at(i, j, k)]);
-        fv = 0.5 * (w[Lw.at(i - 1, j, k + 1)] + w[Lw.at(i, j, k + 1)]);
+        sv = 0.5 * (a0 + a1);
+        nv = 0.5 * (a2 + a3);
+        bv = 0.5 * (b0 + b1);
+        fv = 0.5 * (b2 + b3);
       } else if(c == 1) {
-        wv = 0.5 * (u[Lu.at(i, j - 1, k)] + u[Lu.at(i, j, k)]);
-        ev = 0.5 * (u[Lu.at(i + 1, j - 1, k)] + u[Lu.at(i + 1, j, k)]);
+        wv = 0.5 * (a0 + a1);
+        ev = 0.5 * (a2 + a3);
         sv = 0.5 * (Q(s0, -1, 0) + qc);
         nv = 0.5 * (qc + Q(s0, 1, 0));
-        bv = 0.5 * (w[Lw.at(i, j - 1, k)] + w[Lw.at(i, j, k)]);
-        fv = 0.5 * (w[Lw.at(i, j - 1, k + 1)] + w[Lw.at(i, j, k + 1)]);
+        bv = 0.5 * (b0 + b1);
+        fv = 0.5 * (b2 + b3);
       } else {
-        wv = 0.5 * (u[Lu.at(i, j, k - 1)] + u[Lu.at(i, j, k)]);
-        ev = 0.5 * (u[Lu.at(i + 1, j, k - 1)] + u[Lu.at(i + 1, j, k)]);
-        sv = 0.5 * (v[Lv.at(i, j, k - 1)] + v[Lv.at(i, j, k)]);
-        nv = 0.5 * (v[Lv.at(i, j + 1, k - 1)] + v[Lv.at(i, j + 1, k)]);
+        wv = 0.5 * (a0 + a1);
+        ev = 0.5 * (a2 + a3);
+        sv = 0.5 * (b0 + b1);
+        nv = 0.5 * (b2 + b3);
         bv = 0.5 * (Q(sm, 0, 0) + qc);
         fv = 0.5 * (qc + Q(sp, 0, 0));
       }
@@ -528,13 +601,12 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
       const double bf = bv * reconstruct4(Q(s2m, 0, 0), Q(sm, 0, 0), qc, Q(sp, 0, 0), k, mz, bv, P.scheme);
       const double ff = fv * reconstruct4(Q(sm, 0, 0), qc, Q(sp, 0, 0), Q(s2p, 0, 0), k + 1, mz, fv, P.scheme);
       const double a = (ef - wf) / P.g.dx + (nf - sf) / P.g.dy + (ff - bf) / P.g.dz;
-      const double grad = (p[Lp.at(i, j, k)] - p[Lp.at(i - di, j - dj, k - dk)]) / h;
+      const double grad = (p0 - p1) / h;
       const size_t at = L.at(i, j, k);
       double r = qc;
       r += P.s_grad * grad;
       r += P.s_dif * dif;
       if(P.has_prev) {
-        const double pa = prev[at];
         r += P.s_adv1 * a;
         r += P.s_prev * pa;
       } else {

@@ -1373,8 +1373,11 @@ __global__ void __launch_bounds__(256) k_direction(const T* __restrict__ z, cons
 // The k march of k_apply_dot. FULL: the block owns KZ planes, so there is no per-plane bound
 // test (loads under such a test are waited for plane by plane). The pointers are restrict
 // parameters of this function, so the next planes' loads may be scheduled above the current
-// store.
-template <int KZ, bool UNI, bool FULL>
+// store. SHFL (the wave covers 64 active cells of a row): the x neighbours come from the
+// adjacent lanes, and only the first and last lane load the neighbour beyond the tile (the
+// others load their own cell, in the cache line they already read), so a plane costs four
+// loads instead of five.
+template <int KZ, bool UNI, bool FULL, bool SHFL>
 __device__ __forceinline__ void apply_march(const double* __restrict__ d, double* __restrict__ q, const PLevel& L,
                                             const HotTile& t, double (&v)[3]) {
   const ZWin<UNI> zw(L);
@@ -1383,12 +1386,14 @@ __device__ __forceinline__ void apply_march(const double* __restrict__ d, double
   const Nb x = nb_x(L, t.i), y = nb_y(L, t.j);
   const size_t oxm = (size_t)x.cm + (size_t)L.nx * t.j, oxp = (size_t)x.cp + (size_t)L.nx * t.j;
   const size_t oym = (size_t)t.i + (size_t)L.nx * y.cm, oyp = (size_t)t.i + (size_t)L.nx * y.cp;
+  const int tx = threadIdx.x;
+  const size_t oxe = tx == 0 ? oxm : tx == 63 ? oxp : col;  // SHFL: the edge lanes' outside neighbour
   double prev = d[col + plane * zw.kprev(t.k0)];
   double cur = d[col + plane * t.k0];
   // software pipeline: the lateral neighbours of plane k + 1 and the centre of plane k + 2 are
   // loaded before plane k is computed
-  double lw = d[oxm + plane * t.k0], le = d[oxp + plane * t.k0], ls = d[oym + plane * t.k0],
-         ln = d[oyp + plane * t.k0];
+  double lw = d[(SHFL ? oxe : oxm) + plane * t.k0], le = SHFL ? 0.0 : d[oxp + plane * t.k0],
+         ls = d[oym + plane * t.k0], ln = d[oyp + plane * t.k0];
   double next = d[col + plane * zw.knext(t.k0)];
 #pragma unroll
   for(int kk = 0; kk < KZ; ++kk) {
@@ -1397,15 +1402,24 @@ __device__ __forceinline__ void apply_march(const double* __restrict__ d, double
       double nlw = 0.0, nle = 0.0, nls = 0.0, nln = 0.0, nnext = 0.0;
       if(kk + 1 < KZ && (FULL || k + 1 < t.k1)) {
         const size_t pk1 = plane * (size_t)(k + 1);
-        nlw = d[oxm + pk1];
-        nle = d[oxp + pk1];
+        nlw = d[(SHFL ? oxe : oxm) + pk1];
+        if(!SHFL) nle = d[oxp + pk1];
         nls = d[oym + pk1];
         nln = d[oyp + pk1];
         nnext = d[col + plane * zw.knext(k + 1)];
       }
       const size_t pk = plane * k;
-      const double qv = lap_neg<double>(cur, sgn(x.sm, lw), sgn(x.sp, le), sgn(y.sm, ls), sgn(y.sp, ln),
-                                        zw.below(k, prev, cur), zw.above(k, next, cur), L.ihx2, L.ihy2, L.ihz2);
+      double wv, ev;
+      if(SHFL) {
+        const double up = __shfl_up(cur, 1, 64), dn = __shfl_down(cur, 1, 64);
+        wv = tx == 0 ? sgn(x.sm, lw) : up;
+        ev = tx == 63 ? sgn(x.sp, lw) : dn;
+      } else {
+        wv = sgn(x.sm, lw);
+        ev = sgn(x.sp, le);
+      }
+      const double qv = lap_neg<double>(cur, wv, ev, sgn(y.sm, ls), sgn(y.sp, ln), zw.below(k, prev, cur),
+                                        zw.above(k, next, cur), L.ihx2, L.ihy2, L.ihz2);
       q[col + pk] = qv;
       v[0] += cur;
       v[1] += qv;
@@ -1429,8 +1443,14 @@ __global__ void __launch_bounds__(256) k_apply_dot(const double* __restrict__ d,
   const HotTile t = hot_tile<KZ>(L);
   double v[3] = {0.0, 0.0, 0.0};
   if(t.in) {
-    if(t.k1 - t.k0 == KZ) apply_march<KZ, UNI, true>(d, q, L, t, v);
-    else apply_march<KZ, UNI, false>(d, q, L, t, v);
+    const bool rowfull = t.i - (int)threadIdx.x + 64 <= L.nx;  // uniform over the wave
+    if(t.k1 - t.k0 == KZ) {
+      if(rowfull) apply_march<KZ, UNI, true, true>(d, q, L, t, v);
+      else apply_march<KZ, UNI, true, false>(d, q, L, t, v);
+    } else {
+      if(rowfull) apply_march<KZ, UNI, false, true>(d, q, L, t, v);
+      else apply_march<KZ, UNI, false, false>(d, q, L, t, v);
+    }
   }
   const int ops[3] = {kSum, kSum, kSum};
   store_partials<3>(v, ops, partials);
