@@ -1456,6 +1456,140 @@ __global__ void __launch_bounds__(256) k_apply_dot(const double* __restrict__ d,
   store_partials<3>(v, ops, partials);
 }
 
+// Search direction, operator and dot products in one pass over the grid (k_direction followed
+// by k_apply_dot without the round trip of d through HBM): d = (z - mz) + beta (d_old - md)
+// (first: z - mz) is formed wherever the stencil needs it (the cell, the next plane and the
+// lateral neighbours, the latter from z and d_old read through the cache), stored once, and
+// q = A d with the sums of d, q and d.q follows; the deferred p += alpha_prev (d_old - md) of
+// the previous residual update is applied to the cell. Every d is the expression of
+// k_direction, so d and q are the values of the two separate kernels.
+struct DirCoef {
+  double mz, md, beta, alpha_prev;
+  int first, pend;
+  template <typename T>
+  __device__ double d(T zv, double dold) const {
+    const double zc = (double)zv - mz;
+    return first ? zc : zc + beta * (dold - md);
+  }
+};
+
+template <typename T, int KZ, bool UNI, bool FULL, bool SHFL>
+__device__ __forceinline__ void dir_apply_march(const T* __restrict__ z, const double* __restrict__ d_old,
+                                                double* __restrict__ d_new, double* __restrict__ p,
+                                                double* __restrict__ q, const PLevel& L, const HotTile& t,
+                                                const DirCoef& cf, double (&v)[3]) {
+  const ZWin<UNI> zw(L);
+  const size_t plane = (size_t)L.nx * L.ny;
+  const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+  const Nb x = nb_x(L, t.i), y = nb_y(L, t.j);
+  const size_t oxm = (size_t)x.cm + (size_t)L.nx * t.j, oxp = (size_t)x.cp + (size_t)L.nx * t.j;
+  const size_t oym = (size_t)t.i + (size_t)L.nx * y.cm, oyp = (size_t)t.i + (size_t)L.nx * y.cp;
+  const int tx = threadIdx.x;
+  const size_t oxe = tx == 0 ? oxm : tx == 63 ? oxp : col;  // SHFL: the edge lanes' outside neighbour
+  // lateral inputs of one plane: z and d_old at the x neighbour(s) and the two y neighbours
+  struct Lat {
+    T zw, ze, zs, zn;
+    double dw, de, ds, dn;
+  };
+  auto lat = [&](size_t pk, Lat& o) {
+    o.zw = z[(SHFL ? oxe : oxm) + pk];
+    o.dw = d_old[(SHFL ? oxe : oxm) + pk];
+    if(!SHFL) {
+      o.ze = z[oxp + pk];
+      o.de = d_old[oxp + pk];
+    }
+    o.zs = z[oym + pk];
+    o.ds = d_old[oym + pk];
+    o.zn = z[oyp + pk];
+    o.dn = d_old[oyp + pk];
+  };
+  const size_t kp0 = plane * (size_t)zw.kprev(t.k0);
+  double prev = cf.d(z[col + kp0], d_old[col + kp0]);
+  T zc = z[col + plane * t.k0];
+  double dc = d_old[col + plane * t.k0], pc = p[col + plane * t.k0];
+  Lat la;
+  lat(plane * t.k0, la);
+  const size_t kn0 = plane * (size_t)zw.knext(t.k0);
+  T zn = z[col + kn0];
+  double dn = d_old[col + kn0], pn = p[col + kn0];
+#pragma unroll
+  for(int kk = 0; kk < KZ; ++kk) {
+    const int k = t.k0 + kk;
+    if(FULL || k < t.k1) {
+      // software pipeline: the lateral inputs of plane k + 1 and the centre of plane k + 2
+      Lat nla{};
+      T nzn = T(0);
+      double ndn = 0.0, npn = 0.0;
+      if(kk + 1 < KZ && (FULL || k + 1 < t.k1)) {
+        lat(plane * (size_t)(k + 1), nla);
+        const size_t kn = plane * (size_t)zw.knext(k + 1);
+        nzn = z[col + kn];
+        ndn = d_old[col + kn];
+        npn = p[col + kn];
+      }
+      const double cur = cf.d(zc, dc);
+      const double next = cf.d(zn, dn);
+      double wv, ev;
+      if(SHFL) {
+        const double up = __shfl_up(cur, 1, 64), dnl = __shfl_down(cur, 1, 64);
+        const double edge = cf.d(la.zw, la.dw);
+        wv = tx == 0 ? sgn(x.sm, edge) : up;
+        ev = tx == 63 ? sgn(x.sp, edge) : dnl;
+      } else {
+        wv = sgn(x.sm, cf.d(la.zw, la.dw));
+        ev = sgn(x.sp, cf.d(la.ze, la.de));
+      }
+      const double sv = sgn(y.sm, cf.d(la.zs, la.ds)), nv = sgn(y.sp, cf.d(la.zn, la.dn));
+      const double qv = lap_neg<double>(cur, wv, ev, sv, nv, zw.below(k, prev, cur), zw.above(k, next, cur), L.ihx2,
+                                        L.ihy2, L.ihz2);
+      const size_t id = col + plane * k;
+      d_new[id] = cur;
+      q[id] = qv;
+      if(cf.pend) p[id] = pc + cf.alpha_prev * (dc - cf.md);
+      v[0] += cur;
+      v[1] += qv;
+      v[2] += cur * qv;
+      prev = cur;
+      zc = zn;
+      dc = dn;
+      pc = pn;
+      zn = nzn;
+      dn = ndn;
+      pn = npn;
+      la = nla;
+    }
+  }
+}
+
+template <typename T, int KZ, bool UNI>
+__global__ void __launch_bounds__(256) k_dir_apply(const T* __restrict__ z, const double* __restrict__ d_old,
+                                                   double* __restrict__ d_new, double* __restrict__ p,
+                                                   double* __restrict__ q, PLevel L, double* partials,
+                                                   const double* ps, int first) {
+  if(stopped(ps)) return;
+  DirCoef cf;
+  cf.mz = ps[P_LEVEL + L_MX5];
+  cf.md = ps[P_MD];
+  cf.beta = first ? 0.0 : ps[P_BETA];
+  cf.alpha_prev = ps[P_ALPHA];
+  cf.first = first;
+  cf.pend = !first && ps[P_PEND] != 0.0;
+  const HotTile t = hot_tile<KZ>(L);
+  double v[3] = {0.0, 0.0, 0.0};
+  if(t.in) {
+    const bool rowfull = t.i - (int)threadIdx.x + 64 <= L.nx;  // uniform over the wave
+    if(t.k1 - t.k0 == KZ) {
+      if(rowfull) dir_apply_march<T, KZ, UNI, true, true>(z, d_old, d_new, p, q, L, t, cf, v);
+      else dir_apply_march<T, KZ, UNI, true, false>(z, d_old, d_new, p, q, L, t, cf, v);
+    } else {
+      if(rowfull) dir_apply_march<T, KZ, UNI, false, true>(z, d_old, d_new, p, q, L, t, cf, v);
+      else dir_apply_march<T, KZ, UNI, false, false>(z, d_old, d_new, p, q, L, t, cf, v);
+    }
+  }
+  const int ops[3] = {kSum, kSum, kSum};
+  store_partials<3>(v, ops, partials);
+}
+
 // r = (r - mr) - alpha (q - mq); sums of r, r^2, r w/D; the matching p update is deferred to
 // the next direction kernel (or the final update).
 template <typename T, int KZ, bool UNI>
@@ -1680,6 +1814,13 @@ namespace {
 // fused ones only in rounding); read when a solver is built.
 bool fused_disabled() {
   const char* e = std::getenv("SOLVER_AMD_MG_UNFUSED");
+  return e != nullptr && e[0] == '1';
+}
+
+// SOLVER_AMD_CG_UNFUSED=1 runs the search direction and the operator as two kernels
+// (k_direction, k_apply_dot) instead of k_dir_apply; read when a solve starts.
+bool cg_fused_disabled() {
+  const char* e = std::getenv("SOLVER_AMD_CG_UNFUSED");
   return e != nullptr && e[0] == '1';
 }
 
@@ -2015,6 +2156,7 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   const bool mixed = precision == OF_PRECISION_MIXED;
   const double n0 = (double)F.n;
   const HotLaunch hf = hot_launch(F);
+  const bool cg_fused = !cg_fused_disabled();
   const unsigned pgrid = std::min<unsigned>(div_up((size_t)F.n, 256 * kPointIlp), 8192u);
   OFH_CHECK(hipMemsetAsync(ps, 0, P_COUNT * sizeof(double), c.stream));
   hipLaunchKernelGGL(k_setup1, ll.grid, ll.block, 0, c.stream, rhs, p, Lc, F, r, pc, c.partials, ll.kz);
@@ -2026,12 +2168,16 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   vcycle<T>();
   int it = 0;
   for(; it < maxit; ++it) {
-    {
-      ProfScope prof_(c, "k_direction");
-      hipLaunchKernelGGL(k_direction<T>, dim3(pgrid), dim3(256), 0, c.stream, static_cast<const T*>(x_final(0)),
-                         d[(it + 1) % 2], d[it % 2], pc, F.n, ps, it == 0 ? 1 : 0);
-    }
-    {
+    if(cg_fused) {
+      ProfScope prof_(c, "k_dir_apply");
+      OFH_HOT_LAUNCH(k_dir_apply<T, hf, c.stream, static_cast<const T*>(x_final(0)), d[(it + 1) % 2], d[it % 2], pc,
+                     q, F, c.partials, ps, it == 0 ? 1 : 0);
+    } else {
+      {
+        ProfScope prof_(c, "k_direction");
+        hipLaunchKernelGGL(k_direction<T>, dim3(pgrid), dim3(256), 0, c.stream, static_cast<const T*>(x_final(0)),
+                           d[(it + 1) % 2], d[it % 2], pc, F.n, ps, it == 0 ? 1 : 0);
+      }
       ProfScope prof_(c, "k_apply_dot");
       if(hf.kz == 8) {
         if(hf.uni)

@@ -77,3 +77,43 @@ def test_fused_stepper_matches_unfused(ctx, precision):
         assert rel_l2_active(x, y) <= 1e-11
     assert rel_l2_active(pa, pb, mean_removed=True) <= 1e-10
     assert ma.max_divergence_linf <= 1e-10
+
+
+def solve_cg(ctx, g, pb, rhs, precision, tol, unfused):
+    """Solve with the search direction and operator fused (default) or as two kernels."""
+    os.environ["SOLVER_AMD_CG_UNFUSED"] = "1" if unfused else "0"
+    try:
+        o = sa.ProjectionOptions(1.0, 1.0, 600, tol)
+        sol = zeros(g, sa.CELL)
+        d = sa.PoissonDiagnostics()
+        ctx.run("of_solve_pressure_poisson", C.byref(g), C.byref(pb), C.byref(o), precision, sa.ptr(dev(rhs)),
+                sa.ptr(sol), C.byref(d), None, 0)
+    finally:
+        os.environ.pop("SOLVER_AMD_CG_UNFUSED", None)
+    return host(sol), d
+
+
+# periodic (lane-shuffle rows), ragged rows (nx not a multiple of 64), Neumann / Dirichlet faces
+# (boundary images in x, y and z) and a single-plane grid
+@pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
+@pytest.mark.parametrize("pbc,shape", [
+    ([sa.P_PERIODIC] * 6, (64, 64, 64)),
+    ([sa.P_PERIODIC] * 6, (96, 40, 24)),
+    ([1, 1, 1, 1, 0, 0], (48, 40, 1)),
+    ([2, 2, 1, 1, 0, 0], (32, 16, 24)),
+    ([0, 1, 2, 2, 1, 0], (130, 20, 18)),
+])
+def test_fused_cg_matches_two_kernel_path(ctx, precision, pbc, shape):
+    nx, ny, nz = shape
+    h = (1.0 / nx, 1.3 / ny, 0.9 / nz if nz > 1 else 1.0)
+    g = sa.grid(nx, ny, nz, *h)
+    rng = np.random.default_rng(nx * ny + nz)
+    rhs = rng.standard_normal(sa.storage_shape(nx, ny, nz, sa.CELL))
+    pb = sa.pressure_bc_set(pbc, [0.2, -0.1, 0.0, 0.4, 0.1, 0.0], [0.3, 0.0, -0.2, 0.1, 0.0, 0.2])
+    tol = 1e-12 if precision == sa.FP64 else 1e-10
+    a, da = solve_cg(ctx, g, pb, rhs, precision, tol, unfused=False)
+    b, db = solve_cg(ctx, g, pb, rhs, precision, tol, unfused=True)
+    assert da.converged == db.converged == 1
+    # d is formed by the same expression in both paths, so the iterations agree
+    assert da.iterations == db.iterations, (da.iterations, db.iterations)
+    assert rel_l2_active(a, b, mean_removed=all(t == sa.P_PERIODIC for t in pbc)) <= 1e-11
