@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
-    ap.add_argument("--profile-kernel", default="k_apply_dot")
+    ap.add_argument("--profile-kernel", default="k_dir_apply")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128, help="grid of the bounded CPU baseline sample")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -170,6 +170,7 @@ def run_adjoint(sa, ctx, name, fd_dirs, tl_dirs=0):
 def kernel_bytes(name, n_cells, mixed):
     f64, t = 8, (4 if mixed else 8)
     table = {
+        "k_dir_apply": t + 5 * f64,                     # z, d_old, p read; d, p, q written
         "k_apply_dot": 2 * f64,                         # d read; q written
         "k_direction": t + 2 * f64 + 2 * f64,           # z, d_old, p read; d_new, p written
         "k_update": 3 * f64 + (4 if mixed else 0),      # r, q read; r (+ FP32 copy) written
@@ -190,10 +191,11 @@ def step_bytes_model(n, iters, mixed):
     """Algorithmic HBM bytes of one step (passes over an n^3 FP64 array):
     predictor RHS 3 x 7, ADI staging copies 3 x 2 + sweeps 9 x 2, pressure RHS 4, Poisson
     setup/finish 12 (+1.5 FP32 cast), correction 9, pressure update 4, diagnostics 6; per MGPCG
-    iteration: CG direction/apply/update 10 and the V-cycle 10.25 x 8/7 (hierarchy) in the
-    multigrid precision (x 1/2 for FP32)."""
+    iteration: fused direction/operator 6 (z, d_old, p read; d, p, q written; z in FP32 and an
+    FP32 copy of r in mixed: 5.5 + 3.5) and residual update 3, and the V-cycle 10.25 x 8/7
+    (hierarchy) in the multigrid precision (x 1/2 for FP32)."""
     fixed = 21 + 6 + 18 + 4 + 12 + (1.5 if mixed else 0.0) + 9 + 4 + 6
-    per_it = 10.0 + 10.25 * 8.0 / 7.0 * (0.5 if mixed else 1.0)
+    per_it = 9.0 + 10.25 * 8.0 / 7.0 * (0.5 if mixed else 1.0)
     return (fixed + iters * per_it) * n ** 3 * 8
 
 

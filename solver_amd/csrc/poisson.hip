@@ -87,6 +87,19 @@ struct PLevel {
 // boundary class of a coordinate: 0 interior, 1 first, 2 last, 3 single
 __device__ __host__ inline int bclass(int co, int n) { return (co == 0 ? 1 : 0) | (co == n - 1 ? 2 : 0); }
 
+// Coarse coordinate of a fine one: the refinement ratio of every axis is 1 or 2, so the
+// division is a shift (coordinates are non-negative).
+__device__ inline int coarse_of(int co, int ratio) { return co >> (ratio >> 1); }
+
+// n / d for 0 <= n < 2^22 and d >= 1: a float reciprocal estimate is within 0.5 of n / d,
+// so one correction step gives the exact quotient (an integer division is ~30 instructions).
+__device__ inline int small_div(int n, int d) {
+  int q = (int)((float)n * (1.0f / (float)d));
+  const int r = n - q * d;
+  q += r >= d ? 1 : (r < 0 ? -1 : 0);
+  return q;
+}
+
 // Neighbour coordinate and sign along one axis (homogeneous BCs): periodic wrap inside the
 // active range, Neumann -> the cell itself, Dirichlet -> minus the cell itself
 // (linsolve/poisson_solver.cpp:195-242 with value = gradient = 0).
@@ -181,73 +194,81 @@ __device__ inline double boundary_source(const PLevel& L, int i, int j, int k) {
 // Shared by the tiled level kernels and the one-workgroup tail. Neighbour data (Nb) per axis;
 // the deferred means are subtracted on read.
 
+// UNI: every axis periodic, so the Jacobi weight is the same for every cell (winv[0]) and
+// every neighbour sign is +1 (the values are those of the general path). IX: index type (int
+// where the level has fewer than 2^31 cells).
+
 // Fused first two pre-smoothing sweeps at (i,j,k): x1 = (b - mb) w/D - m1 (pointwise, from
 // x0 = 0), x2 = x1 + w/D (b - mb - A x1).
-template <typename T>
+template <typename T, bool UNI = false, typename IX = size_t>
 __device__ inline T smooth12_cell(const PLevel& L, const T* __restrict__ b, T mb, T m1, int i, int j, int k,
                                   const Nb& x, const Nb& y, const Nb& z) {
-  const size_t plane = (size_t)L.nx * L.ny;
-  auto W = [&](int cx, int cy, int cz) { return T(L.winv[cx + 4 * (cy + 4 * cz)]); };
-  auto B = [&](int ii, int jj, int kk) { return b[(size_t)ii + (size_t)L.nx * jj + plane * kk] - mb; };
+  const IX plane = (IX)L.nx * L.ny;
+  auto W = [&](int cx, int cy, int cz) { return UNI ? T(L.winv[0]) : T(L.winv[cx + 4 * (cy + 4 * cz)]); };
+  auto S = [&](int sg, T v) { return UNI ? v : sgn(sg, v); };
+  auto B = [&](int ii, int jj, int kk) { return b[(IX)ii + (IX)L.nx * jj + plane * kk] - mb; };
   const int cx = bclass(i, L.nx), cy = bclass(j, L.ny), cz = bclass(k, L.nz);
   const T wc = W(cx, cy, cz);
   const T bt = B(i, j, k);
   const T c = bt * wc - m1;
-  const T xw = sgn(x.sm, B(x.cm, j, k) * W(bclass(x.cm, L.nx), cy, cz) - m1);
-  const T xe = sgn(x.sp, B(x.cp, j, k) * W(bclass(x.cp, L.nx), cy, cz) - m1);
-  const T xs = sgn(y.sm, B(i, y.cm, k) * W(cx, bclass(y.cm, L.ny), cz) - m1);
-  const T xn = sgn(y.sp, B(i, y.cp, k) * W(cx, bclass(y.cp, L.ny), cz) - m1);
-  const T xb = sgn(z.sm, B(i, j, z.cm) * W(cx, cy, bclass(z.cm, L.nz)) - m1);
-  const T xf = sgn(z.sp, B(i, j, z.cp) * W(cx, cy, bclass(z.cp, L.nz)) - m1);
+  const T xw = S(x.sm, B(x.cm, j, k) * W(bclass(x.cm, L.nx), cy, cz) - m1);
+  const T xe = S(x.sp, B(x.cp, j, k) * W(bclass(x.cp, L.nx), cy, cz) - m1);
+  const T xs = S(y.sm, B(i, y.cm, k) * W(cx, bclass(y.cm, L.ny), cz) - m1);
+  const T xn = S(y.sp, B(i, y.cp, k) * W(cx, bclass(y.cp, L.ny), cz) - m1);
+  const T xb = S(z.sm, B(i, j, z.cm) * W(cx, cy, bclass(z.cm, L.nz)) - m1);
+  const T xf = S(z.sp, B(i, j, z.cp) * W(cx, cy, bclass(z.cp, L.nz)) - m1);
   const T ax = lap_neg<T>(c, xw, xe, xs, xn, xb, xf, T(L.ihx2), T(L.ihy2), T(L.ihz2));
   return c + wc * (bt + T(-1) * ax);
 }
 
 // Fine residual (b - mb) - A (x2 - mx) at (i,j,k).
-template <typename T>
+template <typename T, bool UNI = false, typename IX = size_t>
 __device__ inline T residual_cell(const PLevel& L, const T* __restrict__ b, const T* __restrict__ x2, T mb, T mx,
                                   int i, int j, int k, const Nb& x, const Nb& y, const Nb& z) {
-  const size_t plane = (size_t)L.nx * L.ny;
-  auto X = [&](int ii, int jj, int kk) { return x2[(size_t)ii + (size_t)L.nx * jj + plane * kk] - mx; };
+  const IX plane = (IX)L.nx * L.ny;
+  auto S = [&](int sg, T v) { return UNI ? v : sgn(sg, v); };
+  auto X = [&](int ii, int jj, int kk) { return x2[(IX)ii + (IX)L.nx * jj + plane * kk] - mx; };
   const T c = X(i, j, k);
-  const T ax = lap_neg<T>(c, sgn(x.sm, X(x.cm, j, k)), sgn(x.sp, X(x.cp, j, k)), sgn(y.sm, X(i, y.cm, k)),
-                          sgn(y.sp, X(i, y.cp, k)), sgn(z.sm, X(i, j, z.cm)), sgn(z.sp, X(i, j, z.cp)), T(L.ihx2),
+  const T ax = lap_neg<T>(c, S(x.sm, X(x.cm, j, k)), S(x.sp, X(x.cp, j, k)), S(y.sm, X(i, y.cm, k)),
+                          S(y.sp, X(i, y.cp, k)), S(z.sm, X(i, j, z.cm)), S(z.sp, X(i, j, z.cp)), T(L.ihx2),
                           T(L.ihy2), T(L.ihz2));
-  return (b[(size_t)i + (size_t)L.nx * j + plane * k] - mb) + T(-1) * ax;
+  return (b[(IX)i + (IX)L.nx * j + plane * k] - mb) + T(-1) * ax;
 }
 
 // Prolongation-add fused with the first post-smoothing sweep:
 // x3 = (x2 - m2) + (xc - mc)[parent]; returns x4 = x3 + w/D (b - mb - A x3).
-template <typename T>
+template <typename T, bool UNI = false, typename IX = size_t>
 __device__ inline T prolong_smooth_cell(const PLevel& L, const PLevel& Cl, const T* __restrict__ b,
                                         const T* __restrict__ x2, const T* __restrict__ xc, T mb, T m2, T mc, int i,
                                         int j, int k, const Nb& x, const Nb& y, const Nb& z) {
-  const size_t plane = (size_t)L.nx * L.ny;
-  const size_t cplane = (size_t)Cl.nx * Cl.ny;
+  const IX plane = (IX)L.nx * L.ny;
+  const IX cplane = (IX)Cl.nx * Cl.ny;
+  auto S = [&](int sg, T v) { return UNI ? v : sgn(sg, v); };
   auto X3 = [&](int ii, int jj, int kk) {
-    return (x2[(size_t)ii + (size_t)L.nx * jj + plane * kk] - m2) +
-           (xc[(size_t)(ii / L.rx) + (size_t)Cl.nx * (jj / L.ry) + cplane * (kk / L.rz)] - mc);
+    return (x2[(IX)ii + (IX)L.nx * jj + plane * kk] - m2) +
+           (xc[(IX)coarse_of(ii, L.rx) + (IX)Cl.nx * coarse_of(jj, L.ry) + cplane * coarse_of(kk, L.rz)] - mc);
   };
-  const T wc = T(L.winv[bclass(i, L.nx) + 4 * (bclass(j, L.ny) + 4 * bclass(k, L.nz))]);
+  const T wc = UNI ? T(L.winv[0]) : T(L.winv[bclass(i, L.nx) + 4 * (bclass(j, L.ny) + 4 * bclass(k, L.nz))]);
   const T c = X3(i, j, k);
-  const T ax = lap_neg<T>(c, sgn(x.sm, X3(x.cm, j, k)), sgn(x.sp, X3(x.cp, j, k)), sgn(y.sm, X3(i, y.cm, k)),
-                          sgn(y.sp, X3(i, y.cp, k)), sgn(z.sm, X3(i, j, z.cm)), sgn(z.sp, X3(i, j, z.cp)), T(L.ihx2),
+  const T ax = lap_neg<T>(c, S(x.sm, X3(x.cm, j, k)), S(x.sp, X3(x.cp, j, k)), S(y.sm, X3(i, y.cm, k)),
+                          S(y.sp, X3(i, y.cp, k)), S(z.sm, X3(i, j, z.cm)), S(z.sp, X3(i, j, z.cp)), T(L.ihx2),
                           T(L.ihy2), T(L.ihz2));
-  return c + wc * ((b[(size_t)i + (size_t)L.nx * j + plane * k] - mb) + T(-1) * ax);
+  return c + wc * ((b[(IX)i + (IX)L.nx * j + plane * k] - mb) + T(-1) * ax);
 }
 
 // Second post-smoothing sweep: x4' = x4 - m4, returns x5 = x4' + w/D (bt - A x4'), bt = b - mb.
-template <typename T>
+template <typename T, bool UNI = false, typename IX = size_t>
 __device__ inline T smooth_last_cell(const PLevel& L, const T* __restrict__ b, const T* __restrict__ x4, T mb, T m4,
                                      int i, int j, int k, const Nb& x, const Nb& y, const Nb& z, T& bt) {
-  const size_t plane = (size_t)L.nx * L.ny;
-  auto X = [&](int ii, int jj, int kk) { return x4[(size_t)ii + (size_t)L.nx * jj + plane * kk] - m4; };
-  const T wc = T(L.winv[bclass(i, L.nx) + 4 * (bclass(j, L.ny) + 4 * bclass(k, L.nz))]);
+  const IX plane = (IX)L.nx * L.ny;
+  auto S = [&](int sg, T v) { return UNI ? v : sgn(sg, v); };
+  auto X = [&](int ii, int jj, int kk) { return x4[(IX)ii + (IX)L.nx * jj + plane * kk] - m4; };
+  const T wc = UNI ? T(L.winv[0]) : T(L.winv[bclass(i, L.nx) + 4 * (bclass(j, L.ny) + 4 * bclass(k, L.nz))]);
   const T c = X(i, j, k);
-  const T ax = lap_neg<T>(c, sgn(x.sm, X(x.cm, j, k)), sgn(x.sp, X(x.cp, j, k)), sgn(y.sm, X(i, y.cm, k)),
-                          sgn(y.sp, X(i, y.cp, k)), sgn(z.sm, X(i, j, z.cm)), sgn(z.sp, X(i, j, z.cp)), T(L.ihx2),
+  const T ax = lap_neg<T>(c, S(x.sm, X(x.cm, j, k)), S(x.sp, X(x.cp, j, k)), S(y.sm, X(i, y.cm, k)),
+                          S(y.sp, X(i, y.cp, k)), S(z.sm, X(i, j, z.cm)), S(z.sp, X(i, j, z.cp)), T(L.ihx2),
                           T(L.ihy2), T(L.ihz2));
-  bt = b[(size_t)i + (size_t)L.nx * j + plane * k] - mb;
+  bt = b[(IX)i + (IX)L.nx * j + plane * k] - mb;
   return c + wc * (bt + T(-1) * ax);
 }
 
@@ -613,7 +634,7 @@ __global__ void __launch_bounds__(256) k_residual_restrict(const T* __restrict__
       acc = (L.rz == 2 && second) ? acc + rr : rr;
       const bool complete = L.rz == 1 || second;
       if(complete && t.in && (L.rx == 1 || (t.i & 1) == 0) && (L.ry == 1 || (t.j & 1) == 0)) {
-        const int ic = t.i / L.rx, jc = t.j / L.ry, kc = k / L.rz;
+        const int ic = coarse_of(t.i, L.rx), jc = coarse_of(t.j, L.ry), kc = coarse_of(k, L.rz);
         const T val = inv_children * acc;
         bc[(size_t)ic + (size_t)Cl.nx * jc + cplane * kc] = val;
         v[0] += (double)val;
@@ -648,24 +669,24 @@ __global__ void __launch_bounds__(256) k_prolong_smooth(const T* __restrict__ b,
     const Nb x = nb_x(L, t.i), y = nb_y(L, t.j);
     const int cx = bclass(t.i, L.nx), cy = bclass(t.j, L.ny);
     // parent columns of the cell and of its x / y neighbours
-    const size_t pc = (size_t)(t.i / L.rx) + (size_t)Cl.nx * (t.j / L.ry);
-    const size_t pxm = (size_t)(x.cm / L.rx) + (size_t)Cl.nx * (t.j / L.ry);
-    const size_t pxp = (size_t)(x.cp / L.rx) + (size_t)Cl.nx * (t.j / L.ry);
-    const size_t pym = (size_t)(t.i / L.rx) + (size_t)Cl.nx * (y.cm / L.ry);
-    const size_t pyp = (size_t)(t.i / L.rx) + (size_t)Cl.nx * (y.cp / L.ry);
+    const size_t pc = (size_t)coarse_of(t.i, L.rx) + (size_t)Cl.nx * coarse_of(t.j, L.ry);
+    const size_t pxm = (size_t)coarse_of(x.cm, L.rx) + (size_t)Cl.nx * coarse_of(t.j, L.ry);
+    const size_t pxp = (size_t)coarse_of(x.cp, L.rx) + (size_t)Cl.nx * coarse_of(t.j, L.ry);
+    const size_t pym = (size_t)coarse_of(t.i, L.rx) + (size_t)Cl.nx * coarse_of(y.cm, L.ry);
+    const size_t pyp = (size_t)coarse_of(t.i, L.rx) + (size_t)Cl.nx * coarse_of(y.cp, L.ry);
     const size_t oxm = (size_t)x.cm + (size_t)L.nx * t.j, oxp = (size_t)x.cp + (size_t)L.nx * t.j;
     const size_t oym = (size_t)t.i + (size_t)L.nx * y.cm, oyp = (size_t)t.i + (size_t)L.nx * y.cp;
     auto X3 = [&](size_t fine, size_t coarse) { return (x2[fine] - m2) + (xc[coarse] - mc); };
     const int kp0 = zw.kprev(t.k0);
-    T prev = X3(col + plane * kp0, pc + cplane * (kp0 / L.rz));
-    T cur = X3(col + plane * t.k0, pc + cplane * (t.k0 / L.rz));
+    T prev = X3(col + plane * kp0, pc + cplane * coarse_of(kp0, L.rz));
+    T cur = X3(col + plane * t.k0, pc + cplane * coarse_of(t.k0, L.rz));
 #pragma unroll
     for(int kk = 0; kk < KZ; ++kk) {
       const int k = t.k0 + kk;
       if(k < t.k1) {
         const int kn = zw.knext(k);
-        const T next = X3(col + plane * kn, pc + cplane * (kn / L.rz));
-        const size_t pk = plane * k, cpk = cplane * (k / L.rz);
+        const T next = X3(col + plane * kn, pc + cplane * coarse_of(kn, L.rz));
+        const size_t pk = plane * k, cpk = cplane * coarse_of(k, L.rz);
         const T ax = lap_neg<T>(cur, sgn(x.sm, X3(oxm + pk, pxm + cpk)), sgn(x.sp, X3(oxp + pk, pxp + cpk)),
                                 sgn(y.sm, X3(oym + pk, pym + cpk)), sgn(y.sp, X3(oyp + pk, pyp + cpk)),
                                 zw.below(k, prev, cur), zw.above(k, next, cur), ihx2, ihy2, ihz2);
@@ -1112,17 +1133,38 @@ struct TailBufs {
 };
 constexpr int kTailLdsBytes = 152 * 1024;
 
-template <typename T>
-__device__ inline double tail_sum(double v) {
-  double a[1] = {v};
-  const int ops[1] = {kSum};
-  block_reduce<1>(a, ops);
-  return a[0];  // thread 0
-}
+// Thread groups of the tail. Levels of more than kWaveCells cells run on the whole workgroup
+// (workgroup barriers, block-wide sums); the smaller ones and the coarsest solve run on its
+// first wave alone, which needs no workgroup barrier: a wave-level fence orders its LDS and
+// global accesses, and sums are wave reductions. Results of sums are valid in thread 0.
+constexpr int kWaveCells = 512;
+struct TailBlock {
+  int tid, nt;
+  __device__ void sync() const { __syncthreads(); }
+  template <int N>
+  __device__ void sums(double (&v)[N]) const {
+    int ops[N];
+#pragma unroll
+    for(int k = 0; k < N; ++k) ops[k] = kSum;
+    block_reduce<N>(v, ops);
+  }
+};
+struct TailWave {
+  int tid, nt;
+  __device__ void sync() const {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  }
+  template <int N>
+  __device__ void sums(double (&v)[N]) const {
+#pragma unroll
+    for(int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+  }
+};
 
 // LDS: every tail level lives in one LDS buffer (b of level l0 is copied in, x of level l0
 // copied out); otherwise the levels are read and written in global memory.
-template <typename T, bool LDS>
+template <typename T, bool LDS, bool UNI>
 __global__ void __launch_bounds__(kTailThreads)
     k_vtail(const PLevel* __restrict__ lvs, int l0, int nl, TailBufs<T> B, const int* __restrict__ perm,
             const double* __restrict__ factor, const double* __restrict__ U, double* __restrict__ gwork, double* ps,
@@ -1144,15 +1186,15 @@ __global__ void __launch_bounds__(kTailThreads)
   __shared__ double lm[kMaxLevels][4];  // per level: mb, m2, m4, m5
   __shared__ double lsbw[kMaxLevels];   // sum of raw b w/D
   __shared__ double rz_sh;
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = threadIdx.x;
   const int s0 = P_LEVEL + 8 * l0;
   if(tid == 0) {
     lm[l0][0] = ps[s0 + L_MB];
     lsbw[l0] = ps[s0 + L_SBW];
   }
   __syncthreads();
-  // ---- down
-  for(int l = l0; l < nl - 1; ++l) {
+  // down leg of level l: two sweeps (x2 and its mean), residual restricted to level l + 1
+  auto down = [&](int l, const auto& G) {
     const PLevel& L = lvs[l];
     const PLevel& Cl = lvs[l + 1];
     const T* b = Bp(l);
@@ -1160,26 +1202,26 @@ __global__ void __launch_bounds__(kTailThreads)
     const T mb = T(lm[l][0]);
     const T m1 = zm ? T((lsbw[l] - lm[l][0] * L.winv_sum) / (double)L.n) : T(0);
     const int pxy = L.nx * L.ny;
-    double acc = 0.0;
-    for(int id = tid; id < L.n; id += nt) {
-      const int k = id / pxy, rem = id - k * pxy, j = rem / L.nx, i = rem - j * L.nx;
-      const T xv = smooth12_cell<T>(L, b, mb, m1, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
+    double acc[1] = {0.0};
+    for(int id = G.tid; id < L.n; id += G.nt) {
+      const int k = small_div(id, pxy), rem = id - k * pxy, j = small_div(rem, L.nx), i = rem - j * L.nx;
+      const T xv = smooth12_cell<T, UNI, int>(L, b, mb, m1, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
       x2[id] = xv;
-      acc += (double)xv;
+      acc[0] += (double)xv;
     }
-    acc = tail_sum<T>(acc);
-    if(tid == 0) lm[l][1] = zm ? acc / (double)L.n : 0.0;
-    __syncthreads();
+    G.sums(acc);
+    if(G.tid == 0) lm[l][1] = zm ? acc[0] / (double)L.n : 0.0;
+    G.sync();
     const T mx = T(lm[l][1]);
     const T inv_children = T(1.0 / (double)(L.rx * L.ry * L.rz));
     T* bc = Bp(l + 1);
     const int cxy = Cl.nx * Cl.ny;
-    double s0c = 0.0, s1c = 0.0;
-    for(int id = tid; id < Cl.n; id += nt) {
-      const int kc = id / cxy, rem = id - kc * cxy, jc = rem / Cl.nx, ic = rem - jc * Cl.nx;
+    double sc[2] = {0.0, 0.0};
+    for(int id = G.tid; id < Cl.n; id += G.nt) {
+      const int kc = small_div(id, cxy), rem = id - kc * cxy, jc = small_div(rem, Cl.nx), ic = rem - jc * Cl.nx;
       const int i0 = ic * L.rx, j0 = jc * L.ry, k0 = kc * L.rz;
       auto R = [&](int i, int j, int k) {
-        return residual_cell<T>(L, b, x2, mb, mx, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
+        return residual_cell<T, UNI, int>(L, b, x2, mb, mx, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
       };
       // same combination order as the tiled kernel: x pairs, then y pairs, then z pairs
       auto row = [&](int j, int k) {
@@ -1196,30 +1238,66 @@ __global__ void __launch_bounds__(kTailThreads)
       if(L.rz == 2) a = a + pl(k0 + 1);
       const T val = inv_children * a;
       bc[id] = val;
-      s0c += (double)val;
-      s1c += (double)val * Cl.winv[bclass(ic, Cl.nx) + 4 * (bclass(jc, Cl.ny) + 4 * bclass(kc, Cl.nz))];
+      sc[0] += (double)val;
+      sc[1] += (double)val * Cl.winv[bclass(ic, Cl.nx) + 4 * (bclass(jc, Cl.ny) + 4 * bclass(kc, Cl.nz))];
     }
-    s0c = tail_sum<T>(s0c);
-    s1c = tail_sum<T>(s1c);
-    if(tid == 0) {
-      lm[l + 1][0] = zm ? s0c / (double)Cl.n : 0.0;
-      lsbw[l + 1] = s1c;
+    G.sums(sc);
+    if(G.tid == 0) {
+      lm[l + 1][0] = zm ? sc[0] / (double)Cl.n : 0.0;
+      lsbw[l + 1] = sc[1];
     }
-    __syncthreads();
-  }
-  // ---- coarsest: elimination with the recorded row swaps and multipliers
-  {
+    G.sync();
+  };
+  // up leg of level l: prolongation + sweep (x4 and its mean), last sweep (x5, its mean, r.z)
+  auto up = [&](int l, const auto& G) {
+    const PLevel& L = lvs[l];
+    const PLevel& Cl = lvs[l + 1];
+    const T* b = Bp(l);
+    const T mb = T(lm[l][0]), m2 = T(lm[l][1]), mc = T(lm[l + 1][3]);
+    const int pxy = L.nx * L.ny;
+    double acc[1] = {0.0};
+    for(int id = G.tid; id < L.n; id += G.nt) {
+      const int k = small_div(id, pxy), rem = id - k * pxy, j = small_div(rem, L.nx), i = rem - j * L.nx;
+      const T xv =
+          prolong_smooth_cell<T, UNI, int>(L, Cl, b, XA(l), XA(l + 1), mb, m2, mc, i, j, k, nb_x(L, i), nb_y(L, j),
+                                           nb_z(L, k));
+      XB(l)[id] = xv;
+      acc[0] += (double)xv;
+    }
+    G.sums(acc);
+    if(G.tid == 0) lm[l][2] = zm ? acc[0] / (double)L.n : 0.0;
+    G.sync();
+    const T m4 = T(lm[l][2]);
+    double s5[2] = {0.0, 0.0};  // sum of x5, r.z
+    for(int id = G.tid; id < L.n; id += G.nt) {
+      const int k = small_div(id, pxy), rem = id - k * pxy, j = small_div(rem, L.nx), i = rem - j * L.nx;
+      T bt;
+      const T xv =
+          smooth_last_cell<T, UNI, int>(L, b, XB(l), mb, m4, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k), bt);
+      XA(l)[id] = xv;
+      s5[0] += (double)xv;
+      s5[1] += (double)bt * (double)xv;
+    }
+    G.sums(s5);
+    if(G.tid == 0) {
+      lm[l][3] = zm ? s5[0] / (double)L.n : 0.0;
+      if(l == 0) rz_sh = s5[1];
+    }
+    G.sync();
+  };
+  // coarsest level: elimination with the recorded row swaps and multipliers
+  auto coarsest = [&](const auto& G) {
     const int lc = nl - 1;
     const int n = lvs[lc].n;
     const T* b = Bp(lc);
     T* x = XA(lc);
     const T mb = T(lm[lc][0]);
-    for(int i = tid; i < n; i += nt) work[i] = (double)(b[i] - mb);
-    __syncthreads();
-    if(zm && tid == 0) work[n - 1] = 0.0;
-    __syncthreads();
+    for(int i = G.tid; i < n; i += G.nt) work[i] = (double)(b[i] - mb);
+    G.sync();
+    if(zm && G.tid == 0) work[n - 1] = 0.0;
+    G.sync();
     for(int p = 0; p < n; ++p) {
-      if(tid == 0) {
+      if(G.tid == 0) {
         const int best = perm[p];
         if(best != p) {
           const double t = work[p];
@@ -1227,15 +1305,15 @@ __global__ void __launch_bounds__(kTailThreads)
           work[best] = t;
         }
       }
-      __syncthreads();
+      G.sync();
       const double wp = work[p];
-      for(int row = p + 1 + tid; row < n; row += nt) {
+      for(int row = p + 1 + G.tid; row < n; row += G.nt) {
         const double f = factor[(size_t)row * n + p];
         if(f != 0.0) work[row] -= f * wp;
       }
-      __syncthreads();
+      G.sync();
     }
-    if(tid == 0) {
+    if(G.tid == 0) {
       for(int row = n - 1; row >= 0; --row) {
         double s = work[row];
         for(int c = row + 1; c < n; ++c) s -= U[(size_t)row * n + c] * work[c];
@@ -1249,55 +1327,40 @@ __global__ void __launch_bounds__(kTailThreads)
       }
       lm[lc][3] = 0.0;
     }
-    __syncthreads();
-    double rz = 0.0;
-    for(int i = tid; i < n; i += nt) {
+    G.sync();
+    double rz[1] = {0.0};
+    for(int i = G.tid; i < n; i += G.nt) {
       const T xv = T(work[i]);
       x[i] = xv;
-      rz += (double)(b[i] - mb) * (double)xv;
+      rz[0] += (double)(b[i] - mb) * (double)xv;
     }
     if(lc == 0) {  // the finest level is the coarsest: r.z of the direct solve
-      rz = tail_sum<T>(rz);
-      if(tid == 0) rz_sh = rz;
+      G.sums(rz);
+      if(G.tid == 0) rz_sh = rz[0];
     }
-    __syncthreads();
+    G.sync();
+  };
+  const TailBlock GB{tid, (int)blockDim.x};
+  // levels l0 .. lw - 1 on the workgroup, lw .. nl - 1 on the first wave
+  int lw = l0;
+  while(lw < nl - 1 && lvs[lw].n > kWaveCells) ++lw;
+  const bool wave_coarsest = lvs[nl - 1].n <= kWaveCells;
+  for(int l = l0; l < lw; ++l) down(l, GB);
+  if(tid < 64) {
+    const TailWave GW{tid, 64};
+    for(int l = lw; l < nl - 1; ++l) down(l, GW);
+    if(wave_coarsest) coarsest(GW);
   }
-  // ---- up
-  for(int l = nl - 2; l >= l0; --l) {
-    const PLevel& L = lvs[l];
-    const PLevel& Cl = lvs[l + 1];
-    const T* b = Bp(l);
-    const T mb = T(lm[l][0]), m2 = T(lm[l][1]), mc = T(lm[l + 1][3]);
-    const int pxy = L.nx * L.ny;
-    double acc = 0.0;
-    for(int id = tid; id < L.n; id += nt) {
-      const int k = id / pxy, rem = id - k * pxy, j = rem / L.nx, i = rem - j * L.nx;
-      const T xv =
-          prolong_smooth_cell<T>(L, Cl, b, XA(l), XA(l + 1), mb, m2, mc, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
-      XB(l)[id] = xv;
-      acc += (double)xv;
-    }
-    acc = tail_sum<T>(acc);
-    if(tid == 0) lm[l][2] = zm ? acc / (double)L.n : 0.0;
-    __syncthreads();
-    const T m4 = T(lm[l][2]);
-    double s5 = 0.0, srz = 0.0;
-    for(int id = tid; id < L.n; id += nt) {
-      const int k = id / pxy, rem = id - k * pxy, j = rem / L.nx, i = rem - j * L.nx;
-      T bt;
-      const T xv = smooth_last_cell<T>(L, b, XB(l), mb, m4, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k), bt);
-      XA(l)[id] = xv;
-      s5 += (double)xv;
-      srz += (double)bt * (double)xv;
-    }
-    s5 = tail_sum<T>(s5);
-    srz = tail_sum<T>(srz);
-    if(tid == 0) {
-      lm[l][3] = zm ? s5 / (double)L.n : 0.0;
-      if(l == 0) rz_sh = srz;
-    }
-    __syncthreads();
+  __syncthreads();
+  if(!wave_coarsest) {
+    coarsest(GB);
   }
+  if(tid < 64) {
+    const TailWave GW{tid, 64};
+    for(int l = nl - 2; l >= lw; --l) up(l, GW);
+  }
+  __syncthreads();
+  for(int l = min(lw, nl - 1) - 1; l >= l0; --l) up(l, GB);
   if(LDS) {
     const int n0 = lvs[l0].n;
     const T* src = XA(l0);
@@ -2104,12 +2167,23 @@ void PoissonSolver::vcycle() {
     tb.owork = (int)work_off;
     const bool in_lds = (work_off + (size_t)nc) * sizeof(double) <= (size_t)kTailLdsBytes;
     ProfScope prof_(c, "k_vtail");
-    if(in_lds)
-      hipLaunchKernelGGL((k_vtail<T, true>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm, factor,
-                         U, work, ps, zmi);
-    else
-      hipLaunchKernelGGL((k_vtail<T, false>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm, factor,
-                         U, work, ps, zmi);
+    // all tail levels share the boundary types of the finest level
+    const bool tuni = lv[l0].per[0] && lv[l0].per[1] && lv[l0].per[2];
+    if(in_lds) {
+      if(tuni)
+        hipLaunchKernelGGL((k_vtail<T, true, true>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
+                           factor, U, work, ps, zmi);
+      else
+        hipLaunchKernelGGL((k_vtail<T, true, false>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
+                           factor, U, work, ps, zmi);
+    } else {
+      if(tuni)
+        hipLaunchKernelGGL((k_vtail<T, false, true>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
+                           factor, U, work, ps, zmi);
+      else
+        hipLaunchKernelGGL((k_vtail<T, false, false>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb,
+                           perm, factor, U, work, ps, zmi);
+    }
   }
   for(int l = l0 - 1; l >= 0; --l) {
     const PLevel& P = lv[l];
