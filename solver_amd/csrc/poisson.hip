@@ -66,7 +66,7 @@ enum PS : int {
   P_LEVEL = 32,    // per level: 8 slots (see LS)
   P_COUNT = 32 + 8 * 24
 };
-constexpr int kHostMirror = 8;  // leading scalars published to the host after every residual update
+constexpr int kHostMirror = 8;  // host mirror slot stride; P_DONE is published after every residual update
 enum LS : int { L_MB = 0, L_SBW, L_MX2, L_MX4, L_MX5, L_SUMB, L_SPARE1, L_SPARE2 };
 
 struct PLevel {
@@ -392,8 +392,8 @@ struct EpiDir {  // means of d and q, d.q, alpha or breakdown (linsolve/poisson_
 };
 
 // Residual RMS and convergence test (linsolve/poisson_solver.cpp:764-776). Always runs: it
-// also publishes the first scalars to the mapped host mirror, from which the host learns that
-// the solve has stopped.
+// also publishes the stop flag to the mapped host mirror, from which the host learns that the
+// solve has stopped.
 struct EpiUpdate {
   static constexpr int NV = 3;
   double* ps;
@@ -421,8 +421,8 @@ struct EpiUpdate {
         ps[P_DONE] = 1.0;
       }
     }
-    for(int q = 0; q < kHostMirror; ++q)
-      __hip_atomic_store(host_out + q, ps[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    // the host only polls the stop flag; the other scalars are copied back once at the end
+    __hip_atomic_store(host_out + P_DONE, ps[P_DONE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 };
 
