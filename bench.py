@@ -199,15 +199,22 @@ def step_bytes_model(n, iters, mixed):
     return (fixed + iters * per_it) * n ** 3 * 8
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
+def measured_traffic(kernel, mixed):
+    """HBM bytes per finest-grid launch of `kernel` from the committed rocprofv3 counter summary
+    (rows are named with template arguments, e.g. k_dir_apply<float, 8, true>; the first row of
+    a kernel is its finest-grid launch)."""
     try:
         text = open(TRAFFIC_TABLE).read()
     except OSError:
         return None
+    elem = "float" if mixed else "double"
     for line in text.splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 6 and cells[0] == kernel:
+        if len(cells) < 6:
+            continue
+        name = cells[0]
+        if name == kernel or name.startswith(f"{kernel}<{elem}") or (name.startswith(f"{kernel}<") and "," not in name
+                                                                       and elem in name):
             try:
                 return (float(cells[4]) + float(cells[5])) * 1e6
             except ValueError:
@@ -378,7 +385,7 @@ def main():
         "step_hbm": step_hbm(r, main_prec == sa.MIXED),
         "roofline": {"bound": "hbm", "kernel": a.profile_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": measured_traffic(a.profile_kernel),
+                     "traffic": measured_traffic(a.profile_kernel, main_prec == sa.MIXED),
                      "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": avg_ms,
                      "launches": prof["launches"]},
         "adjoint_s_per_step": None,
