@@ -860,7 +860,10 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
   T acc = T(0);
   T q1[2] = {T(0), T(0)}, q2[2] = {T(0), T(0)};  // tile rhs of planes kp - 1, kp - 2
   const int last = (t.k1 - t.k0) + 1;
-  auto step = [&](int s, T (&cur)[NB], T (&ctb)[2], T (&nxt)[NB], T (&ntb)[2]) {
+  // ALWAYS: the step is known to fetch (steady state), so the fetch is not under a branch. A
+  // fetch under a branch makes the wait counts at the join assume the path without it, which
+  // waits for every prefetched plane one step after it was requested.
+  auto step = [&](int s, T (&cur)[NB], T (&ctb)[2], T (&nxt)[NB], T (&ntb)[2], auto always) {
     const int kp = t.k0 + s;
     {
       T* d1 = sb[ring3(kp)];
@@ -875,7 +878,7 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
     }
     const T tb0 = own0 ? ctb[0] - mb : T(0), tb1 = own1 ? ctb[1] - mb : T(0);
     __syncthreads();
-    if(s + 2 <= last) fetch(kp + 2, nxt, ntb);
+    if(decltype(always)::value || s + 2 <= last) fetch(kp + 2, nxt, ntb);
     if(s >= 0) {  // x2 on plane kq = kp - 1, tile + halo 1
       const int kq = kp - 1;
       const T* x1m = sb[ring3(kq - 1)];
@@ -939,13 +942,16 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
   fetch(t.k0 - 2, pa, ta);
   fetch(t.k0 - 1, pb, tb);
   int s = -2;
-  for(; s + 2 <= last; s += 3) {
-    step(s, pa, ta, pc, tc);
-    step(s + 1, pb, tb, pa, ta);
-    step(s + 2, pc, tc, pb, tb);
+  for(; s + 4 <= last; s += 3) {  // every step of the group fetches
+    step(s, pa, ta, pc, tc, std::true_type{});
+    step(s + 1, pb, tb, pa, ta, std::true_type{});
+    step(s + 2, pc, tc, pb, tb, std::true_type{});
   }
-  if(s <= last) step(s, pa, ta, pc, tc);
-  if(s + 1 <= last) step(s + 1, pb, tb, pa, ta);
+  // at most four steps remain; whether they fetch depends on the position
+  if(s <= last) step(s, pa, ta, pc, tc, std::false_type{});
+  if(s + 1 <= last) step(s + 1, pb, tb, pa, ta, std::false_type{});
+  if(s + 2 <= last) step(s + 2, pc, tc, pb, tb, std::false_type{});
+  if(s + 3 <= last) step(s + 3, pa, ta, pc, tc, std::false_type{});
   const int ops[3] = {kSum, kSum, kSum};
   store_partials<3>(v, ops, partials);
 }
@@ -1018,7 +1024,7 @@ __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b,
   double v[2] = {0.0, 0.0};
   T q1[2] = {T(0), T(0)}, q2[2] = {T(0), T(0)};  // tile rhs of planes kp - 1, kp - 2
   const int last = (t.k1 - t.k0) + 1;
-  auto step = [&](int s, Pre& cur, Pre& nxt) {
+  auto step = [&](int s, Pre& cur, Pre& nxt, auto always) {  // ALWAYS: see k_down_fused
     const int kp = t.k0 + s;
     {
       T* d3 = s3[ring3(kp)];
@@ -1036,7 +1042,7 @@ __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b,
     }
     const T tb0 = own0 ? cur.tb[0] - mb : T(0), tb1 = own1 ? cur.tb[1] - mb : T(0);
     __syncthreads();
-    if(s + 2 <= last) fetch(kp + 2, nxt);
+    if(decltype(always)::value || s + 2 <= last) fetch(kp + 2, nxt);
     if(s >= 0) {  // x4 on plane kq = kp - 1, tile + halo 1
       const int kq = kp - 1;
       const int sm = ring3(kq - 1), sc = ring3(kq), sp = ring3(kq + 1);
@@ -1085,13 +1091,15 @@ __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b,
   fetch(t.k0 - 2, pa);
   fetch(t.k0 - 1, pb);
   int s = -2;
-  for(; s + 2 <= last; s += 3) {
-    step(s, pa, pc);
-    step(s + 1, pb, pa);
-    step(s + 2, pc, pb);
+  for(; s + 4 <= last; s += 3) {  // every step of the group fetches
+    step(s, pa, pc, std::true_type{});
+    step(s + 1, pb, pa, std::true_type{});
+    step(s + 2, pc, pb, std::true_type{});
   }
-  if(s <= last) step(s, pa, pc);
-  if(s + 1 <= last) step(s + 1, pb, pa);
+  if(s <= last) step(s, pa, pc, std::false_type{});
+  if(s + 1 <= last) step(s + 1, pb, pa, std::false_type{});
+  if(s + 2 <= last) step(s + 2, pc, pb, std::false_type{});
+  if(s + 3 <= last) step(s + 3, pa, pc, std::false_type{});
   const int ops[2] = {kSum, kSum};
   store_partials<2>(v, ops, partials);
 }
@@ -1792,25 +1800,34 @@ HotLaunch hot_launch(const PLevel& L) {
 
 // Launch shape of the fused all-periodic level kernels: 64x8 tiles, kz planes per block (even,
 // so that z pairs of the restriction stay inside a block). Each block re-reads the 4 planes
-// around its range, so long ranges are cheaper, but a level must still have enough blocks to
-// fill the chip: the longest range in {16, 8, 4, 2} that gives at least 1024 blocks.
+// around its range and pays a pipeline fill of 4 plane steps, so long ranges are cheaper; the
+// grid is sized to one round of resident blocks (occupancy x compute units), which also leaves
+// no partial last round: kz is the shortest even range that needs at most `resident` blocks
+// (at least 2 planes).
 struct FusedLaunch {
   dim3 grid;
   int kz;
 };
-FusedLaunch fused_launch(const PLevel& L) {
+FusedLaunch fused_launch(const PLevel& L, int resident) {
   FusedLaunch f;
   const size_t tiles = (size_t)((L.nx + kFX - 1) / kFX) * (size_t)((L.ny + kFY - 1) / kFY);
-  f.kz = 2;
-  for(int kz : {16, 8, 4})
-    if(tiles * (size_t)((L.nz + kz - 1) / kz) >= 1024) {
-      f.kz = kz;
-      break;
-    }
+  const size_t chunks = std::max<size_t>(1, (size_t)std::max(resident, 1) / tiles);
+  int kz = (int)(((size_t)L.nz + chunks - 1) / chunks);
+  kz = std::max(2, kz + (kz & 1));
+  f.kz = kz;
   const size_t n = tiles * (size_t)((L.nz + f.kz - 1) / f.kz);
   if(n > (size_t)kMaxPartials) throw InvalidArgument("grid too large for the reduction buffer");
   f.grid = dim3((unsigned)n);
   return f;
+}
+
+// Blocks of `kernel` (launched with `threads` threads) that the device holds at once.
+template <typename K>
+int resident_blocks(K kernel, int threads, int device) {
+  int per_cu = 0, cus = 0;
+  OFH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+  OFH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  return std::max(1, per_cu) * std::max(1, cus);
 }
 
 // Instantiates KERN<pre..., KZ, UNI> for the launch shape h.
@@ -1852,6 +1869,7 @@ class PoissonSolver {
   PLevel* dlv = nullptr;      // level descriptors for the tail kernel
   int l0 = 0;                 // first level handled by the one-workgroup tail
   std::vector<char> fused;    // per level: the fused all-periodic kernels run it
+  int resident_down = 0, resident_up = 0;  // blocks of the fused kernels the device holds at once
   // coarse solve data
   int nc = 0;
   int* perm = nullptr;
@@ -2075,6 +2093,13 @@ void PoissonSolver::build() {
     const bool uni = P.per[0] && P.per[1] && P.per[2];
     fused[l] = uni && P.rx == 2 && P.ry == 2 && P.rz == 2 && P.nx >= 4 && P.ny >= 4 && P.nz >= 4 && !fused_disabled();
   }
+  if(precision == OF_PRECISION_MIXED) {
+    resident_down = resident_blocks(k_down_fused<float>, kFThreads, c.device);
+    resident_up = resident_blocks(k_up_fused<float>, kFThreads, c.device);
+  } else {
+    resident_down = resident_blocks(k_down_fused<double>, kFThreads, c.device);
+    resident_up = resident_blocks(k_up_fused<double>, kFThreads, c.device);
+  }
   const size_t n0 = (size_t)lv[0].n;
   OFH_CHECK(hipMalloc(&pc, n0 * sizeof(double)));
   OFH_CHECK(hipMalloc(&r, n0 * sizeof(double)));
@@ -2128,7 +2153,7 @@ void PoissonSolver::vcycle() {
     const PLevel& P = lv[l];
     const PLevel& C = lv[l + 1];
     if(fused[l]) {
-      const FusedLaunch f = fused_launch(P);
+      const FusedLaunch f = fused_launch(P, resident_down);
       {
         ProfScope prof_(c, "k_down_fused", l == 0);
         hipLaunchKernelGGL(k_down_fused<T>, f.grid, dim3(kFThreads), 0, c.stream, bptr(l), static_cast<T*>(lxa[l]), P,
@@ -2190,7 +2215,7 @@ void PoissonSolver::vcycle() {
     const PLevel& C = lv[l + 1];
     const T* xc = static_cast<const T*>(x_final(l + 1));
     if(fused[l]) {
-      const FusedLaunch f = fused_launch(P);
+      const FusedLaunch f = fused_launch(P, resident_up);
       {
         ProfScope prof_(c, "k_up_fused", l == 0);
         hipLaunchKernelGGL(k_up_fused<T>, f.grid, dim3(kFThreads), 0, c.stream, bptr(l),
