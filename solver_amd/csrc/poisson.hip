@@ -750,6 +750,55 @@ __global__ void __launch_bounds__(256) k_smooth_last(const T* __restrict__ b, co
   store_partials<2>(v, ops, partials);
 }
 
+// ---------------------------------------------------------------- folded reductions
+// A consumer kernel can fold the partials of the kernel before it instead of a one-block
+// finalize launch between them (for producers of at most a few thousand blocks): every block
+// combines the partials in the same fixed order, so every block sees the same values, and
+// block 0 stores the epilogue's scalars for the later kernels. The producer and the consumer
+// use different partial buffers.
+template <int N>
+__device__ inline void fold_all(const double* __restrict__ src, unsigned nb, double (&out)[N]) {
+  __shared__ double fsh[N];
+  int ops[N];
+#pragma unroll
+  for(int k = 0; k < N; ++k) ops[k] = kSum;
+  fold_partials<N>(src, nb, ops, out);
+  if(threadIdx.x == 0 && threadIdx.y == 0) {
+#pragma unroll
+    for(int k = 0; k < N; ++k) fsh[k] = out[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for(int k = 0; k < N; ++k) out[k] = fsh[k];
+}
+
+// Folded EpiDown of the finer level pl into level l (the coarse one): returns mb and the sum of
+// b w/D of level l; block 0 stores what EpiDown stores.
+struct DownFold {
+  const double* src;  // partials of the finer level's k_down_fused (null: read ps)
+  unsigned nb;
+  int pslot;  // slot of the finer level
+  double pn;  // cells of the finer level
+};
+__device__ inline void down_fold(const DownFold& f, double* ps, int lslot, double n, int zm, double& mb,
+                                 double& sbw) {
+  if(f.src == nullptr) {
+    mb = ps[lslot + L_MB];
+    sbw = ps[lslot + L_SBW];
+    return;
+  }
+  double r[3];
+  fold_all<3>(f.src, f.nb, r);
+  mb = zm ? r[1] / n : 0.0;
+  sbw = r[2];
+  if(blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    ps[f.pslot + L_MX2] = zm ? r[0] / f.pn : 0.0;
+    ps[lslot + L_MB] = mb;
+    ps[lslot + L_SBW] = sbw;
+    ps[lslot + L_SUMB] = r[1];
+  }
+}
+
 // ---------------------------------------------------------------- fused all-periodic levels
 // On a level that is periodic on every axis and halves every axis, the down leg (two
 // pre-smoothing sweeps, residual, restriction) and the up leg (prolongation, two post-smoothing
@@ -764,6 +813,7 @@ __global__ void __launch_bounds__(256) k_smooth_last(const T* __restrict__ b, co
 // FP64, against 33 B for the two unfused kernels); the up kernel reads x2, b and the coarse
 // correction and writes x5 (25 B against 49 B).
 constexpr int kFX = 64, kFY = 8, kFThreads = 256;
+constexpr int kMaxFoldPartials = 4096;  // producers with more blocks keep their finalize launch
 
 __device__ inline int wrap_index(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 __device__ inline int ring3(int k) { return (k + 6) % 3; }  // k >= -6
@@ -794,7 +844,7 @@ __device__ inline FusedTile fused_tile(const PLevel& L, int kz) {
 template <typename T>
 __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ b, T* __restrict__ x2g, PLevel L,
                                                           PLevel Cl, T* __restrict__ bc, double* partials,
-                                                          const double* ps, int lslot, int zm, int kz) {
+                                                          double* ps, int lslot, int zm, int kz, DownFold fold) {
   if(stopped(ps)) return;
   constexpr int BW = kFX + 4, BH = kFY + 4, BP = BW * BH;  // x1, halo 2
   constexpr int XW = kFX + 2, XH = kFY + 2, XP = XW * XH;  // b - mb and x2, halo 1
@@ -807,8 +857,10 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
   const size_t cplane = (size_t)Cl.nx * Cl.ny;
-  const T mb = T(ps[lslot + L_MB]);
-  const T m1 = zm ? T((ps[lslot + L_SBW] - ps[lslot + L_MB] * L.winv_sum) / (double)L.n) : T(0);
+  double mbd, sbw;
+  down_fold(fold, ps, lslot, (double)L.n, zm, mbd, sbw);
+  const T mb = T(mbd);
+  const T m1 = zm ? T((sbw - mbd * L.winv_sum) / (double)L.n) : T(0);
   const T w = T(L.winv[0]);
   const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
   const T inv_children = T(1.0 / 8.0);
@@ -959,8 +1011,9 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
 template <typename T>
 __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b, const T* __restrict__ x2,
                                                         const T* __restrict__ xc, PLevel L, PLevel Cl,
-                                                        T* __restrict__ x5g, double* partials, const double* ps,
-                                                        int lslot, int cslot, int kz) {
+                                                        T* __restrict__ x5g, double* partials, double* ps,
+                                                        int lslot, int cslot, int kz, const double* fold_src,
+                                                        unsigned fold_nb, int zm) {
   if(stopped(ps)) return;
   constexpr int AW = kFX + 4, AH = kFY + 4, AP = AW * AH;  // x3, halo 2
   constexpr int XW = kFX + 2, XH = kFY + 2, XP = XW * XH;  // x4 and b - mb, halo 1
@@ -974,7 +1027,18 @@ __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b,
   const size_t cplane = (size_t)Cl.nx * Cl.ny;
   const T mb = T(ps[lslot + L_MB]);
   const T m2 = T(ps[lslot + L_MX2]);
-  const T mc = T(ps[cslot + L_MX5]);
+  // mean of the coarse correction: EpiSmoothLast of the coarser level, folded here when its
+  // partials are passed (fold_src), else stored in ps by its finalize or by the tail
+  double mcd;
+  if(fold_src != nullptr) {
+    double r[2];
+    fold_all<2>(fold_src, fold_nb, r);
+    mcd = zm ? r[0] / (double)Cl.n : 0.0;
+    if(blockIdx.x == 0 && tid == 0) ps[cslot + L_MX5] = mcd;
+  } else {
+    mcd = ps[cslot + L_MX5];
+  }
+  const T mc = T(mcd);
   const T w = T(L.winv[0]);
   const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
   const FusedTile t = fused_tile(L, kz);
@@ -1176,7 +1240,7 @@ template <typename T, bool LDS, bool UNI>
 __global__ void __launch_bounds__(kTailThreads)
     k_vtail(const PLevel* __restrict__ lvs, int l0, int nl, TailBufs<T> B, const int* __restrict__ perm,
             const double* __restrict__ factor, const double* __restrict__ U, double* __restrict__ gwork, double* ps,
-            int zm) {
+            int zm, DownFold fold) {
   if(stopped(ps)) return;
   __shared__ double sbuf[kTailLdsBytes / sizeof(double)];
   T* sT = reinterpret_cast<T*>(sbuf);
@@ -1196,9 +1260,11 @@ __global__ void __launch_bounds__(kTailThreads)
   __shared__ double rz_sh;
   const int tid = threadIdx.x;
   const int s0 = P_LEVEL + 8 * l0;
+  double mb0, sbw0;
+  down_fold(fold, ps, s0, (double)lvs[l0].n, zm, mb0, sbw0);
   if(tid == 0) {
-    lm[l0][0] = ps[s0 + L_MB];
-    lsbw[l0] = ps[s0 + L_SBW];
+    lm[l0][0] = mb0;
+    lsbw[l0] = sbw0;
   }
   __syncthreads();
   // down leg of level l: two sweeps (x2 and its mean), residual restricted to level l + 1
@@ -1870,6 +1936,7 @@ class PoissonSolver {
   int l0 = 0;                 // first level handled by the one-workgroup tail
   std::vector<char> fused;    // per level: the fused all-periodic kernels run it
   int resident_down = 0, resident_up = 0;  // blocks of the fused kernels the device holds at once
+  double* partials2 = nullptr;  // second partial buffer: a folding consumer writes beside its input
   // coarse solve data
   int nc = 0;
   int* perm = nullptr;
@@ -2045,6 +2112,7 @@ PoissonSolver::~PoissonSolver() {
   for(void* p : lxa) fr(p);
   for(void* p : lxb) fr(p);
   fr(ps);
+  fr(partials2);
   fr(history);
   fr(perm);
   fr(factor);
@@ -2120,6 +2188,7 @@ void PoissonSolver::build() {
   }
   OFH_CHECK(hipMalloc(&ps, P_COUNT * sizeof(double)));
   OFH_CHECK(hipMemsetAsync(ps, 0, P_COUNT * sizeof(double), c.stream));
+  OFH_CHECK(hipMalloc(&partials2, (size_t)kMaxRed * kMaxPartials * sizeof(double)));
   OFH_CHECK(hipHostMalloc(&host_ps, 2 * kHostMirror * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
   OFH_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&host_ps_dev), host_ps, 0));
   for(int q = 0; q < 2 * kHostMirror; ++q) host_ps[q] = 0.0;
@@ -2149,18 +2218,31 @@ void PoissonSolver::vcycle() {
     if(l == 0 && precision == OF_PRECISION_FP64) return reinterpret_cast<T*>(r);
     return static_cast<T*>(lb[l]);
   };
+  // A fused level whose consumer is the next fused level or the tail leaves its partials for the
+  // consumer to fold (no finalize launch); the two partial buffers alternate along the chain.
+  double* pbuf[2] = {c.partials, partials2};
+  int which = 0;
+  DownFold pend{nullptr, 0u, 0, 0.0};
+  auto foldable = [](const dim3& g) { return g.x <= (unsigned)kMaxFoldPartials; };
   for(int l = 0; l < l0; ++l) {
     const PLevel& P = lv[l];
     const PLevel& C = lv[l + 1];
     if(fused[l]) {
       const FusedLaunch f = fused_launch(P, resident_down);
+      double* out = pbuf[which];
       {
         ProfScope prof_(c, "k_down_fused", l == 0);
         hipLaunchKernelGGL(k_down_fused<T>, f.grid, dim3(kFThreads), 0, c.stream, bptr(l), static_cast<T*>(lxa[l]), P,
-                           C, static_cast<T*>(lb[l + 1]), c.partials, ps, slot(l), zmi, f.kz);
+                           C, static_cast<T*>(lb[l + 1]), out, ps, slot(l), zmi, f.kz, pend);
       }
-      launch_finalize(c.stream, c.partials, f.grid,
-                      EpiDown{ps, slot(l), slot(l + 1), (double)P.n, (double)C.n, zmi});
+      const bool consumer_folds = (l + 1 == l0 || fused[l + 1]) && foldable(f.grid);
+      if(consumer_folds) {
+        pend = DownFold{out, f.grid.x, slot(l), (double)P.n};
+        which ^= 1;
+      } else {
+        launch_finalize(c.stream, out, f.grid, EpiDown{ps, slot(l), slot(l + 1), (double)P.n, (double)C.n, zmi});
+        pend = DownFold{nullptr, 0u, 0, 0.0};
+      }
       continue;
     }
     const HotLaunch h = hot_launch(P);
@@ -2197,32 +2279,45 @@ void PoissonSolver::vcycle() {
     if(in_lds) {
       if(tuni)
         hipLaunchKernelGGL((k_vtail<T, true, true>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
-                           factor, U, work, ps, zmi);
+                           factor, U, work, ps, zmi, pend);
       else
         hipLaunchKernelGGL((k_vtail<T, true, false>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
-                           factor, U, work, ps, zmi);
+                           factor, U, work, ps, zmi, pend);
     } else {
       if(tuni)
         hipLaunchKernelGGL((k_vtail<T, false, true>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
-                           factor, U, work, ps, zmi);
+                           factor, U, work, ps, zmi, pend);
       else
         hipLaunchKernelGGL((k_vtail<T, false, false>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb,
-                           perm, factor, U, work, ps, zmi);
+                           perm, factor, U, work, ps, zmi, pend);
     }
   }
+  const double* up_src = nullptr;  // partials of the coarser fused up kernel, to be folded
+  unsigned up_nb = 0;
   for(int l = l0 - 1; l >= 0; --l) {
     const PLevel& P = lv[l];
     const PLevel& C = lv[l + 1];
     const T* xc = static_cast<const T*>(x_final(l + 1));
     if(fused[l]) {
       const FusedLaunch f = fused_launch(P, resident_up);
+      double* out = pbuf[which];
       {
         ProfScope prof_(c, "k_up_fused", l == 0);
         hipLaunchKernelGGL(k_up_fused<T>, f.grid, dim3(kFThreads), 0, c.stream, bptr(l),
-                           static_cast<const T*>(lxa[l]), xc, P, C, static_cast<T*>(lxb[l]), c.partials, ps, slot(l),
-                           slot(l + 1), f.kz);
+                           static_cast<const T*>(lxa[l]), xc, P, C, static_cast<T*>(lxb[l]), out, ps, slot(l),
+                           slot(l + 1), f.kz, up_src, up_nb, zmi);
       }
-      launch_finalize(c.stream, c.partials, f.grid, EpiSmoothLast{ps, slot(l), (double)P.n, zmi, l == 0 ? 1 : 0});
+      // the finer fused level folds this level's mean of x5; the finest level's finalize also
+      // forms r.z and beta
+      if(l > 0 && fused[l - 1] && foldable(f.grid)) {
+        up_src = out;
+        up_nb = f.grid.x;
+        which ^= 1;
+      } else {
+        launch_finalize(c.stream, out, f.grid, EpiSmoothLast{ps, slot(l), (double)P.n, zmi, l == 0 ? 1 : 0});
+        up_src = nullptr;
+        up_nb = 0;
+      }
       continue;
     }
     const HotLaunch h = hot_launch(P);
