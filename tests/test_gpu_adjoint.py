@@ -102,10 +102,11 @@ def test_gradient_and_divergence_adjoint(ctx):
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
 
 
-@pytest.mark.parametrize("alpha", [0.0, 0.02, 0.7])
-def test_predictor_adi_adjoint(ctx, alpha):
-    g = periodic_grid(9, 8, 6)
-    b = sa.taylor_green_bcs(6)
+# (40, 18, 34): lines of several 16-element segments of the two-pass line solver
+@pytest.mark.parametrize("alpha,shape", [(0.0, (9, 8, 6)), (0.02, (9, 8, 6)), (0.7, (9, 8, 6)), (0.7, (40, 18, 34))])
+def test_predictor_adi_adjoint(ctx, alpha, shape):
+    g = periodic_grid(*shape)
+    b = sa.taylor_green_bcs(shape[2])
     rng = np.random.default_rng(2)
     r, y = rand_fields(rng, g, LOCS), rand_fields(rng, g, LOCS)
     out = [dev(np.zeros_like(a)) for a in r]

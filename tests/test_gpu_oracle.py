@@ -188,3 +188,36 @@ def test_taylor_green_predictor_rhs_vs_oracle(ctx, shape, scheme, has_prev):
         a = host(t)[1:-1, 1:-1, 1:-1]
         b = ref[1:-1, 1:-1, 1:-1]
         assert rel_max(a, b) <= 1e-14, f"{k}: {rel_max(a, b):.3e}"
+
+
+# ADI predictor on lines longer than one 16-element segment of the two-pass line solver (first,
+# interior and last segments, a partial last segment), for every boundary family: walls with a
+# moving lid, symmetry and fixed-pressure faces (non-cyclic lines, Dirichlet normal faces
+# skipped), and periodic axes (cyclic lines with the duplicated normal face).
+@pytest.mark.parametrize("types,shape,alpha", [
+    ([sa.NO_SLIP, sa.NO_SLIP, sa.NO_SLIP, sa.PRESCRIBED, sa.SYMMETRY, sa.SYMMETRY], (40, 37, 33), 0.02),
+    ([sa.PERIODIC, sa.PERIODIC, sa.NO_SLIP, sa.FIXED_PRESSURE, sa.PERIODIC, sa.PERIODIC], (35, 48, 20), 0.5),
+    ([sa.PERIODIC] * 6, (64, 18, 50), 3.0),
+])
+def test_predictor_adi_long_lines_vs_oracle(ctx, types, shape, alpha):
+    nx, ny, nz = shape
+    h = (1.0 / nx, 1.2 / ny, 0.8 / nz)
+    g = sa.grid(nx, ny, nz, *h)
+    og = ofref.make_grid(nx, ny, nz, *h)
+    vel = np.zeros((6, 3))
+    vel[3] = [0.7, 0.0, 0.2]  # moving lid / inflow velocity where the face prescribes one
+    b = sa.bc_set(types, vel)
+    ob = ofref.bcset(types, vel)
+    rng = np.random.default_rng(nx + 7 * ny + 13 * nz)
+    locs = [sa.FACE_X, sa.FACE_Y, sa.FACE_Z]
+    rhs = [rng.standard_normal(sa.storage_shape(nx, ny, nz, loc)) for loc in locs]
+    out = [zeros(g, loc) for loc in locs]
+    ls = C.c_int()
+    ctx.run("of_solve_predictor_adi", C.byref(g), C.byref(b), C.c_double(alpha), *[sa.ptr(dev(a)) for a in rhs],
+            *map(sa.ptr, out), C.byref(ls))
+    ref, rls = ofref.predictor_adi(og, ob, alpha, [a.copy() for a in rhs])
+    assert ls.value == rls
+    for k, (a, r) in enumerate(zip(out, ref)):
+        # storage including ghosts: the line solves and the ghost fills both compared
+        err = np.max(np.abs(host(a) - r)) / np.max(np.abs(r))
+        assert err <= 1e-13, (k, err)
