@@ -193,6 +193,18 @@ int of_solve_predictor_adi(of_context* ctx, const of_grid* grid, const of_bc_set
                            const double* rhs_u, const double* rhs_v, const double* rhs_w, double* out_u,
                            double* out_v, double* out_w, int* line_solves);
 
+/* The CG alternative to the line solves, on grids periodic on every axis: unpreconditioned
+ * conjugate gradients on the factorised operator per component, started from the right-hand
+ * side, stopping at sqrt(r.r / r0.r0) <= tolerance (metal/taylor_green_backend.mm:546-621, the
+ * reference's Metal backend; its FP64 tolerance is 1e-12 with at most 80 iterations). The result
+ * carries BC ghosts like of_solve_predictor_adi. iterations (may be NULL): the largest count of
+ * the three components; relative_residual (may be NULL): the largest final relative residual.
+ * Fails with a message when a component does not converge or a boundary is not periodic. */
+int of_solve_predictor_cg(of_context* ctx, const of_grid* grid, const of_bc_set* bcs, double alpha,
+                          const double* rhs_u, const double* rhs_v, const double* rhs_w, double* out_u,
+                          double* out_v, double* out_w, double tolerance, int max_iterations, int* iterations,
+                          double* relative_residual);
+
 /* ---------------------------------------------------------------- projection */
 /* solver/projection.hpp:54-57 build_pressure_rhs */
 int of_build_pressure_rhs(of_context* ctx, const of_grid* grid, const of_bc_set* bcs,
@@ -293,6 +305,14 @@ int of_stepper_kinetic_energy(of_stepper* s, double* out);
  * predictor right-hand side as rhs += dt f after the reference's terms
  * (solver/taylor_green.cpp:227-256 has no such term). NULL arrays clear it. */
 int of_stepper_set_forcing(of_stepper* s, const double* fu, const double* fv, const double* fw);
+
+/* Helmholtz predictor of the stepper's steps: OF_PREDICTOR_ADI (default: the reference's exact
+ * line solves) or OF_PREDICTOR_CG (of_solve_predictor_cg with the given tolerance and iteration
+ * limit; periodic Taylor-Green only). No counterpart in the reference's CPU path; the CG form is
+ * its Metal backend's. */
+#define OF_PREDICTOR_ADI 0
+#define OF_PREDICTOR_CG 1
+int of_stepper_set_predictor(of_stepper* s, int kind, double tolerance, int max_iterations);
 
 /* ---------------------------------------------------------------- discrete adjoint */
 /* The reference has no derivative code. These entry points add the discrete adjoint and the

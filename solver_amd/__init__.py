@@ -22,6 +22,7 @@ OF_OK, OF_INVALID_ARGUMENT, OF_RUNTIME_ERROR, OF_HIP_ERROR = 0, -1, -2, -3
 CELL, FACE_X, FACE_Y, FACE_Z = 0, 1, 2, 3
 NO_SLIP, PRESCRIBED, SYMMETRY, FIXED_PRESSURE, PERIODIC = 0, 1, 2, 3, 4
 P_NEUMANN, P_DIRICHLET, P_PERIODIC = 0, 1, 2
+PREDICTOR_ADI, PREDICTOR_CG = 0, 1
 FP64, MIXED = 0, 1
 TAYLOR_GREEN, LID_DRIVEN_CAVITY = 0, 1
 
@@ -106,12 +107,13 @@ EXPORTS = [
     "of_apply_total_pressure_boundary_conditions", "of_derive_pressure_correction_boundary_conditions",
     "of_compute_gradient", "of_compute_divergence", "of_compute_laplacian", "of_compute_advection_term",
     "of_compute_diffusion_term", "of_compute_advective_cfl", "of_compute_factorized_correction",
-    "of_assemble_predictor_rhs", "of_solve_predictor_adi", "of_build_pressure_rhs", "of_correct_velocity",
+    "of_assemble_predictor_rhs", "of_solve_predictor_adi", "of_solve_predictor_cg", "of_build_pressure_rhs",
+    "of_correct_velocity",
     "of_project_velocity", "of_solve_pressure_poisson", "of_build_poisson_rhs_from_pressure",
     "of_apply_poisson_operator", "of_taylor_green_dt", "of_lid_driven_cavity_dt", "of_stepper_create",
     "of_stepper_destroy", "of_stepper_initialize", "of_stepper_upload", "of_stepper_download",
     "of_stepper_device_fields", "of_stepper_run", "of_stepper_run_batched", "of_stepper_kinetic_energy",
-    "of_stepper_set_forcing",
+    "of_stepper_set_forcing", "of_stepper_set_predictor",
     "of_apply_velocity_boundary_conditions_adjoint", "of_apply_pressure_boundary_conditions_adjoint",
     "of_compute_gradient_adjoint", "of_compute_divergence_adjoint", "of_solve_predictor_adi_adjoint",
     "of_compute_advection_tangent", "of_compute_advection_adjoint",
@@ -355,6 +357,11 @@ class Stepper:
         fs = [None] * 3 if f is None else list(f)
         call("of_stepper_set_forcing", self.h, *map(ptr, fs))
         self.ctx.sync()
+
+    def set_predictor(self, kind=PREDICTOR_ADI, tolerance=1e-12, max_iterations=80):
+        """Helmholtz predictor of the following steps: PREDICTOR_ADI (exact line solves) or
+        PREDICTOR_CG (conjugate gradients on the factorised operator; periodic grids only)."""
+        call("of_stepper_set_predictor", self.h, int(kind), C.c_double(tolerance), int(max_iterations))
 
 
 class Adjoint:

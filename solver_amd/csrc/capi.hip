@@ -348,6 +348,28 @@ int of_solve_predictor_adi(of_context* c, const of_grid* g, const of_bc_set* b, 
   });
 }
 
+int of_solve_predictor_cg(of_context* c, const of_grid* g, const of_bc_set* b, double alpha, const double* ru,
+                          const double* rv, const double* rw, double* ou, double* ov, double* ow, double tol, int maxit,
+                          int* iterations, double* relative_residual) {
+  return guarded([&] {
+    validate_grid(g);
+    need(b, "bcs");
+    if(!(tol > 0.0)) throw InvalidArgument("solve_predictor_cg expects tolerance > 0");
+    Context& x = C(c);
+    HelmholtzCG* h = helmholtz_cg_create(x, *g);
+    try {
+      double rel = 0.0;
+      const int n = predictor_cg(*h, *g, *b, alpha, ru, rv, rw, ou, ov, ow, tol, maxit, &rel);
+      if(iterations) *iterations = n;
+      if(relative_residual) *relative_residual = rel;
+    } catch(...) {
+      helmholtz_cg_destroy(h);
+      throw;
+    }
+    helmholtz_cg_destroy(h);
+  });
+}
+
 int of_build_pressure_rhs(of_context* c, const of_grid* g, const of_bc_set* b, const of_projection_options* o,
                           const double* u, const double* v, const double* w, double* rhs) {
   return guarded([&] {
@@ -650,6 +672,14 @@ int of_stepper_set_forcing(of_stepper* s, const double* fu, const double* fv, co
     OFH_CHECK(hipSetDevice(s->s->c.device));
     const double* f[3] = {fu, fv, fw};
     s->s->set_forcing(fu || fv || fw ? f : nullptr);
+  });
+}
+
+int of_stepper_set_predictor(of_stepper* s, int kind, double tol, int maxit) {
+  return guarded([&] {
+    need(s, "stepper");
+    OFH_CHECK(hipSetDevice(s->s->c.device));
+    s->s->set_predictor(kind, tol, maxit);
   });
 }
 

@@ -90,6 +90,16 @@ void launch_axpy_active(Context& c, const Lay& L, double a, const double* x, dou
 // ------------------------------------------------------------------ helmholtz.hip
 int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* ru,
                   const double* rv, const double* rw, double* ou, double* ov, double* ow);
+// Conjugate-gradient alternative to predictor_adi for grids periodic on every axis (work arrays
+// of one grid in a HelmholtzCG). Returns the largest iteration count of the three components;
+// throws when one did not reach tol within max_iterations. worst_rel (may be null) receives the
+// largest final relative residual.
+struct HelmholtzCG;
+HelmholtzCG* helmholtz_cg_create(Context& c, const of_grid& g);
+void helmholtz_cg_destroy(HelmholtzCG* h);
+int predictor_cg(HelmholtzCG& S, const of_grid& g, const of_bc_set& b, double alpha, const double* ru,
+                 const double* rv, const double* rw, double* ou, double* ov, double* ow, double tol,
+                 int max_iterations, double* worst_rel);
 // Transpose of predictor_adi (rhs -> out) for periodic boundaries; rbar may alias ybar.
 void predictor_adi_adjoint(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* const* ybar,
                            double* const* rbar);

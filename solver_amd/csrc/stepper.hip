@@ -203,7 +203,21 @@ Stepper::Stepper(Context& ctx, const of_stepper_config& cfg) : c(ctx), cfg(cfg) 
 
 Stepper::~Stepper() {
   if(poisson) poisson_destroy(poisson);
+  if(hcg) helmholtz_cg_destroy(hcg);
   for(double* p : owned) (void)hipFree(p);
+}
+
+void Stepper::set_predictor(int kind, double tol, int maxit) {
+  if(kind != OF_PREDICTOR_ADI && kind != OF_PREDICTOR_CG) throw InvalidArgument("unknown predictor kind");
+  if(kind == OF_PREDICTOR_CG) {
+    if(!all_periodic) throw InvalidArgument("the CG Helmholtz predictor supports periodic boundaries");
+    if(!(tol > 0.0)) throw InvalidArgument("the CG Helmholtz predictor expects tolerance > 0");
+    if(maxit <= 0) throw InvalidArgument("the CG Helmholtz predictor expects max_iterations > 0");
+    if(!hcg) hcg = helmholtz_cg_create(c, g);
+    predictor_tol = tol;
+    predictor_maxit = maxit;
+  }
+  predictor_kind = kind;
 }
 
 void Stepper::total_pressure_bc(const double* const* v) {
@@ -269,7 +283,11 @@ void Stepper::step() {
   if(force[0])
     for(int a = 0; a < 3; ++a) launch_axpy_active(c, make_lay(g, OF_FACE_X + a), dt, force[a], rhs[a]);
   // Helmholtz predictor (exact line solves); output carries BC ghosts
-  predictor_adi(c, g, bcs, 0.5 * nu * dt, rhs[0], rhs[1], rhs[2], pred[0], pred[1], pred[2]);
+  if(predictor_kind == OF_PREDICTOR_CG)
+    predictor_iterations = predictor_cg(*hcg, g, bcs, 0.5 * nu * dt, rhs[0], rhs[1], rhs[2], pred[0], pred[1],
+                                        pred[2], predictor_tol, predictor_maxit, nullptr);
+  else
+    predictor_adi(c, g, bcs, 0.5 * nu * dt, rhs[0], rhs[1], rhs[2], pred[0], pred[1], pred[2]);
   // projection (solver/projection.cpp:1097-1131); pred already carries its BC ghosts
   launch_divergence_scaled(c, g, pred[0], pred[1], pred[2], 1.0 / dt, prhs);
   OFH_CHECK(hipMemsetAsync(phi, 0, make_lay(g, OF_CELL).size() * sizeof(double), c.stream));

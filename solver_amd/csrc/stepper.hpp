@@ -28,6 +28,12 @@ struct Stepper {
   double* force[3]{};  // optional body force f: rhs += dt f on the active range (null: none)
   double* force_buf[3]{};
   PoissonSolver* poisson = nullptr;
+  // Helmholtz predictor: OF_PREDICTOR_ADI, or OF_PREDICTOR_CG with its work arrays
+  int predictor_kind = OF_PREDICTOR_ADI;
+  double predictor_tol = 1e-12;
+  int predictor_maxit = 80;
+  HelmholtzCG* hcg = nullptr;
+  int predictor_iterations = 0;  // CG iterations of the last step (largest component)
   int has_prev = 0;
   of_step_metrics metrics{};
   std::vector<double*> owned;
@@ -41,6 +47,8 @@ struct Stepper {
   // Copies a device forcing field (three face arrays in the velocity layout) into the stepper;
   // null clears it.
   void set_forcing(const double* const* f);
+  // Selects the predictor solver (the CG one needs every axis periodic).
+  void set_predictor(int kind, double tol, int maxit);
 };
 
 }  // namespace ofh

@@ -221,3 +221,63 @@ def test_predictor_adi_long_lines_vs_oracle(ctx, types, shape, alpha):
         # storage including ghosts: the line solves and the ghost fills both compared
         err = np.max(np.abs(host(a) - r)) / np.max(np.abs(r))
         assert err <= 1e-13, (k, err)
+
+
+# The CG Helmholtz predictor (the reference's Metal backend's form) solves the same factorised
+# system as the line solves, so at a tight tolerance it reproduces the oracle's ADI result.
+# (a / h^2 up to about 0.4: unpreconditioned CG needs more iterations as the operator's condition
+# number (1 + 4 a / h^2)^3 grows; the Taylor-Green workloads have a / h^2 of 1e-3 to 1e-2)
+@pytest.mark.parametrize("shape,alpha", [((40, 24, 36), 0.01), ((64, 64, 64), 0.002), ((17, 9, 12), 0.05)])
+def test_predictor_cg_matches_oracle_adi(ctx, shape, alpha):
+    nx, ny, nz = shape
+    h = (2 * math.pi / nx, 2 * math.pi / ny, 2 * math.pi / nz)
+    g = sa.grid(nx, ny, nz, *h)
+    og = ofref.make_grid(nx, ny, nz, *h)
+    b = sa.bc_set([sa.PERIODIC] * 6)
+    rng = np.random.default_rng(nx * ny + nz)
+    locs = [sa.FACE_X, sa.FACE_Y, sa.FACE_Z]
+    rhs = [rng.standard_normal(sa.storage_shape(nx, ny, nz, loc)) for loc in locs]
+    out = [zeros(g, loc) for loc in locs]
+    its, rel = C.c_int(), C.c_double()
+    ctx.run("of_solve_predictor_cg", C.byref(g), C.byref(b), C.c_double(alpha), *[sa.ptr(dev(a)) for a in rhs],
+            *map(sa.ptr, out), C.c_double(1e-14), 400, C.byref(its), C.byref(rel))
+    assert 0 < its.value <= 400 and rel.value <= 1e-14
+    ref, _ = ofref.predictor_adi(og, ofref.bcset([ofref.PERIODIC] * 6), alpha, [a.copy() for a in rhs])
+    for k, (a, r) in enumerate(zip(out, ref)):
+        err = np.max(np.abs(host(a) - r)) / np.max(np.abs(r))
+        assert err <= 1e-11, (k, err)
+
+
+def test_predictor_cg_rejects_walls(ctx):
+    g = sa.grid(8, 8, 8, 0.1, 0.1, 0.1)
+    b = sa.bc_set([sa.NO_SLIP] * 6)
+    f = [zeros(g, loc) for loc in (sa.FACE_X, sa.FACE_Y, sa.FACE_Z)]
+    with pytest.raises(sa.InvalidArgument, match="periodic"):
+        ctx.run("of_solve_predictor_cg", C.byref(g), C.byref(b), C.c_double(0.1), *map(sa.ptr, f), *map(sa.ptr, f),
+                C.c_double(1e-12), 80, None, None)
+
+
+# A Taylor-Green run with the CG predictor at the Metal backend's FP64 tolerance follows the
+# oracle's ADI run to the parity tolerance, with the same pressure iteration counts.
+@pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
+def test_taylor_green_cg_predictor_vs_oracle(ctx, precision):
+    n, nz, viscosity, steps = 32, 32, 0.01, 5
+    tol = 1e-12
+    c = tg_case(n, nz, viscosity, 0.5, steps, tol)
+    s = ofref.tg_init(c, convention=1)
+    dt, nsteps = sa.taylor_green_dt(n, n, nz, 0.5, c.final_time)
+    st = sa.Stepper(ctx, sa.TAYLOR_GREEN, n, n, nz, viscosity, dt, 400, tol, precision)
+    st.set_predictor(sa.PREDICTOR_CG, 1e-12, 80)
+    st.upload(s.vel[0], s.vel[1], s.vel[2], s.p)
+    iters = [st.run(1).pressure_iterations for _ in range(steps)]
+    vel, p, _, _, _ = st.download()
+    ref_iters = []
+    for _ in range(steps):
+        ofref.tg_steps(c, s, 1)
+        ref_iters.append(s.metrics.pressure_iterations)
+    if precision == sa.FP64:
+        assert iters == ref_iters
+    for a, b, k in zip(vel + [p], s.vel + [s.p], "uvwp"):
+        if not np.any(b[1:-1, 1:-1, 1:-1]):
+            continue
+        assert rel_l2_active(a, b, mean_removed=(k == "p")) <= 1e-10, k
