@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--adjoint-tl", type=int, default=0,
                     help="adjoint-versus-tangent-linear checks along this many seeded directions")
     ap.add_argument("--adjoint-only", action="store_true", help="skip the forward benchmark")
+    ap.add_argument("--predictor-cg-steps", type=int, default=20,
+                    help="FP64 steps timed with the CG Helmholtz predictor beside the line solves (0: skip)")
     return ap.parse_args()
 
 
@@ -266,10 +268,12 @@ def cpu_baseline(n_target, n, nu, cfl):
             "host_cpu": host_cpu(), "host_logical_cpus": os.cpu_count()}
 
 
-def run_precision(sa, ctx, n, nu, dt, precision, warmup, steps, prof_kernel):
+def run_precision(sa, ctx, n, nu, dt, precision, warmup, steps, prof_kernel, predictor_cg=False):
     import torch
 
     st = sa.Stepper(ctx, sa.TAYLOR_GREEN, n, n, n, nu, dt, 400, 1e-10, precision)
+    if predictor_cg:  # the Metal backend's predictor at its FP64 tolerance
+        st.set_predictor(sa.PREDICTOR_CG, 1e-12, 80)
     st.initialize(convention=1)
     div_inf = 0.0
     for _ in range(warmup):
@@ -399,6 +403,17 @@ def main():
             out["mixed_speedup_vs_fp64"] = value / out["fp64"]["steps_per_s"]
             out["mixed_vs_fp64_rel_l2"] = {k: rel_l2(x, y) for k, x, y in
                                            zip("uvwp", res[sa.MIXED]["fields"], res[sa.FP64]["fields"])}
+    if a.predictor_cg_steps > 0 and sa.FP64 in res:
+        # the CG Helmholtz predictor in place of the line solves, FP64, on the same first steps
+        k = min(a.predictor_cg_steps, a.steps)
+        rc = run_precision(sa, ctx, n, nu, dt, sa.FP64, a.warmup, k, "", predictor_cg=True)
+        rf = run_precision(sa, ctx, n, nu, dt, sa.FP64, a.warmup, k, "")
+        out["fp64_cg_predictor"] = {
+            "steps": k, "steps_per_s": k / rc["seconds"], "ms_per_step": 1e3 * rc["seconds"] / k,
+            "adi_ms_per_step": 1e3 * rf["seconds"] / k, "max_div_linf": rc["div_linf"],
+            "vs_adi_rel_l2": {key: rel_l2(x, y) for key, x, y in zip("uvwp", rc["fields"], rf["fields"])}}
+        print(f"# fp64 with the CG predictor: {k / rc['seconds']:.2f} steps/s ({1e3 * rc['seconds'] / k:.2f} ms/step "
+              f"vs {1e3 * rf['seconds'] / k:.2f} with the line solves)", flush=True)
     if a.adjoint != "none":
         for k in list(res):
             res[k].pop("fields", None)
