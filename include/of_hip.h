@@ -235,23 +235,31 @@ int of_apply_poisson_operator(of_context* ctx, const of_grid* grid, const of_pre
                               double* out);
 
 /* ---------------------------------------------------------------- device-resident stepper */
-/* solver/taylor_green.hpp:18-57 TaylorGreenConfig / TaylorGreenState and
- * solver/lid_driven_cavity.hpp:12-25,88-97 (case = 1). The stepper owns the device state
- * (u, v, w, p_total, A_prev x3, has_previous_advection, metrics) and runs
- * run_taylor_green_steps (solver/taylor_green.cpp:540-634) /
- * run_lid_driven_cavity_steps (solver/lid_driven_cavity.cpp:524-632) on the device. */
-enum { OF_CASE_TAYLOR_GREEN = 0, OF_CASE_LID_DRIVEN_CAVITY = 1 };
+/* solver/taylor_green.hpp:18-57 TaylorGreenConfig / TaylorGreenState,
+ * solver/lid_driven_cavity.hpp:12-25,88-97 (case 1) and solver/channel_flow.hpp:10-28 (cases 2
+ * and 3). The stepper owns the device state (u, v, w, p_total, A_prev x3,
+ * has_previous_advection, metrics) and runs run_taylor_green_steps
+ * (solver/taylor_green.cpp:540-634), run_lid_driven_cavity_steps
+ * (solver/lid_driven_cavity.cpp:524-632) or the step loop of run_channel_flow
+ * (solver/channel_flow.cpp:468-523) on the device. */
+enum {
+  OF_CASE_TAYLOR_GREEN = 0,
+  OF_CASE_LID_DRIVEN_CAVITY = 1,
+  OF_CASE_CHANNEL_COUETTE = 2,    /* periodic x, moving upper wall (ChannelFlowCase::couette) */
+  OF_CASE_CHANNEL_POISEUILLE = 3  /* fixed-pressure x ends, walls in y (ChannelFlowCase::poiseuille) */
+};
 
 typedef struct of_stepper_config {
   int flow_case;
   int nx, ny, nz;
-  double viscosity;           /* TG: viscosity; cavity: lid_velocity / reynolds */
-  double dt;                  /* from of_taylor_green_dt / of_lid_driven_cavity_dt */
+  double viscosity;           /* TG, channel: viscosity; cavity: lid_velocity / reynolds */
+  double dt;                  /* from of_taylor_green_dt / of_lid_driven_cavity_dt / of_channel_flow_dt */
   int scheme;                 /* advection scheme, 0 = tvd */
   int poisson_max_iterations;
   double poisson_tolerance;
   int precision;              /* OF_PRECISION_* */
-  double lid_velocity;        /* cavity only */
+  double lid_velocity;        /* cavity: lid speed; Couette: top_velocity */
+  double pressure_drop;       /* Poiseuille: pressure at x_min (0 at x_max) */
 } of_stepper_config;
 
 /* solver/taylor_green.hpp:32-42 TaylorGreenStepMetrics (+ L-infinity divergence) */
@@ -275,6 +283,10 @@ typedef struct of_stepper of_stepper;
 int of_taylor_green_dt(int nx, int ny, int nz, double cfl_limit, double final_time, double* dt, int* steps);
 /* solver/lid_driven_cavity.cpp:95-98,386-388 lid_driven_cavity_dt */
 int of_lid_driven_cavity_dt(int nx, int ny, double lid_velocity, double cfl_limit, double* dt);
+/* solver/channel_flow.cpp:142-158 fixed_dt: cfl_limit / (max_u / dx) with max_u = top_velocity
+ * (Couette) or pressure_drop / (8 viscosity) (Poiseuille); dx = 1 / nx. */
+int of_channel_flow_dt(int flow_case, int nx, double viscosity, double top_velocity, double pressure_drop,
+                       double cfl_limit, double* dt);
 
 int of_stepper_create(of_context* ctx, const of_stepper_config* cfg, of_stepper** out);
 int of_stepper_destroy(of_stepper* s);

@@ -1437,6 +1437,46 @@ static double cavity_dt(const CavityConfig& c) {  // solver/lid_driven_cavity.cp
   return c.cfl_limit / (c.lid_velocity / g.dx + c.lid_velocity / g.dy);
 }
 
+// solver/channel_flow.hpp:10-28 ChannelFlowConfig (advection: TVD, the default options)
+struct ChannelConfig {
+  int poiseuille;  // ChannelFlowCase: 0 couette, 1 poiseuille
+  int nx, ny;
+  double viscosity, top_velocity, pressure_drop, cfl_limit;
+  int steps, max_iterations;
+  double tolerance;
+};
+static Grid channel_grid(const ChannelConfig& c) {  // solver/channel_flow.cpp:435-442
+  return Grid{c.nx, c.ny, 1, 1.0 / (double)c.nx, 1.0 / (double)c.ny, 1.0};
+}
+static BCSet channel_bcs(const ChannelConfig& c) {  // solver/channel_flow.cpp:408-430
+  BCSet b{};
+  for(int f = 0; f < 6; ++f) b.f[f] = BC{kNoSlip, {0, 0, 0}, 0, 0};
+  b.f[4].type = kSymmetry;
+  b.f[5].type = kSymmetry;
+  if(!c.poiseuille) {
+    b.f[0].type = kPeriodic;
+    b.f[1].type = kPeriodic;
+    b.f[3].type = kPrescribed;
+    b.f[3].velocity[0] = c.top_velocity;
+  } else {
+    b.f[0].type = kFixedPressure;
+    b.f[0].pressure = c.pressure_drop;
+    b.f[1].type = kFixedPressure;
+    b.f[1].pressure = 0.0;
+  }
+  return b;
+}
+static double channel_dt(const ChannelConfig& c) {  // solver/channel_flow.cpp:142-158
+  const double max_u = c.poiseuille ? c.pressure_drop / (8.0 * c.viscosity) : c.top_velocity;
+  return c.cfl_limit / (max_u / channel_grid(c).dx);
+}
+static double channel_u(const ChannelConfig& c, double y) {  // solver/channel_flow.cpp:160-169
+  return c.poiseuille ? 0.5 * c.pressure_drop * y * (1.0 - y) / c.viscosity : c.top_velocity * y;
+}
+static double channel_p(const ChannelConfig& c, double x) {  // solver/channel_flow.cpp:171-180
+  return c.poiseuille ? c.pressure_drop * (1.0 - x) : 0.0;
+}
+
 }  // namespace ofref
 
 // =====================================================================================
@@ -1774,6 +1814,65 @@ int ofref_cavity_steps(const CavityConfig* c, int steps, int stop_on_steady, dou
                       "lid-driven cavity", false);
       if(stop_on_steady && m->step >= c->min_steps && m->max_velocity_change <= c->steady_tolerance) break;
     }
+  });
+}
+
+// solver/channel_flow.cpp:182-224 initialize_steady_channel_state and the set-up of
+// run_channel_flow (:465-466): analytic profile, velocity BCs, total-pressure BCs from nu lap u.
+int ofref_channel_init(const ChannelConfig* c, double* u, double* v, double* w, double* p, double* dt) {
+  return guarded([&] {
+    const Grid g = channel_grid(*c);
+    const BCSet b = channel_bcs(*c);
+    Vel vel(g, u, v, w);
+    Field pf(make_layout(g, kCell), p);
+    vel.fill(0.0);
+    pf.fill(0.0);
+    for_range(vel.x.L.active(), [&](int i, int j, int k) { vel.x(i, j, k) = channel_u(*c, coord(vel.x.L, 1, j)); });
+    for_range(pf.L.active(), [&](int i, int j, int k) { pf(i, j, k) = channel_p(*c, coord(pf.L, 0, i)); });
+    apply_velocity_bc(b, vel);
+    Vel dif(g);
+    diffusion(vel, c->viscosity, dif);
+    apply_total_pressure_bc(b, dif, pf);
+    *dt = channel_dt(*c);
+  });
+}
+
+// The step loop of run_channel_flow (solver/channel_flow.cpp:468-523); its predictor
+// right-hand side is the cavity's (two-dimensional factorisation correction).
+int ofref_channel_steps(const ChannelConfig* c, int steps, double* u, double* v, double* w, double* p, double* au,
+                        double* av, double* aw, int* has_prev, StepMetrics* m) {
+  return guarded([&] {
+    const Grid g = channel_grid(*c);
+    const BCSet b = channel_bcs(*c);
+    const double dt = channel_dt(*c);
+    Vel vel(g, u, v, w), prev(g, au, av, aw);
+    Field pf(make_layout(g, kCell), p);
+    for(int s = 0; s < steps; ++s)
+      fractional_step(b, c->viscosity, dt, 0, true, c->max_iterations, c->tolerance, vel, prev, pf, *has_prev, *m,
+                      "channel flow", false);
+  });
+}
+
+// solver/channel_flow.cpp:258-301: x-averaged u per active row (first k plane) and its relative
+// L2 distance to the analytic profile; profile receives ny values.
+int ofref_channel_profile(const ChannelConfig* c, const double* u, double* profile, double* relative_l2) {
+  return guarded([&] {
+    const Grid g = channel_grid(*c);
+    Field fu(make_layout(g, kFaceX), const_cast<double*>(u));
+    const auto r = fu.L.active();
+    double num = 0.0, den = 0.0;
+    int n = 0;
+    for(int j = r.jb; j < r.je; ++j, ++n) {
+      double sum = 0.0;
+      for(int i = r.ib; i < r.ie; ++i) sum += fu(i, j, r.kb);
+      const double val = sum / (double)(r.ie - r.ib);
+      profile[n] = val;
+      const double ref = channel_u(*c, coord(fu.L, 1, j));
+      const double d = val - ref;
+      num += d * d;
+      den += ref * ref;
+    }
+    *relative_l2 = den == 0.0 ? std::sqrt(num) : std::sqrt(num / den);
   });
 }
 

@@ -77,6 +77,12 @@ class TGConfig(C.Structure):
                 ("max_iterations", C.c_int), ("tolerance", C.c_double)]
 
 
+class ChannelConfig(C.Structure):
+    _fields_ = [("poiseuille", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("viscosity", C.c_double),
+                ("top_velocity", C.c_double), ("pressure_drop", C.c_double), ("cfl_limit", C.c_double),
+                ("steps", C.c_int), ("max_iterations", C.c_int), ("tolerance", C.c_double)]
+
+
 class CavityConfig(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("reynolds", C.c_double),
                 ("lid_velocity", C.c_double), ("cfl_limit", C.c_double),
@@ -395,6 +401,38 @@ def cavity_steps(c: CavityConfig, s: State, steps: int, stop_on_steady=False):
     _call("ofref_cavity_steps", C.byref(c), C.c_int(steps), C.c_int(int(stop_on_steady)),
           *map(_p, s.vel), _p(s.p), *map(_p, s.adv), C.byref(s.has_prev), C.byref(s.metrics))
     return s
+
+
+def channel_config(case="couette", nx=128, ny=128, viscosity=0.1, top_velocity=1.0, pressure_drop=0.8,
+                   cfl_limit=0.5, steps=8, max_iterations=200, tolerance=1e-10) -> ChannelConfig:
+    return ChannelConfig(int(case == "poiseuille"), nx, ny, viscosity, top_velocity, pressure_drop, cfl_limit,
+                         steps, max_iterations, tolerance)
+
+
+def channel_grid(c: ChannelConfig) -> Grid:
+    return Grid(c.nx, c.ny, 1, 1.0 / c.nx, 1.0 / c.ny, 1.0)
+
+
+def channel_init(c: ChannelConfig) -> State:
+    s = State(channel_grid(c))
+    dt = C.c_double(0)
+    _call("ofref_channel_init", C.byref(c), *map(_p, s.vel), _p(s.p), C.byref(dt))
+    s.metrics.dt = dt.value
+    return s
+
+
+def channel_steps(c: ChannelConfig, s: State, steps: int):
+    _call("ofref_channel_steps", C.byref(c), C.c_int(steps), *map(_p, s.vel), _p(s.p), *map(_p, s.adv),
+          C.byref(s.has_prev), C.byref(s.metrics))
+    return s
+
+
+def channel_profile(c: ChannelConfig, u):
+    """x-averaged streamwise profile (ny values) and its relative L2 error to the analytic one."""
+    prof = np.zeros(c.ny)
+    err = C.c_double(0)
+    _call("ofref_channel_profile", C.byref(c), _p(np.ascontiguousarray(u)), _p(prof), C.byref(err))
+    return prof, err.value
 
 
 def mms(n):

@@ -14,6 +14,7 @@
 
 #include "linsolve/poisson_solver.hpp"
 #include "operators/discrete_operators.hpp"
+#include "solver/channel_flow.hpp"
 #include "solver/lid_driven_cavity.hpp"
 #include "solver/momentum_terms.hpp"
 #include "solver/operator_verification.hpp"
@@ -245,6 +246,35 @@ int run(const std::string& cmd, const Args& a) {
     w.put(s.velocity);
     w.put(s.pressure_total);
     w.put(s.advection_previous);
+    return 0;
+  }
+  if(cmd == "channel") {  // out: streamwise profile (ny values)
+    solver::ChannelFlowConfig c;
+    c.case_kind = need(a, "case") == "poiseuille" ? solver::ChannelFlowCase::poiseuille
+                                                  : solver::ChannelFlowCase::couette;
+    c.nx = inum(a, "nx");
+    c.ny = inum(a, "ny");
+    c.viscosity = num(a, "viscosity");
+    c.top_velocity = num(a, "top");
+    c.pressure_drop = num(a, "drop");
+    c.cfl_limit = num(a, "cfl");
+    c.steps = inum(a, "steps");
+    c.poisson_max_iterations = inum(a, "maxit");
+    c.poisson_tolerance = num(a, "tol");
+    c.validate_profile = true;
+    solver::ChannelFlowResult r;
+    try {
+      r = solver::run_channel_flow(c);
+    } catch(const std::exception& e) {
+      w.kv("error", e.what());
+      return 0;
+    }
+    put_metrics(w, r.final_step);
+    w.kv("reference_dataset", r.validation.reference_dataset);
+    w.kv("relative_l2_error", r.validation.relative_l2_error);
+    w.kv("pass", r.validation.pass ? 1 : 0);
+    w.out.write(reinterpret_cast<const char*>(r.streamwise_profile.value.data()),
+                (std::streamsize)(r.streamwise_profile.value.size() * sizeof(double)));
     return 0;
   }
   if(cmd == "mms") {

@@ -24,7 +24,7 @@ NO_SLIP, PRESCRIBED, SYMMETRY, FIXED_PRESSURE, PERIODIC = 0, 1, 2, 3, 4
 P_NEUMANN, P_DIRICHLET, P_PERIODIC = 0, 1, 2
 PREDICTOR_ADI, PREDICTOR_CG = 0, 1
 FP64, MIXED = 0, 1
-TAYLOR_GREEN, LID_DRIVEN_CAVITY = 0, 1
+TAYLOR_GREEN, LID_DRIVEN_CAVITY, CHANNEL_COUETTE, CHANNEL_POISEUILLE = 0, 1, 2, 3
 
 
 class Grid(C.Structure):
@@ -78,7 +78,7 @@ class StepperConfig(C.Structure):
     _fields_ = [("flow_case", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("viscosity", C.c_double), ("dt", C.c_double), ("scheme", C.c_int),
                 ("poisson_max_iterations", C.c_int), ("poisson_tolerance", C.c_double),
-                ("precision", C.c_int), ("lid_velocity", C.c_double)]
+                ("precision", C.c_int), ("lid_velocity", C.c_double), ("pressure_drop", C.c_double)]
 
 
 class StepMetrics(C.Structure):
@@ -110,7 +110,8 @@ EXPORTS = [
     "of_assemble_predictor_rhs", "of_solve_predictor_adi", "of_solve_predictor_cg", "of_build_pressure_rhs",
     "of_correct_velocity",
     "of_project_velocity", "of_solve_pressure_poisson", "of_build_poisson_rhs_from_pressure",
-    "of_apply_poisson_operator", "of_taylor_green_dt", "of_lid_driven_cavity_dt", "of_stepper_create",
+    "of_apply_poisson_operator", "of_taylor_green_dt", "of_lid_driven_cavity_dt", "of_channel_flow_dt",
+    "of_stepper_create",
     "of_stepper_destroy", "of_stepper_initialize", "of_stepper_upload", "of_stepper_download",
     "of_stepper_device_fields", "of_stepper_run", "of_stepper_run_batched", "of_stepper_kinetic_energy",
     "of_stepper_set_forcing", "of_stepper_set_predictor",
@@ -231,6 +232,13 @@ def lid_driven_cavity_dt(nx, ny, lid_velocity, cfl_limit):
     return dt.value
 
 
+def channel_flow_dt(flow_case, nx, viscosity, top_velocity, pressure_drop, cfl_limit):
+    dt = C.c_double()
+    call("of_channel_flow_dt", flow_case, nx, C.c_double(viscosity), C.c_double(top_velocity),
+         C.c_double(pressure_drop), C.c_double(cfl_limit), C.byref(dt))
+    return dt.value
+
+
 # ------------------------------------------------------------------ context
 class Context:
     """Device context: reduction buffers and the stream all launches are queued on."""
@@ -280,13 +288,16 @@ class Context:
 
 
 class Stepper:
-    """Device-resident Taylor-Green / lid-driven-cavity state and step loop."""
+    """Device-resident Taylor-Green / lid-driven-cavity / channel state and step loop.
+
+    lid_velocity is the cavity's lid speed and the Couette channel's top-wall speed;
+    pressure_drop is the Poiseuille channel's inlet pressure."""
 
     def __init__(self, ctx: Context, flow_case, nx, ny, nz, viscosity, dt, poisson_max_iterations=200,
-                 poisson_tolerance=1e-10, precision=FP64, scheme=0, lid_velocity=1.0):
+                 poisson_tolerance=1e-10, precision=FP64, scheme=0, lid_velocity=1.0, pressure_drop=0.0):
         self.ctx = ctx
         cfg = StepperConfig(flow_case, nx, ny, nz, viscosity, dt, scheme, poisson_max_iterations,
-                            poisson_tolerance, precision, lid_velocity)
+                            poisson_tolerance, precision, lid_velocity, pressure_drop)
         self.cfg = cfg
         h = C.c_void_p()
         call("of_stepper_create", ctx.h, C.byref(cfg), C.byref(h))
@@ -372,7 +383,7 @@ class Adjoint:
                  poisson_tolerance=1e-10, precision=FP64, scheme=0):
         self.ctx = ctx
         self.cfg = StepperConfig(TAYLOR_GREEN, nx, ny, nz, viscosity, dt, scheme, poisson_max_iterations,
-                                 poisson_tolerance, precision, 0.0)
+                                 poisson_tolerance, precision, 0.0, 0.0)
         h = C.c_void_p()
         call("of_adjoint_create", ctx.h, C.byref(self.cfg), int(checkpoint_every), C.byref(h))
         self.h = h

@@ -1,7 +1,9 @@
 // Host-only checks of the C++ interface (no device work): layouts, descriptors, the .cfg
-// surface and its error messages, and the cavity envelope check on a synthetic profile.
+// surface and its error messages, the cavity envelope check and the channel profile check on
+// synthetic profiles.
 // Prints one line per check and exits non-zero on the first failure.
 #include "core/fields.hpp"
+#include "solver/channel_flow.hpp"
 #include "solver/lid_driven_cavity.hpp"
 #include "solver/taylor_green.hpp"
 
@@ -127,6 +129,56 @@ int main() {
   expect(std::abs(s0 - want) < 1e-12, "cubic interpolation is exact on a quadratic profile");
   expect(std::abs(v.points[2].relative_error - (0.2 - 0.17960) / 0.17960) < 1e-15, "miss against the envelope");
   expect(!v.pass, "a profile outside the envelope fails");
+  // channel .cfg surface, boundary sets and profile check (solver/channel_flow.cpp:303-430)
+  const ChannelFlowConfig ch = load_channel_flow_config(write_cfg(
+      "chan.cfg", "case = couette\nnx = 32\nny = 32\nsteps = 4\nvalidate_profile = true\n"));
+  expect(ch.case_kind == ChannelFlowCase::couette && ch.nx == 32 && ch.ny == 32 && ch.steps == 4 &&
+             ch.validate_profile,
+         "channel cfg keys");
+  expect(describe(ch).find("case=couette, nx=32, ny=32, viscosity=0.1, top_velocity=1") == 0, "channel describe()");
+  const BoundaryConditionSet cou = make_channel_flow_boundary_conditions(ch);
+  expect(cou[BoundaryFace::x_min].type == PhysicalBoundaryType::periodic &&
+             cou[BoundaryFace::x_max].type == PhysicalBoundaryType::periodic &&
+             cou[BoundaryFace::y_min].type == PhysicalBoundaryType::no_slip_wall &&
+             cou[BoundaryFace::y_max].type == PhysicalBoundaryType::prescribed_velocity &&
+             cou[BoundaryFace::y_max].velocity[0] == ch.top_velocity &&
+             cou[BoundaryFace::z_min].type == PhysicalBoundaryType::symmetry,
+         "Couette boundaries: periodic x, moving upper wall, symmetry z");
+  ChannelFlowConfig pz = ch;
+  pz.case_kind = ChannelFlowCase::poiseuille;
+  const BoundaryConditionSet poi = make_channel_flow_boundary_conditions(pz);
+  expect(poi[BoundaryFace::x_min].type == PhysicalBoundaryType::fixed_pressure &&
+             poi[BoundaryFace::x_min].pressure == pz.pressure_drop &&
+             poi[BoundaryFace::x_max].type == PhysicalBoundaryType::fixed_pressure &&
+             poi[BoundaryFace::x_max].pressure == 0.0 &&
+             poi[BoundaryFace::y_max].type == PhysicalBoundaryType::no_slip_wall,
+         "Poiseuille boundaries: fixed pressure drop along x, no-slip walls");
+  expect(to_string(ChannelFlowCase::poiseuille) == "poiseuille", "channel case names");
+  expect_throw<std::runtime_error>(
+      [] { (void)load_channel_flow_config(write_cfg("chan_bad.cfg", "case = duct\n")); },
+      "unsupported channel flow case: duct", "unknown channel case");
+  expect_throw<std::runtime_error>(
+      [] { (void)load_channel_flow_config(write_cfg("chan_bad2.cfg", "reynolds = 3\n")); },
+      "unsupported channel flow config key: reynolds", "unknown channel key");
+  expect_throw<std::invalid_argument>(
+      [] { (void)load_channel_flow_config(write_cfg("chan_bad3.cfg", "case = poiseuille\npressure_drop = 0\n")); },
+      "Poiseuille config expects pressure_drop > 0", "Poiseuille validation");
+  expect_throw<std::invalid_argument>(
+      [] { (void)load_channel_flow_config(write_cfg("chan_bad4.cfg", "steps = 0\n")); },
+      "channel flow config expects steps > 0", "channel step validation");
+  ChannelFlowResult cr;
+  cr.config = pz;
+  for(int j = 0; j < 32; ++j) {
+    const double y = (j + 0.5) / 32.0;
+    cr.streamwise_profile.coordinate.push_back(y);
+    cr.streamwise_profile.value.push_back(0.5 * pz.pressure_drop * y * (1.0 - y) / pz.viscosity);
+  }
+  const ChannelFlowValidation cv = validate_channel_flow_profile(cr);
+  expect(cv.reference_dataset == "analytic_poiseuille_profile" && cv.relative_l2_error == 0.0 && cv.pass,
+         "the analytic Poiseuille profile passes with zero error");
+  for(double& val : cr.streamwise_profile.value) val *= 1.01;
+  const ChannelFlowValidation cv2 = validate_channel_flow_profile(cr);
+  expect(std::abs(cv2.relative_l2_error - 0.01) < 1e-12 && !cv2.pass, "a 1% profile error fails the 0.5% check");
   std::cout << (failures == 0 ? "all host checks passed" : "host checks failed") << '\n';
   return failures == 0 ? 0 : 1;
 }

@@ -553,6 +553,19 @@ int of_lid_driven_cavity_dt(int nx, int ny, double lid, double cfl, double* dt) 
   });
 }
 
+int of_channel_flow_dt(int flow_case, int nx, double nu, double top, double drop, double cfl, double* dt) {
+  return guarded([&] {  // solver/channel_flow.cpp:142-158
+    need(dt, "dt");
+    if(nx <= 0) throw InvalidArgument("channel flow config expects positive nx and ny");
+    double max_u;
+    if(flow_case == OF_CASE_CHANNEL_COUETTE) max_u = top;
+    else if(flow_case == OF_CASE_CHANNEL_POISEUILLE) max_u = drop / (8.0 * nu);
+    else throw InvalidArgument("unsupported channel flow case");
+    const double dx = 1.0 / (double)nx;
+    *dt = cfl / (max_u / dx);
+  });
+}
+
 int of_stepper_create(of_context* c, const of_stepper_config* cfg, of_stepper** out) {
   return guarded([&] {
     need(cfg, "config");

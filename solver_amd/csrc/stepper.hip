@@ -1,6 +1,8 @@
 // Device-resident fractional-step driver: the step loop of run_taylor_green_steps
-// (solver/taylor_green.cpp:540-634) and advance_lid_driven_cavity_impl
-// (solver/lid_driven_cavity.cpp:524-624) with every field kept in HBM.
+// (solver/taylor_green.cpp:540-634), advance_lid_driven_cavity_impl
+// (solver/lid_driven_cavity.cpp:524-624) and run_channel_flow (solver/channel_flow.cpp:432-523)
+// with every field kept in HBM. The cavity and the channels share the two-dimensional predictor
+// right-hand side (solver/lid_driven_cavity.cpp:488-518, solver/channel_flow.cpp:226-256).
 
 #include "stepper.hpp"
 
@@ -103,6 +105,27 @@ __global__ void k_tg_init(double* f, Lay L, int loc, of_grid g, int three_d, int
   }
 }
 
+// Steady channel profile (solver/channel_flow.cpp:160-224) on the active range: u(y) on the x
+// faces (Couette U y, Poiseuille G y (1 - y) / (2 nu)), total pressure G (1 - x) for Poiseuille;
+// v, w and the Couette pressure stay zero.
+__global__ void k_channel_init(double* f, Lay L, int loc, of_grid g, int poiseuille, double top, double drop,
+                               double nu) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if(i > L.ax || j > L.ay) return;
+  for(int k = 1; k <= L.az; ++k) {
+    double val;
+    if(loc == OF_FACE_X) {
+      const double y = ((double)(j - 1) + 0.5) * g.dy;  // core/field_layout.hpp:366-377
+      val = poiseuille ? 0.5 * drop * y * (1.0 - y) / nu : top * y;
+    } else {
+      const double x = ((double)(i - 1) + 0.5) * g.dx;
+      val = poiseuille ? drop * (1.0 - x) : 0.0;
+    }
+    f[L.at(i, j, k)] = val;
+  }
+}
+
 // kinetic energy sum (solver/taylor_green.cpp:269-288)
 __global__ void k_kinetic(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
                           Lay Lu, Lay Lv, Lay Lw, of_grid g, double* partials, int kz) {
@@ -129,11 +152,23 @@ of_bc_set stepper_bcs(const of_stepper_config& c) {
   if(c.flow_case == OF_CASE_TAYLOR_GREEN) {  // solver/taylor_green.cpp:510-517
     for(int f = 0; f < 6; ++f) b.faces[f].type = OF_PERIODIC;
     if(c.nz <= 1) b.faces[4].type = b.faces[5].type = OF_SYMMETRY;
-  } else {  // solver/lid_driven_cavity.cpp:362-370
+  } else if(c.flow_case == OF_CASE_LID_DRIVEN_CAVITY) {  // solver/lid_driven_cavity.cpp:362-370
     for(int f = 0; f < 6; ++f) b.faces[f].type = OF_NO_SLIP_WALL;
     b.faces[3].type = OF_PRESCRIBED_VELOCITY;
     b.faces[3].velocity[0] = c.lid_velocity;
     b.faces[4].type = b.faces[5].type = OF_SYMMETRY;
+  } else {  // solver/channel_flow.cpp:408-430
+    for(int f = 0; f < 6; ++f) b.faces[f].type = OF_NO_SLIP_WALL;
+    b.faces[4].type = b.faces[5].type = OF_SYMMETRY;
+    if(c.flow_case == OF_CASE_CHANNEL_COUETTE) {
+      b.faces[0].type = b.faces[1].type = OF_PERIODIC;
+      b.faces[3].type = OF_PRESCRIBED_VELOCITY;
+      b.faces[3].velocity[0] = c.lid_velocity;
+    } else {
+      b.faces[0].type = b.faces[1].type = OF_FIXED_PRESSURE;
+      b.faces[0].pressure = c.pressure_drop;
+      b.faces[1].pressure = 0.0;
+    }
   }
   return b;
 }
@@ -167,6 +202,14 @@ Stepper::Stepper(Context& ctx, const of_stepper_config& cfg) : c(ctx), cfg(cfg) 
     g = of_grid{cfg.nx, cfg.ny, cfg.nz, L / (double)cfg.nx, L / (double)cfg.ny, cfg.nz > 1 ? L / (double)cfg.nz : 1.0};
   } else if(cfg.flow_case == OF_CASE_LID_DRIVEN_CAVITY) {  // solver/lid_driven_cavity.cpp:372-380
     if(cfg.nz != 1) throw InvalidArgument("the lid-driven cavity is two-dimensional (nz = 1)");
+    g = of_grid{cfg.nx, cfg.ny, 1, 1.0 / (double)cfg.nx, 1.0 / (double)cfg.ny, 1.0};
+  } else if(cfg.flow_case == OF_CASE_CHANNEL_COUETTE ||
+            cfg.flow_case == OF_CASE_CHANNEL_POISEUILLE) {  // solver/channel_flow.cpp:435-449
+    if(cfg.nz != 1) throw InvalidArgument("channel flow is two-dimensional (nz = 1)");
+    if(cfg.flow_case == OF_CASE_CHANNEL_COUETTE && !(cfg.lid_velocity > 0.0))
+      throw InvalidArgument("Couette config expects top_velocity > 0");
+    if(cfg.flow_case == OF_CASE_CHANNEL_POISEUILLE && !(cfg.pressure_drop > 0.0))
+      throw InvalidArgument("Poiseuille config expects pressure_drop > 0");
     g = of_grid{cfg.nx, cfg.ny, 1, 1.0 / (double)cfg.nx, 1.0 / (double)cfg.ny, 1.0};
   } else {
     throw InvalidArgument("unknown flow case");
@@ -260,6 +303,16 @@ void Stepper::initialize(int convention) {
       const dim3 grid((unsigned)((L.ax + 63) / 64), (unsigned)((L.ay + 3) / 4), 1);
       hipLaunchKernelGGL(k_tg_init, grid, block, 0, c.stream, f, L, loc, g, three_d, convention);
     }
+  } else if(cfg.flow_case != OF_CASE_LID_DRIVEN_CAVITY) {
+    const int pois = cfg.flow_case == OF_CASE_CHANNEL_POISEUILLE;
+    for(int loc : {OF_FACE_X, OF_CELL}) {
+      const Lay L = make_lay(g, loc);
+      double* f = loc == OF_CELL ? p_total : vel[0];
+      const dim3 block(64, 4, 1);
+      const dim3 grid((unsigned)((L.ax + 63) / 64), (unsigned)((L.ay + 3) / 4), 1);
+      hipLaunchKernelGGL(k_channel_init, grid, block, 0, c.stream, f, L, loc, g, pois, cfg.lid_velocity,
+                         cfg.pressure_drop, cfg.viscosity);
+    }
   }
   launch_velocity_bc(c, g, bcs, vel[0], vel[1], vel[2]);
   total_pressure_bc(vel);
@@ -272,7 +325,9 @@ void Stepper::initialize(int convention) {
 
 void Stepper::step() {
   const double dt = cfg.dt, nu = cfg.viscosity;
-  const char* name = cfg.flow_case == OF_CASE_TAYLOR_GREEN ? "Taylor-Green" : "lid-driven cavity";
+  const char* name = cfg.flow_case == OF_CASE_TAYLOR_GREEN        ? "Taylor-Green"
+                     : cfg.flow_case == OF_CASE_LID_DRIVEN_CAVITY ? "lid-driven cavity"
+                                                                  : "channel flow";
   // current velocity with BC ghosts (idempotent on a state that already carries them)
   launch_velocity_bc(c, g, bcs, vel[0], vel[1], vel[2]);
   // predictor RHS; A overwrites A_prev in place after it has been read

@@ -249,6 +249,23 @@ def main():
         _, meta = refrun.run("mms", (), (), n=n)
         save(f"mms_{n}", dict(n=n), {}, {}, meta)
 
+    channel_fixtures()
+
+
+def channel_fixtures():
+    """run_channel_flow (solver/channel_flow.cpp:432-541) on the reference's smoke and benchmark
+    configurations (benchmarks/channel_*_smoke.cfg, channel_*_128.cfg): final metrics, the
+    x-averaged streamwise profile and the validation figures; plus the non-converged rejection."""
+    base = dict(viscosity=0.1, top=1.0, drop=0.8, cfl=0.5, maxit=200, tol=1e-10)
+    for case in ("couette", "poiseuille"):
+        for n, steps in ((32, 4), (128, 8)):
+            cp = dict(base, case=case, nx=n, ny=n, steps=steps)
+            outs, meta = refrun.run("channel", (), [(n,)], **cp)
+            save(f"channel_{case}_{n}", cp, {}, dict(profile=outs[0]), meta)
+    cp = dict(base, case="poiseuille", nx=32, ny=32, steps=4, maxit=1, tol=1e-30)
+    _, meta = refrun.run("channel", (), (), **cp)
+    save("channel_nonconverged", cp, {}, {}, meta)
+
 
 if __name__ == "__main__":
     main()

@@ -237,6 +237,42 @@ def test_cavity_init_and_steps():
         close(a, f.out[k], 1e-12, k)
 
 
+@pytest.mark.parametrize("name", ["channel_couette_32", "channel_poiseuille_32", "channel_couette_128",
+                                  "channel_poiseuille_128"])
+def test_channel_flow(name):
+    """run_channel_flow (solver/channel_flow.cpp:432-541): the reference's final metrics, its
+    x-averaged streamwise profile and its validation figure, from the reference itself."""
+    f = fx.load(name)
+    p = f.params
+    c = ofref.channel_config(p["case"], p["nx"], p["ny"], p["viscosity"], p["top"], p["drop"], p["cfl"], p["steps"],
+                             p["maxit"], p["tol"])
+    s = ofref.channel_init(c)
+    ofref.channel_steps(c, s, p["steps"])
+    m = s.metrics
+    assert m.step == f.meta["step"] == p["steps"]
+    assert m.pressure_iterations == f.meta["pressure_iterations"]
+    assert m.dt == f.meta["dt"]
+    for k in ["max_cfl", "max_velocity_change"]:
+        assert abs(getattr(m, k) - f.meta[k]) <= 1e-12 * abs(f.meta[k]), k
+    assert abs(m.divergence_l2 - f.meta["divergence_l2"]) <= 1e-3 * f.meta["divergence_l2"] + 1e-16
+    prof, err = ofref.channel_profile(c, s.vel[0])
+    close(prof, f.out["profile"], 1e-13, "profile")
+    assert abs(err - f.meta["relative_l2_error"]) <= 1e-9 * f.meta["relative_l2_error"]
+    # the reference's own acceptance (solver/channel_flow.cpp:421-430)
+    assert f.meta["pass"] == 1 and err <= 5e-3 and m.divergence_l2 <= 1e-10
+
+
+def test_channel_flow_rejects_nonconverged_projection():
+    f = fx.load("channel_nonconverged")
+    p = f.params
+    c = ofref.channel_config(p["case"], p["nx"], p["ny"], p["viscosity"], p["top"], p["drop"], p["cfl"], p["steps"],
+                             p["maxit"], p["tol"])
+    s = ofref.channel_init(c)
+    with pytest.raises(RuntimeError) as e:
+        ofref.channel_steps(c, s, p["steps"])
+    assert str(e.value) == f.meta["error"]
+
+
 @pytest.mark.parametrize("n", [16, 32, 64])
 def test_mms(n):
     f = fx.load(f"mms_{n}")
