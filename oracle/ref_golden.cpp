@@ -12,6 +12,8 @@
 // Grid keys: nx ny nz dx dy dz. Velocity BC keys: bc=t0,..,t5 bcvel=18 numbers bcp=6 numbers.
 // Pressure BC keys: pbc=t0,..,t5 pbcval=6 numbers pbcgrad=6 numbers.
 
+#include "io/checkpoint.hpp"
+#include "io/vtk_export.hpp"
 #include "linsolve/poisson_solver.hpp"
 #include "operators/discrete_operators.hpp"
 #include "solver/channel_flow.hpp"
@@ -243,6 +245,8 @@ int run(const std::string& cmd, const Args& a) {
       w.kv("v_horizontal_max", r.extrema.v_horizontal_max);
       w.kv("v_horizontal_min", r.extrema.v_horizontal_min);
     }
+    if(a.count("ckpt")) solver::io::write_lid_driven_cavity_checkpoint(a.at("ckpt"), c, s);
+    if(a.count("vtk")) solver::io::write_lid_driven_cavity_vtk(a.at("vtk"), s);
     w.put(s.velocity);
     w.put(s.pressure_total);
     w.put(s.advection_previous);

@@ -119,5 +119,10 @@ void run_lid_driven_cavity_steps(const LidDrivenCavityConfig& config, int step_c
 [[nodiscard]] LidDrivenCavityResult finalize_lid_driven_cavity_result(const LidDrivenCavityConfig& config,
                                                                       const LidDrivenCavityState& state);
 [[nodiscard]] LidDrivenCavityResult run_lid_driven_cavity(const LidDrivenCavityConfig& config);
+// The steady-state loop of run_lid_driven_cavity (and of the reference tool's default run,
+// tools/lid_driven_cavity_main.cpp:67-75) continued from `state`: steps on the device until
+// min_steps have run and max |u_new - u_old| <= steady_tolerance, or max_steps. Used to resume
+// from a checkpoint; not part of the reference's header.
+void run_lid_driven_cavity_to_steady_state(const LidDrivenCavityConfig& config, LidDrivenCavityState& state);
 
 }  // namespace solver

@@ -355,9 +355,13 @@ LidDrivenCavityResult finalize_lid_driven_cavity_result(const LidDrivenCavityCon
   return r;
 }
 
+void run_lid_driven_cavity_to_steady_state(const LidDrivenCavityConfig& c, LidDrivenCavityState& st) {
+  advance(c, std::max(0, c.max_steps - st.metrics.step), true, st);
+}
+
 LidDrivenCavityResult run_lid_driven_cavity(const LidDrivenCavityConfig& c) {
   LidDrivenCavityState st = initialize_lid_driven_cavity_state(c);
-  advance(c, c.max_steps - st.metrics.step, true, st);
+  run_lid_driven_cavity_to_steady_state(c, st);
   return finalize_lid_driven_cavity_result(c, st);
 }
 

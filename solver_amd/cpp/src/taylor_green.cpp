@@ -292,6 +292,7 @@ double taylor_green_dt(const TaylorGreenConfig& c) {
 
 TaylorGreenState initialize_taylor_green_state(const TaylorGreenConfig& c) {
   validate_config(c);
+  require_hip_backend(c);
   TaylorGreenState st{make_taylor_green_grid(c)};
   DeviceRun run(c);
   run.initialize();
@@ -306,6 +307,7 @@ TaylorGreenState initialize_taylor_green_state(const TaylorGreenConfig& c) {
 void run_taylor_green_steps(const TaylorGreenConfig& c, const int step_count, TaylorGreenState& st) {
   validate_config(c);
   if(step_count < 0) throw std::invalid_argument("run_taylor_green_steps expects a non-negative step count");
+  require_hip_backend(c);
   const Grid g = make_taylor_green_grid(c);
   if(st.grid.nx != g.nx || st.grid.ny != g.ny || st.grid.nz != g.nz || st.grid.dx != g.dx || st.grid.dy != g.dy ||
      st.grid.dz != g.dz)

@@ -1,5 +1,8 @@
-// solver_cavity: the reference's tools/lid_driven_cavity_main.cpp command line and output,
-// with the step loop on the GPU. Checkpoints and VTK output are not part of this build.
+// solver_cavity: the reference's tools/lid_driven_cavity_main.cpp command line and output
+// (--steps, --restart <checkpoint>, --checkpoint-out <path>, --vtk-out <path>), with the step
+// loop on the GPU.
+#include "io/checkpoint.hpp"
+#include "io/vtk_export.hpp"
 #include "solver/lid_driven_cavity.hpp"
 
 #include <iostream>
@@ -11,6 +14,7 @@ int main(int argc, char** argv) {
   try {
     std::string config_path = "solver_amd/cpp/configs/lid_driven_cavity_re100_128.cfg";
     std::optional<int> steps;
+    std::string restart_path, checkpoint_out, vtk_out;
     bool have_config = false;
     for(int a = 1; a < argc; ++a) {
       const std::string arg = argv[a];
@@ -20,10 +24,12 @@ int main(int argc, char** argv) {
       };
       if(arg == "--steps") {
         steps = std::stoi(value());
-      } else if(arg == "--restart" || arg == "--checkpoint-out") {
-        throw std::runtime_error(arg + ": checkpoints are not part of this build");
+      } else if(arg == "--restart") {
+        restart_path = value();
+      } else if(arg == "--checkpoint-out") {
+        checkpoint_out = value();
       } else if(arg == "--vtk-out") {
-        throw std::runtime_error("--vtk-out: VTK export is not part of this build");
+        vtk_out = value();
       } else if(!have_config && !arg.empty() && arg.rfind("--", 0) != 0) {
         config_path = arg;
         have_config = true;
@@ -32,16 +38,18 @@ int main(int argc, char** argv) {
       }
     }
     const solver::LidDrivenCavityConfig config = solver::load_lid_driven_cavity_config(config_path);
-    solver::LidDrivenCavityResult r;
+    solver::LidDrivenCavityState state = restart_path.empty()
+                                             ? solver::initialize_lid_driven_cavity_state(config)
+                                             : solver::io::load_lid_driven_cavity_checkpoint(restart_path, config).state;
     if(steps) {
-      solver::LidDrivenCavityState state = solver::initialize_lid_driven_cavity_state(config);
       solver::run_lid_driven_cavity_steps(config, *steps, state);
-      r = solver::finalize_lid_driven_cavity_result(config, state);
     } else {
-      // the tool's steady-state loop: steps until min_steps have run and
-      // max |u_new - u_old| <= steady_tolerance, or max_steps
-      r = solver::run_lid_driven_cavity(config);
+      // steps until min_steps have run and max |u_new - u_old| <= steady_tolerance, or max_steps
+      solver::run_lid_driven_cavity_to_steady_state(config, state);
     }
+    if(!checkpoint_out.empty()) solver::io::write_lid_driven_cavity_checkpoint(checkpoint_out, config, state);
+    if(!vtk_out.empty()) solver::io::write_lid_driven_cavity_vtk(vtk_out, state);
+    const solver::LidDrivenCavityResult r = solver::finalize_lid_driven_cavity_result(config, state);
     std::cout << "config_path: " << config_path << '\n';
     std::cout << "config: " << solver::describe(config) << '\n';
     std::cout << "steps: " << r.final_step.step << '\n';
@@ -65,6 +73,9 @@ int main(int argc, char** argv) {
                 << ", reference_lower=" << p.reference_lower_bound << ", reference_upper=" << p.reference_upper_bound
                 << ", sample=" << p.sample_value << ", relative_error=" << p.relative_error << '\n';
     }
+    if(!restart_path.empty()) std::cout << "restart_path: " << restart_path << '\n';
+    if(!checkpoint_out.empty()) std::cout << "checkpoint_out: " << checkpoint_out << '\n';
+    if(!vtk_out.empty()) std::cout << "vtk_out: " << vtk_out << '\n';
     if(steps || !config.validate_reference) {
       std::cout << "benchmark_status: skipped" << '\n';
       return 0;

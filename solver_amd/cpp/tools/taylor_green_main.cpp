@@ -1,23 +1,39 @@
 // solver_taylor_green: the reference's tools/taylor_green_main.cpp command line and key: value
-// output, running the step loop on the GPU (backend = hip).
+// output (--backend, --vtk-out <path>), running the step loop on the GPU (backend = hip). This
+// build adds --steps <n> (run n steps instead of up to final_time, no validation), --restart
+// <checkpoint> and --checkpoint-out <path> (SLVCHKP1 Taylor-Green checkpoints).
+#include "io/checkpoint.hpp"
+#include "io/vtk_export.hpp"
 #include "solver/taylor_green.hpp"
 
+#include <cmath>
 #include <iostream>
+#include <optional>
 #include <stdexcept>
 #include <string>
 
 int main(int argc, char** argv) {
   try {
     std::string config_path = "solver_amd/cpp/configs/taylor_green_128.cfg";
-    std::string backend_override;
+    std::string backend_override, vtk_out, restart_path, checkpoint_out;
+    std::optional<int> steps;
     bool have_config = false;
     for(int a = 1; a < argc; ++a) {
       const std::string arg = argv[a];
+      auto value = [&]() -> std::string {
+        if(a + 1 >= argc) throw std::runtime_error("missing value for " + arg);
+        return argv[++a];
+      };
       if(arg == "--backend") {
-        if(a + 1 >= argc) throw std::runtime_error("missing value for --backend");
-        backend_override = argv[++a];
+        backend_override = value();
       } else if(arg == "--vtk-out") {
-        throw std::runtime_error("--vtk-out: VTK export is not part of this build");
+        vtk_out = value();
+      } else if(arg == "--steps") {
+        steps = std::stoi(value());
+      } else if(arg == "--restart") {
+        restart_path = value();
+      } else if(arg == "--checkpoint-out") {
+        checkpoint_out = value();
       } else if(!have_config && !arg.empty() && arg.rfind("--", 0) != 0) {
         config_path = arg;
         have_config = true;
@@ -28,7 +44,21 @@ int main(int argc, char** argv) {
     solver::TaylorGreenConfig config = solver::load_taylor_green_config(config_path);
     if(!backend_override.empty()) config.backend = solver::parse_execution_backend(backend_override);
     solver::TaylorGreenState state{solver::make_taylor_green_grid(config)};
-    const solver::TaylorGreenResult r = solver::run_taylor_green(config, &state);
+    solver::TaylorGreenResult r;
+    if(restart_path.empty() && !steps) {
+      r = solver::run_taylor_green(config, &state);
+    } else {
+      if(restart_path.empty())
+        state = solver::initialize_taylor_green_state(config);
+      else
+        state = solver::io::load_taylor_green_checkpoint(restart_path, config).state;
+      const int total = static_cast<int>(std::ceil(config.final_time / solver::taylor_green_dt(config)));
+      const int n = steps ? *steps : std::max(0, total - state.metrics.step);
+      solver::run_taylor_green_steps(config, n, state);
+      r = solver::finalize_taylor_green_result(config, state);
+    }
+    if(!checkpoint_out.empty()) solver::io::write_taylor_green_checkpoint(checkpoint_out, config, state);
+    if(!vtk_out.empty()) solver::io::write_mac_fields_vtk(vtk_out, state.velocity, state.pressure_total);
     std::cout << "config_path: " << config_path << '\n';
     std::cout << "config: " << solver::describe(config) << '\n';
     std::cout << "backend_used: " << solver::to_string(r.backend_used) << '\n';
@@ -50,7 +80,10 @@ int main(int argc, char** argv) {
     std::cout << "exact_kinetic_energy: " << r.validation.exact_kinetic_energy << '\n';
     std::cout << "normalized_energy_error: " << r.validation.normalized_energy_error << '\n';
     std::cout << "velocity_relative_l2_error: " << r.validation.velocity_relative_l2_error << '\n';
-    if(!config.validate_energy) {
+    if(!restart_path.empty()) std::cout << "restart_path: " << restart_path << '\n';
+    if(!checkpoint_out.empty()) std::cout << "checkpoint_out: " << checkpoint_out << '\n';
+    if(!vtk_out.empty()) std::cout << "vtk_out: " << vtk_out << '\n';
+    if(steps || !config.validate_energy) {
       std::cout << "benchmark_status: skipped" << '\n';
       return 0;
     }

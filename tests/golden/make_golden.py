@@ -250,6 +250,7 @@ def main():
         save(f"mms_{n}", dict(n=n), {}, {}, meta)
 
     channel_fixtures()
+    io_fixtures()
 
 
 def channel_fixtures():
@@ -265,6 +266,27 @@ def channel_fixtures():
     cp = dict(base, case="poiseuille", nx=32, ny=32, steps=4, maxit=1, tol=1e-30)
     _, meta = refrun.run("channel", (), (), **cp)
     save("channel_nonconverged", cp, {}, {}, meta)
+
+
+
+def io_fixtures():
+    """A cavity state after 6 steps with the SLVCHKP1 checkpoint and the legacy-VTK file the
+    reference's own io/checkpoint.cpp and io/vtk_export.cpp write for it (file bytes kept as
+    uint8 arrays)."""
+    import tempfile
+
+    cp = dict(nx=16, ny=16, reynolds=100.0, lid=1.0, cfl=0.5, tol=1e-10, maxit=200, max_steps=12, min_steps=1,
+              steady_tolerance=1e-12, validate=0)
+    g = dict(nx=16, ny=16, nz=1)
+    shp = shapes(g, VEL + [CELL] + VEL)
+    with tempfile.TemporaryDirectory() as td:
+        ck, vtk = os.path.join(td, "state.chk"), os.path.join(td, "state.vtk")
+        outs, meta = refrun.run("cavity_steps", (), shp, steps=6, ckpt=ck, vtk=vtk, **cp)
+        ck_bytes = np.fromfile(ck, dtype=np.uint8)
+        vtk_bytes = np.fromfile(vtk, dtype=np.uint8)
+    save("io_cavity16", dict(cp, steps=6), {},
+         dict(u=outs[0], v=outs[1], w=outs[2], p=outs[3], au=outs[4], av=outs[5], aw=outs[6], checkpoint=ck_bytes,
+              vtk=vtk_bytes), meta)
 
 
 if __name__ == "__main__":

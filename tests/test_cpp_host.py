@@ -7,8 +7,8 @@ import subprocess
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CPP = os.path.join(ROOT, "solver_amd", "cpp")
 BIN = os.path.join(CPP, "build", "bin")
-TOOLS = ["solver_taylor_green", "solver_cavity", "solver_pressure_profile", "solver_advection_profile",
-         "solver_operator_mms", "api_roundtrip", "host_checks"]
+TOOLS = ["solver_taylor_green", "solver_cavity", "solver_channel", "solver_pressure_profile", "solver_advection_profile",
+         "solver_operator_mms", "api_roundtrip", "host_checks", "io_check"]
 
 
 def _build():

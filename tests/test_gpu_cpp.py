@@ -7,6 +7,8 @@ of the reference and the reference's own golden values.
 * solver_operator_mms reproduces the reference's manufactured-solution errors (golden fixtures).
 * solver_taylor_green and solver_cavity reproduce the oracle's step metrics.
 * solver_pressure_profile reproduces the oracle's MGPCG iteration count.
+* Restarts from SLVCHKP1 checkpoints (cavity and Taylor-Green) are bitwise identical to
+  uninterrupted runs; the tools write legacy-VTK files.
 """
 import math
 import os
@@ -193,3 +195,70 @@ def test_advection_profile_tool_runs():
     out = kv(r.stdout)
     assert out["kernel"] == "advection" and int(out["logical_cells"]) == 64 * 64
     assert out["advection"] == "scheme=tvd, limiter=van_leer"
+
+
+def _report(text):
+    """Step metrics and extrema of a tool report, without the path lines."""
+    return {k: v for k, v in kv(text).items() if k not in ("config_path", "restart_path", "checkpoint_out", "vtk_out")}
+
+
+def test_cavity_restart_is_bitwise(tmp_path):
+    """200 uninterrupted steps against 100 + checkpoint + restart + 100, on the device: the two
+    final checkpoints (whole state and metrics) are the same bytes (tests/test_runtime.cpp:2093-2152)."""
+    cfg = tmp_path / "cav.cfg"
+    cfg.write_text("nx = 24\nny = 24\nreynolds = 100\nlid_velocity = 1.0\ncfl_limit = 0.5\nmax_steps = 200\n"
+                   "min_steps = 200\nsteady_tolerance = 1e-30\npoisson_max_iterations = 200\n"
+                   "poisson_tolerance = 1e-10\nvalidate_reference = false\n")
+    full, half, resumed, steady = (tmp_path / n for n in ("full.chk", "half.chk", "resumed.chk", "steady.chk"))
+    r_full = run_tool("solver_cavity", cfg, "--steps", 200, "--checkpoint-out", full)
+    assert r_full.returncode == 0, r_full.stderr
+    r = run_tool("solver_cavity", cfg, "--steps", 100, "--checkpoint-out", half)
+    assert r.returncode == 0, r.stderr
+    r_res = run_tool("solver_cavity", cfg, "--restart", half, "--steps", 100, "--checkpoint-out", resumed)
+    assert r_res.returncode == 0, r_res.stderr
+    assert kv(r_res.stdout)["restart_path"] == str(half)
+    assert resumed.read_bytes() == full.read_bytes()
+    assert _report(r_res.stdout) == _report(r_full.stdout)
+    # the default (steady-state) loop resumed from the checkpoint runs on to max_steps
+    r = run_tool("solver_cavity", cfg, "--restart", half, "--checkpoint-out", steady)
+    assert r.returncode == 0, r.stderr
+    assert int(kv(r.stdout)["steps"]) == 200
+    assert steady.read_bytes() == full.read_bytes()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_taylor_green_restart_is_bitwise(tmp_path, precision):
+    cfg = tmp_path / "tg.cfg"
+    cfg.write_text("nx = 24\nny = 24\nnz = 24\nviscosity = 0.01\ncfl_limit = 0.5\nfinal_time = 0.5\n"
+                   f"poisson_max_iterations = 400\npoisson_tolerance = 1e-10\nprecision = {precision}\n")
+    full, half, resumed = (tmp_path / n for n in ("full.chk", "half.chk", "resumed.chk"))
+    r_full = run_tool("solver_taylor_green", cfg, "--steps", 8, "--checkpoint-out", full)
+    assert r_full.returncode == 0, r_full.stderr
+    assert kv(r_full.stdout)["benchmark_status"] == "skipped"
+    assert run_tool("solver_taylor_green", cfg, "--steps", 5, "--checkpoint-out", half).returncode == 0
+    r_res = run_tool("solver_taylor_green", cfg, "--restart", half, "--steps", 3, "--checkpoint-out", resumed)
+    assert r_res.returncode == 0, r_res.stderr
+    assert resumed.read_bytes() == full.read_bytes()
+    drop = ("backend_elapsed_seconds",)
+    assert {k: v for k, v in _report(r_res.stdout).items() if k not in drop} == \
+        {k: v for k, v in _report(r_full.stdout).items() if k not in drop}
+
+
+def test_tools_write_vtk(tmp_path):
+    cfg = tmp_path / "cav.cfg"
+    cfg.write_text("nx = 32\nny = 32\nreynolds = 100\nmax_steps = 12\nmin_steps = 1\nsteady_tolerance = 1e-12\n"
+                   "validate_reference = false\n")
+    vtk = tmp_path / "cav.vtk"
+    r = run_tool("solver_cavity", cfg, "--steps", 2, "--vtk-out", vtk)
+    assert r.returncode == 0, r.stderr
+    text = vtk.read_text()
+    for line in ["# vtk DataFile Version 3.0", "DATASET STRUCTURED_POINTS", "DIMENSIONS 32 32 1",
+                 "POINT_DATA 1024", "VECTORS velocity double", "SCALARS pressure double 1"]:
+        assert line in text, line
+    assert len(text.splitlines()) == 9 + 1024 + 2 + 1024  # header, vectors, scalar header, scalars
+    tg = tmp_path / "tg.cfg"
+    tg.write_text("nx = 8\nny = 8\nnz = 8\nviscosity = 0.01\nfinal_time = 0.05\nvalidate_energy = false\n")
+    vtk = tmp_path / "tg.vtk"
+    r = run_tool("solver_taylor_green", tg, "--vtk-out", vtk)
+    assert r.returncode == 0, r.stderr
+    assert "DIMENSIONS 8 8 8" in vtk.read_text() and kv(r.stdout)["vtk_out"] == str(vtk)
