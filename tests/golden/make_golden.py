@@ -251,6 +251,7 @@ def main():
 
     channel_fixtures()
     io_fixtures()
+    validation_fixture()
 
 
 def channel_fixtures():
@@ -287,6 +288,21 @@ def io_fixtures():
     save("io_cavity16", dict(cp, steps=6), {},
          dict(u=outs[0], v=outs[1], w=outs[2], p=outs[3], au=outs[4], av=outs[5], aw=outs[6], checkpoint=ck_bytes,
               vtk=vtk_bytes), meta)
+
+
+
+def validation_fixture():
+    """The reference's published validation-suite results (validation/latest/*.csv, data files
+    of the reference checkout): operator MMS errors, Taylor-Green temporal refinement and the
+    benchmark-suite metrics."""
+    import csv
+
+    latest = os.path.join(os.environ.get("REF", "/root/reference"), "validation", "latest")
+    tables = {}
+    for name in ("operator_spatial", "taylor_green_temporal", "benchmark_summary"):
+        with open(os.path.join(latest, name + ".csv"), newline="") as f:
+            tables[name] = list(csv.DictReader(f))
+    save("validation_latest", {"source": "validation/latest"}, {}, {}, {"tables": tables})
 
 
 if __name__ == "__main__":
