@@ -184,7 +184,7 @@ def test_pressure_profile_tool_vs_oracle(precision):
     if precision == "fp64":
         assert out["average_iterations"] == d.iterations
     else:
-        assert abs(out["average_iterations"] - d.iterations) <= 2
+        assert out["average_iterations"] <= d.iterations + 1
     assert out["average_relative_residual"] <= 1e-10
     assert int(out["multigrid_levels"]) == d.multigrid_levels
 

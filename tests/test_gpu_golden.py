@@ -182,8 +182,8 @@ def test_poisson(ctx, name, precision):
         assert d.iterations == f.meta["iterations"]
         ref_hist = f.arrays["residual_history"]
         np.testing.assert_allclose(hist[: d.iterations + 1], ref_hist, rtol=1e-6, atol=1e-9 * ref_hist[0])
-    else:
-        assert abs(d.iterations - f.meta["iterations"]) <= 3
+    else:  # tuned FP32 cycle: at most one iteration above the reference
+        assert d.iterations <= f.meta["iterations"] + 1
     assert d.relative_residual <= p["tol"]
     a = host(sol)[1:-1, 1:-1, 1:-1]
     b = f.out["p"][1:-1, 1:-1, 1:-1]

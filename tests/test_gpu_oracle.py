@@ -10,6 +10,7 @@
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 import pytest
@@ -60,7 +61,8 @@ def test_taylor_green_multistep(ctx, precision, n, nz, viscosity, steps, convent
     if precision == sa.FP64:
         assert iters == ref_iters
     else:
-        assert all(abs(a - b) <= 3 for a, b in zip(iters, ref_iters)), (iters, ref_iters)
+        # the FP32 cycle's tuned smoother: never more than one iteration above the reference
+        assert all(a <= b + 1 for a, b in zip(iters, ref_iters)), (iters, ref_iters)
     for a, b, k in zip(vel + [p] + adv, s.vel + [s.p] + s.adv, ["u", "v", "w", "p", "au", "av", "aw"]):
         if not np.any(b[1:-1, 1:-1, 1:-1]):
             assert np.max(np.abs(a[1:-1, 1:-1, 1:-1])) <= 1e-14, k
@@ -139,6 +141,40 @@ def test_poisson_bc_families_vs_oracle(ctx, pbc, shape):
     assert d.converged == rd.converged == 1
     assert d.iterations == rd.iterations
     assert rel_max(host(sol)[1:-1, 1:-1, 1:-1], ref[1:-1, 1:-1, 1:-1]) <= 1e-10
+
+
+def test_mixed_smoother_weight(ctx):
+    """The FP32 cycle's tuned Jacobi weight cuts outer iterations (24 -> 17 on this problem);
+    SOLVER_AMD_MIXED_OMEGA = 2/3 restores the reference's cycle in FP32, whose count then stays
+    within one of the FP64 solve. Every variant meets the tolerance with the FP64 solution."""
+    n = 64
+    h = (2 * math.pi / n,) * 3
+    g = sa.grid(n, n, n, *h)
+    rng = np.random.default_rng(7)
+    rhs = dev(rng.standard_normal(sa.storage_shape(n, n, n, sa.CELL)))
+    pb = sa.pressure_bc_set([sa.P_PERIODIC] * 6)
+
+    def solve(prec, omega=None):
+        if omega is not None:
+            os.environ["SOLVER_AMD_MIXED_OMEGA"] = repr(omega)
+        try:
+            sol = zeros(g, sa.CELL)
+            d = sa.PoissonDiagnostics()
+            ctx.run("of_solve_pressure_poisson", C.byref(g), C.byref(pb),
+                    C.byref(sa.ProjectionOptions(1.0, 1.0, 400, 1e-12)), prec, sa.ptr(rhs), sa.ptr(sol), C.byref(d),
+                    None, 0)
+        finally:
+            os.environ.pop("SOLVER_AMD_MIXED_OMEGA", None)
+        assert d.converged == 1 and d.relative_residual <= 1e-12
+        return host(sol)[1:-1, 1:-1, 1:-1], d.iterations
+
+    ref, it64 = solve(sa.FP64)
+    tuned, it_tuned = solve(sa.MIXED)
+    plain, it_plain = solve(sa.MIXED, 2.0 / 3.0)
+    assert it_tuned <= it64 - 5, (it_tuned, it64)
+    assert abs(it_plain - it64) <= 1, (it_plain, it64)
+    for x in (tuned, plain):
+        assert rel_max(x, ref) <= 1e-10
 
 
 def test_solve_is_deterministic(ctx):
