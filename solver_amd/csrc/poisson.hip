@@ -25,8 +25,8 @@
 //    iterations past convergence return immediately (flag check), so the host queues the next
 //    iteration before it has read the previous one's residual.
 //  * Precision: FP64, or MIXED = FP32 V-cycle (levels, smoothing, residual, transfers) inside
-//    the FP64 CG (residual, dot products, pressure update). The FP32 V-cycle's Jacobi weight is
-//    tuned (kMixedOmega) instead of the reference's 2/3.
+//    the FP64 CG (residual, dot products, pressure update). The FP32 V-cycle's two sweeps per
+//    leg use Chebyshev weights (chebyshev_pair) instead of the reference's 2/3.
 //  * Reductions are deterministic (fixed launch shape, fixed combination order).
 
 #include "ofh.hpp"
@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -42,13 +43,23 @@ namespace ofh {
 namespace {
 
 constexpr double kOmega = 2.0 / 3.0;  // linsolve/poisson_solver.hpp:17-23
-// Jacobi weight of the FP32 V-cycle in MIXED solves. The mixed mode is held to the outer
-// tolerance, not to the reference's iteration counts, so its smoother is tuned: on the 256^3
-// Taylor-Green step 0.87 takes 15 outer iterations per solve against 23 at 2/3, and on the
-// Dirichlet, Neumann, mixed and periodic test families it needs at most one iteration more than
-// the FP64 solve. Any weight below 1 keeps the smoother convergent (the eigenvalues of D^-1 A
-// lie in [0, 2]) and the V-cycle symmetric positive definite.
-constexpr double kMixedOmega = 0.87;
+// Smoother of the FP32 V-cycle in MIXED solves. The mixed mode is held to the outer tolerance,
+// not to the reference's iteration counts, so its two Jacobi sweeps per leg use the weights of
+// the degree-2 Chebyshev polynomial on [a, 2] (the roots of T2 mapped to that interval), which
+// damps the modes with eigenvalues of D^-1 A in [a, 2] -- the modes the coarse levels cannot
+// represent -- far better than one repeated weight: a = 0.4 on three-dimensional grids (the
+// smallest such eigenvalue of the 7-point operator is 1/3) and 0.5 on two-dimensional ones
+// (1/2 for the 5-point operator). On the 256^3 Taylor-Green step this takes 10 outer
+// iterations per solve against 23 with the reference's 2/3 and 15 with the best single weight
+// (0.87); on the Dirichlet, Neumann, mixed and periodic test families it never needs more
+// iterations than the FP64 solve. The pair's polynomial (1 - w1 t)(1 - w2 t) lies in (-1, 1)
+// for t in (0, 2], so the smoother converges and the V-cycle stays symmetric positive definite.
+constexpr double kMixedCheb3D = 0.4, kMixedCheb2D = 0.5;
+void chebyshev_pair(double a, double& w1, double& w2) {
+  const double c = 0.5 * (2.0 + a), h = 0.5 * (2.0 - a) / std::sqrt(2.0);
+  w1 = 1.0 / (c - h);
+  w2 = 1.0 / (c + h);
+}
 constexpr double kSmallNumber = 1.0e-30;
 
 // Poisson scalar block (device doubles)
@@ -89,6 +100,8 @@ struct PLevel {
   double lo_val[3], hi_val[3], lo_grad[3], hi_grad[3], h[3];
   double winv[64];          // omega / D for the 4x4x4 boundary classes
   double winv_sum;          // sum over cells of omega / D
+  double wr;                // second-sweep weight ratio: the second pre-smoothing sweep and the
+                            // first post-smoothing sweep use wr omega / D (1 in FP64)
   int rx, ry, rz;           // refinement ratio to the next coarser level
 };
 
@@ -226,7 +239,7 @@ __device__ inline T smooth12_cell(const PLevel& L, const T* __restrict__ b, T mb
   const T xb = S(z.sm, B(i, j, z.cm) * W(cx, cy, bclass(z.cm, L.nz)) - m1);
   const T xf = S(z.sp, B(i, j, z.cp) * W(cx, cy, bclass(z.cp, L.nz)) - m1);
   const T ax = lap_neg<T>(c, xw, xe, xs, xn, xb, xf, T(L.ihx2), T(L.ihy2), T(L.ihz2));
-  return c + wc * (bt + T(-1) * ax);
+  return c + (wc * T(L.wr)) * (bt + T(-1) * ax);
 }
 
 // Fine residual (b - mb) - A (x2 - mx) at (i,j,k).
@@ -261,7 +274,7 @@ __device__ inline T prolong_smooth_cell(const PLevel& L, const PLevel& Cl, const
   const T ax = lap_neg<T>(c, S(x.sm, X3(x.cm, j, k)), S(x.sp, X3(x.cp, j, k)), S(y.sm, X3(i, y.cm, k)),
                           S(y.sp, X3(i, y.cp, k)), S(z.sm, X3(i, j, z.cm)), S(z.sp, X3(i, j, z.cp)), T(L.ihx2),
                           T(L.ihy2), T(L.ihz2));
-  return c + wc * ((b[(IX)i + (IX)L.nx * j + plane * k] - mb) + T(-1) * ax);
+  return c + (wc * T(L.wr)) * ((b[(IX)i + (IX)L.nx * j + plane * k] - mb) + T(-1) * ax);
 }
 
 // Second post-smoothing sweep: x4' = x4 - m4, returns x5 = x4' + w/D (bt - A x4'), bt = b - mb.
@@ -578,7 +591,7 @@ __global__ void __launch_bounds__(256) k_smooth12(const T* __restrict__ b, T* __
         const T xs = sgn(y.sm, X1(oym + pk, wdiag<T, UNI>(L, cx, cym, cz)));
         const T xn = sgn(y.sp, X1(oyp + pk, wdiag<T, UNI>(L, cx, cyp, cz)));
         const T ax = lap_neg<T>(cur, xw, xe, xs, xn, zw.below(k, prev, cur), zw.above(k, next, cur), ihx2, ihy2, ihz2);
-        const T out = cur + wc * (btc + T(-1) * ax);
+        const T out = cur + (wc * T(L.wr)) * (btc + T(-1) * ax);
         x2[col + pk] = out;
         v[0] += (double)out;
         prev = cur;
@@ -699,7 +712,7 @@ __global__ void __launch_bounds__(256) k_prolong_smooth(const T* __restrict__ b,
                                 sgn(y.sm, X3(oym + pk, pym + cpk)), sgn(y.sp, X3(oyp + pk, pyp + cpk)),
                                 zw.below(k, prev, cur), zw.above(k, next, cur), ihx2, ihy2, ihz2);
         const T wc = wdiag<T, UNI>(L, cx, cy, bclass(k, L.nz));
-        const T out = cur + wc * ((b[col + pk] - mb) + T(-1) * ax);
+        const T out = cur + (wc * T(L.wr)) * ((b[col + pk] - mb) + T(-1) * ax);
         x4[col + pk] = out;
         v[0] += (double)out;
         prev = cur;
@@ -870,6 +883,7 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
   const T mb = T(mbd);
   const T m1 = zm ? T((sbw - mbd * L.winv_sum) / (double)L.n) : T(0);
   const T w = T(L.winv[0]);
+  const T w2 = w * T(L.wr);  // second sweep
   const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
   const T inv_children = T(1.0 / 8.0);
   const FusedTile t = fused_tile(L, kz);
@@ -956,7 +970,7 @@ __global__ void __launch_bounds__(kFThreads) k_down_fused(const T* __restrict__ 
           const T c = x1c[be];
           const T ax = lap_neg<T>(c, x1c[be - 1], x1c[be + 1], x1c[be - BW], x1c[be + BW], x1m[be], x1p[be], ihx2,
                                   ihy2, ihz2);
-          const T out = c + w * (btc[e] + T(-1) * ax);
+          const T out = c + w2 * (btc[e] + T(-1) * ax);
           dst[e] = out;
           if(own && hg[m] >= 0) {
             x2g[(size_t)hg[m] + pq] = out;
@@ -1048,6 +1062,7 @@ __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b,
   }
   const T mc = T(mcd);
   const T w = T(L.winv[0]);
+  const T w2 = w * T(L.wr);  // first post-smoothing sweep
   const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
   const FusedTile t = fused_tile(L, kz);
   const int tx = tid & 63, ty = tid >> 6;
@@ -1131,7 +1146,7 @@ __global__ void __launch_bounds__(kFThreads) k_up_fused(const T* __restrict__ b,
           const T c = x3c[ae];
           const T ax =
               lap_neg<T>(c, x3c[ae - 1], x3c[ae + 1], x3c[ae - AW], x3c[ae + AW], x3m[ae], x3p[ae], ihx2, ihy2, ihz2);
-          dst[e] = c + w * (bq[e] + T(-1) * ax);
+          dst[e] = c + w2 * (bq[e] + T(-1) * ax);
         }
       }
     }
@@ -1950,7 +1965,7 @@ class PoissonSolver {
   int* perm = nullptr;
   double *factor = nullptr, *U = nullptr, *work = nullptr;
   int hist_cap = 0;
-  double omega = kOmega;  // Jacobi weight of every level's smoother
+  double omega = kOmega;  // Jacobi weight of every level's first pre-smoothing sweep
 
   PoissonSolver(Context& ctx, const of_grid& grid, const of_pressure_bc_set& b, int prec)
       : c(ctx), g(grid), bcs(b), precision(prec) {}
@@ -1981,14 +1996,23 @@ bool cg_fused_disabled() {
   return e != nullptr && e[0] == '1';
 }
 
-// Jacobi weight of the FP32 V-cycle of a MIXED solve: SOLVER_AMD_MIXED_OMEGA in (0, 1) when set
-// (read when a solver is built), else kMixedOmega.
-double mixed_omega() {
+// Jacobi weights of the FP32 V-cycle of a MIXED solve on a grid of dimension `dims`.
+// SOLVER_AMD_MIXED_OMEGA, read when a solver is built, overrides them: one weight w for both
+// sweeps, or a pair "w1,w2" (first and second pre-smoothing sweep; the post-smoothing sweeps
+// run them in reverse order) whose polynomial (1 - w1 t)(1 - w2 t) stays within (-1, 1) on
+// t in (0, 2].
+void mixed_omega(int dims, double& w1, double& w2) {
+  chebyshev_pair(dims >= 3 ? kMixedCheb3D : kMixedCheb2D, w1, w2);
   const char* e = std::getenv("SOLVER_AMD_MIXED_OMEGA");
-  if(e == nullptr || e[0] == 0) return kMixedOmega;
-  const double w = std::atof(e);
-  if(!(w > 0.0 && w < 1.0)) throw InvalidArgument("SOLVER_AMD_MIXED_OMEGA must lie in (0, 1)");
-  return w;
+  if(e == nullptr || e[0] == 0) return;
+  const char* comma = std::strchr(e, ',');
+  w1 = std::atof(e);
+  w2 = comma != nullptr ? std::atof(comma + 1) : w1;
+  const double p2 = (1.0 - 2.0 * w1) * (1.0 - 2.0 * w2);  // at t = 2
+  const double tm = (w1 + w2) / (2.0 * w1 * w2);           // where the polynomial is smallest
+  const double pm = 1.0 - (w1 + w2) * (w1 + w2) / (4.0 * w1 * w2);
+  if(!(w1 > 0.0 && w2 > 0.0 && std::fabs(p2) < 1.0 && (tm > 2.0 || pm > -1.0)))
+    throw InvalidArgument("SOLVER_AMD_MIXED_OMEGA must give a convergent smoother");
 }
 
 bool p_axis_periodic(const of_pressure_bc_set& b, int axis) {
@@ -2049,6 +2073,7 @@ PLevel make_level(int nx, int ny, int nz, double dx, double dy, double dz, const
       for(int cx = 0; cx < 4; ++cx)
         s += count(nx, cx) * count(ny, cy) * count(nz, cz) * L.winv[cx + 4 * (cy + 4 * cz)];
   L.winv_sum = s;
+  L.wr = 1.0;
   return L;
 }
 
@@ -2145,7 +2170,8 @@ PoissonSolver::~PoissonSolver() {
 }
 
 void PoissonSolver::build() {
-  if(precision == OF_PRECISION_MIXED) omega = mixed_omega();
+  double w2 = omega;
+  if(precision == OF_PRECISION_MIXED) mixed_omega((g.nx > 1) + (g.ny > 1) + (g.nz > 1), omega, w2);
   for(int f = 0; f < 6; ++f)
     if(bcs.faces[f].type == OF_PRESSURE_DIRICHLET) zm = false;
   // hierarchy (linsolve/poisson_solver.cpp:316-384)
@@ -2170,6 +2196,7 @@ void PoissonSolver::build() {
     lv.push_back(make_level(nx, ny, nz, dx, dy, dz, bcs, omega));
   }
   if((int)lv.size() > kMaxLevels) throw InvalidArgument("multigrid hierarchy deeper than supported");
+  for(PLevel& L : lv) L.wr = w2 / omega;  // exactly 1 with one weight
   l0 = (int)lv.size() - 1;
   for(int l = 0; l < (int)lv.size(); ++l)
     if(lv[l].n <= kTailCells) {
