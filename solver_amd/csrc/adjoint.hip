@@ -581,7 +581,8 @@ struct AdjointRun {
     (void)std::frexp(h[1], &e);  // h[1] = m 2^e with m in [0.5, 1)
     const double down = std::ldexp(1.0, -e), up = std::ldexp(1.0, e);
     launch_scale_active(c, Lc, down, rhs, rhs);
-    pr = poisson_solve(fwd.poisson, rhs, out, fwd.cfg.poisson_max_iterations, fwd.cfg.poisson_tolerance, false);
+    pr = poisson_solve(fwd.poisson, rhs, out, fwd.cfg.poisson_max_iterations, fwd.cfg.poisson_tolerance, false,
+                       true);  // out was zeroed above
     if(!pr.diag.converged) solve_failed(what, pr);
     launch_scale_active(c, Lc, up, rhs, rhs);
     launch_scale_active(c, Lc, up, out, out);

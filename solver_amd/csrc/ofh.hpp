@@ -113,9 +113,10 @@ class PoissonSolver;  // MGPCG on the device, reused across solves of one grid/B
 PoissonSolver* poisson_create(Context& c, const of_grid& g, const of_pressure_bc_set& b, int precision);
 void poisson_destroy(PoissonSolver* s);
 // Solves -lap(p) = rhs (reference sign conventions); rhs and p in ghosted cell storage.
-// p holds the warm start on entry. Only the active range of p is written.
+// p holds the warm start on entry (zero_guess: the warm start is zero and p is not read). Only
+// the active range of p is written.
 PoissonResult poisson_solve(PoissonSolver* s, const double* rhs, double* p, int max_iterations, double tol,
-                            bool want_history);
+                            bool want_history, bool zero_guess = false);
 void launch_poisson_apply(Context& c, const of_grid& g, const of_pressure_bc_set& b, bool homogeneous,
                           const double* in, double* out);
 

@@ -482,6 +482,45 @@ __global__ void k_setup1(const double* __restrict__ rhs_g, const double* __restr
   store_partials<2>(v, ops, partials);
 }
 
+// k_setup1 for a zero warm start: p_c = 0 without reading p.
+__global__ void k_setup1_zero(const double* __restrict__ rhs_g, Lay Lc, PLevel L, double* __restrict__ sys,
+                              double* __restrict__ pc, double* partials, int kz) {
+  const Tile t = tile(L, kz);
+  double v[2] = {0.0, 0.0};
+  if(t.i < L.nx && t.j < L.ny) {
+    for(int k = t.k0; k < t.k1; ++k) {
+      const size_t id = (size_t)t.i + (size_t)L.nx * ((size_t)t.j + (size_t)L.ny * k);
+      const double s = -rhs_g[Lc.at(t.i + 1, t.j + 1, k + 1)] + -1.0 * boundary_source(L, t.i, t.j, k);
+      sys[id] = s;
+      pc[id] = 0.0;
+      v[0] += s;
+    }
+  }
+  const int ops[2] = {kSum, kSum};
+  store_partials<2>(v, ops, partials);
+}
+
+// k_setup2 for a zero warm start: A(p - m_p) = A 0 = 0 exactly, so r = sys - m_sys pointwise
+// (the same values as the stencil path).
+__global__ void k_setup2_zero(double* __restrict__ r, PLevel L, double* partials, const double* ps, int kz) {
+  const double msys = ps[P_MSYS];
+  const Tile t = tile(L, kz);
+  double v[3] = {0.0, 0.0, 0.0};
+  if(t.i < L.nx && t.j < L.ny) {
+    const int cxy = bclass(t.i, L.nx) + 4 * bclass(t.j, L.ny);
+    for(int k = t.k0; k < t.k1; ++k) {
+      const size_t id = (size_t)t.i + (size_t)L.nx * ((size_t)t.j + (size_t)L.ny * k);
+      const double rv = r[id] - msys;
+      r[id] = rv;
+      v[0] += rv;
+      v[1] += rv * rv;
+      v[2] += rv * L.winv[cxy + 16 * bclass(k, L.nz)];
+    }
+  }
+  const int ops[3] = {kSum, kSum, kSum};
+  store_partials<3>(v, ops, partials);
+}
+
 // r = (sys - m_sys) - A(p - m_p); sums of r, r^2, r w/D.
 __global__ void k_setup2(double* __restrict__ r, const double* __restrict__ pc, PLevel L, double* partials,
                          const double* ps, int kz) {
@@ -1974,7 +2013,7 @@ class PoissonSolver {
   template <typename T>
   void vcycle();
   template <typename T>
-  PoissonResult solve_t(const double* rhs, double* p, int maxit, double tol, bool want_history);
+  PoissonResult solve_t(const double* rhs, double* p, int maxit, double tol, bool want_history, bool zero_guess);
   // the array holding a level's smoothed correction after the up leg: the fused up kernel
   // writes it beside x2 (its halo is still being read by neighbouring tiles)
   void* x_final(int l) const { return l < l0 && fused[l] ? lxb[l] : lxa[l]; }
@@ -2385,7 +2424,8 @@ void PoissonSolver::vcycle() {
 }
 
 template <typename T>
-PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, double tol, bool want_history) {
+PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, double tol, bool want_history,
+                                     bool zero_guess) {
   if(hist_cap < maxit + 1) {
     if(history) OFH_CHECK(hipFree(history));
     OFH_CHECK(hipMalloc(&history, (size_t)(maxit + 1) * sizeof(double)));
@@ -2401,9 +2441,15 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   const bool cg_fused = !cg_fused_disabled();
   const unsigned pgrid = std::min<unsigned>(div_up((size_t)F.n, 256 * kPointIlp), 8192u);
   OFH_CHECK(hipMemsetAsync(ps, 0, P_COUNT * sizeof(double), c.stream));
-  hipLaunchKernelGGL(k_setup1, ll.grid, ll.block, 0, c.stream, rhs, p, Lc, F, r, pc, c.partials, ll.kz);
+  if(zero_guess)
+    hipLaunchKernelGGL(k_setup1_zero, ll.grid, ll.block, 0, c.stream, rhs, Lc, F, r, pc, c.partials, ll.kz);
+  else
+    hipLaunchKernelGGL(k_setup1, ll.grid, ll.block, 0, c.stream, rhs, p, Lc, F, r, pc, c.partials, ll.kz);
   launch_finalize(c.stream, c.partials, ll.grid, EpiSetup1{ps, n0, zmi});
-  hipLaunchKernelGGL(k_setup2, ll.grid, ll.block, 0, c.stream, r, pc, F, c.partials, ps, ll.kz);
+  if(zero_guess)
+    hipLaunchKernelGGL(k_setup2_zero, ll.grid, ll.block, 0, c.stream, r, F, c.partials, ps, ll.kz);
+  else
+    hipLaunchKernelGGL(k_setup2, ll.grid, ll.block, 0, c.stream, r, pc, F, c.partials, ps, ll.kz);
   launch_finalize(c.stream, c.partials, ll.grid, EpiSetup2{ps, history, n0, tol, zmi});
   if(mixed)  // FP32 copy of the initial residual: the rhs of the finest FP32 level
     hipLaunchKernelGGL(k_cast<T>, 2048, 256, 0, c.stream, r, static_cast<T*>(lb[0]), F.n, ps);
@@ -2493,11 +2539,12 @@ PoissonSolver* poisson_create(Context& c, const of_grid& g, const of_pressure_bc
 
 void poisson_destroy(PoissonSolver* s) { delete s; }
 
-PoissonResult poisson_solve(PoissonSolver* s, const double* rhs, double* p, int maxit, double tol, bool want_history) {
+PoissonResult poisson_solve(PoissonSolver* s, const double* rhs, double* p, int maxit, double tol, bool want_history,
+                            bool zero_guess) {
   if(maxit <= 0) throw InvalidArgument("solve_pressure_poisson expects iterations > 0");
   if(!(tol > 0.0)) throw InvalidArgument("solve_pressure_poisson expects tolerance > 0");
-  if(s->precision == OF_PRECISION_MIXED) return s->solve_t<float>(rhs, p, maxit, tol, want_history);
-  return s->solve_t<double>(rhs, p, maxit, tol, want_history);
+  if(s->precision == OF_PRECISION_MIXED) return s->solve_t<float>(rhs, p, maxit, tol, want_history, zero_guess);
+  return s->solve_t<double>(rhs, p, maxit, tol, want_history, zero_guess);
 }
 
 void launch_poisson_apply(Context& c, const of_grid& g, const of_pressure_bc_set& b, bool homogeneous,

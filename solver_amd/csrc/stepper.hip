@@ -346,7 +346,8 @@ void Stepper::step() {
   // projection (solver/projection.cpp:1097-1131); pred already carries its BC ghosts
   launch_divergence_scaled(c, g, pred[0], pred[1], pred[2], 1.0 / dt, prhs);
   OFH_CHECK(hipMemsetAsync(phi, 0, make_lay(g, OF_CELL).size() * sizeof(double), c.stream));
-  const PoissonResult pr = poisson_solve(poisson, prhs, phi, cfg.poisson_max_iterations, cfg.poisson_tolerance, false);
+  const PoissonResult pr =
+      poisson_solve(poisson, prhs, phi, cfg.poisson_max_iterations, cfg.poisson_tolerance, false, true);
   if(!pr.diag.converged) {  // solver/projection.cpp:1133-1147
     char buf[512];
     std::snprintf(buf, sizeof buf,
