@@ -529,6 +529,7 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
     commit(kk);
   }
   fetch(kb + 2);
+  double ff_prev = 0.0;  // upper-face flux of the previous plane
   for(int k = kb; k < ke; ++k) {
     commit(k + 2);
     point_loads(k);
@@ -598,8 +599,12 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
       const double ef = ev * reconstruct4(Q(s0, 0, -1), qc, Q(s0, 0, 1), Q(s0, 0, 2), i + 1, mx, ev, P.scheme);
       const double sf = sv * reconstruct4(Q(s0, -2, 0), Q(s0, -1, 0), qc, Q(s0, 1, 0), j, my, sv, P.scheme);
       const double nf = nv * reconstruct4(Q(s0, -1, 0), qc, Q(s0, 1, 0), Q(s0, 2, 0), j + 1, my, nv, P.scheme);
-      const double bf = bv * reconstruct4(Q(s2m, 0, 0), Q(sm, 0, 0), qc, Q(sp, 0, 0), k, mz, bv, P.scheme);
+      // the lower face of plane k is the upper face of plane k - 1 with the same operands (values,
+      // face velocity, index), so its flux is carried over from the previous plane
+      const double bf =
+          k > kb ? ff_prev : bv * reconstruct4(Q(s2m, 0, 0), Q(sm, 0, 0), qc, Q(sp, 0, 0), k, mz, bv, P.scheme);
       const double ff = fv * reconstruct4(Q(sm, 0, 0), qc, Q(sp, 0, 0), Q(s2p, 0, 0), k + 1, mz, fv, P.scheme);
+      ff_prev = ff;
       const double a = (ef - wf) / P.g.dx + (nf - sf) / P.g.dy + (ff - bf) / P.g.dz;
       const double grad = (p0 - p1) / h;
       const size_t at = L.at(i, j, k);

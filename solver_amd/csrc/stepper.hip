@@ -345,7 +345,9 @@ void Stepper::step() {
     predictor_adi(c, g, bcs, 0.5 * nu * dt, rhs[0], rhs[1], rhs[2], pred[0], pred[1], pred[2]);
   // projection (solver/projection.cpp:1097-1131); pred already carries its BC ghosts
   launch_divergence_scaled(c, g, pred[0], pred[1], pred[2], 1.0 / dt, prhs);
-  OFH_CHECK(hipMemsetAsync(phi, 0, make_lay(g, OF_CELL).size() * sizeof(double), c.stream));
+  // phi starts from zero (solver/taylor_green.cpp:596): the zero-warm-start solve neither reads
+  // it nor needs it cleared; it writes the active range and the BC fill the face ghosts (the
+  // edge ghosts keep their zeros from allocation)
   const PoissonResult pr =
       poisson_solve(poisson, prhs, phi, cfg.poisson_max_iterations, cfg.poisson_tolerance, false, true);
   if(!pr.diag.converged) {  // solver/projection.cpp:1133-1147
