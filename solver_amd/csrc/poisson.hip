@@ -1933,7 +1933,7 @@ HotLaunch hot_launch(const PLevel& L) {
   return h;
 }
 
-// Launch shape of the fused all-periodic level kernels: 64x8 tiles, kz planes per block (even,
+// Launch shape of the fused all-periodic level kernels: 64 x rows tiles, kz planes per block (even,
 // so that z pairs of the restriction stay inside a block). Each block re-reads the 4 planes
 // around its range and pays a pipeline fill of 4 plane steps, so long ranges are cheaper; the
 // grid is sized to one round of resident blocks (occupancy x compute units), which also leaves
