@@ -1273,6 +1273,7 @@ struct TailBufs {
   int owork;  // LDS offset of the coarse work vector (doubles)
 };
 constexpr int kTailLdsBytes = 152 * 1024;
+constexpr int kCoarseStage = 16;  // coarsest systems of up to this many unknowns are read from LDS
 
 // Thread groups of the tail. Levels of more than kWaveCells cells run on the whole workgroup
 // (workgroup barriers, block-wide sums); the smaller ones and the coarsest solve run on its
@@ -1323,6 +1324,19 @@ __global__ void __launch_bounds__(kTailThreads)
     T* dst = Bp(l0);
     for(int id = threadIdx.x; id < n0; id += blockDim.x) dst[id] = src[id];
     __syncthreads();
+  }
+  // a small coarsest system's row swaps, multipliers and U are staged in LDS at the start, so the
+  // serial elimination below reads no global memory (ordered by the barrier after the fold)
+  __shared__ double cfac[kCoarseStage * kCoarseStage], cU[kCoarseStage * kCoarseStage];
+  __shared__ int cperm[kCoarseStage];
+  const int ncs = lvs[nl - 1].n;
+  const bool cstage = ncs <= kCoarseStage;
+  if(cstage) {
+    for(int i = threadIdx.x; i < ncs * ncs; i += blockDim.x) {
+      cfac[i] = factor[i];
+      cU[i] = U[i];
+    }
+    for(int i = threadIdx.x; i < ncs; i += blockDim.x) cperm[i] = perm[i];
   }
   __shared__ double lm[kMaxLevels][4];  // per level: mb, m2, m4, m5
   __shared__ double lsbw[kMaxLevels];   // sum of raw b w/D
@@ -1435,13 +1449,16 @@ __global__ void __launch_bounds__(kTailThreads)
     const T* b = Bp(lc);
     T* x = XA(lc);
     const T mb = T(lm[lc][0]);
+    const int* pv = cstage ? cperm : perm;
+    const double* fv = cstage ? cfac : factor;
+    const double* uv = cstage ? cU : U;
     for(int i = G.tid; i < n; i += G.nt) work[i] = (double)(b[i] - mb);
     G.sync();
     if(zm && G.tid == 0) work[n - 1] = 0.0;
     G.sync();
     for(int p = 0; p < n; ++p) {
       if(G.tid == 0) {
-        const int best = perm[p];
+        const int best = pv[p];
         if(best != p) {
           const double t = work[p];
           work[p] = work[best];
@@ -1451,7 +1468,7 @@ __global__ void __launch_bounds__(kTailThreads)
       G.sync();
       const double wp = work[p];
       for(int row = p + 1 + G.tid; row < n; row += G.nt) {
-        const double f = factor[(size_t)row * n + p];
+        const double f = fv[(size_t)row * n + p];
         if(f != 0.0) work[row] -= f * wp;
       }
       G.sync();
@@ -1459,8 +1476,8 @@ __global__ void __launch_bounds__(kTailThreads)
     if(G.tid == 0) {
       for(int row = n - 1; row >= 0; --row) {
         double s = work[row];
-        for(int c = row + 1; c < n; ++c) s -= U[(size_t)row * n + c] * work[c];
-        work[row] = s / U[(size_t)row * n + row];
+        for(int c = row + 1; c < n; ++c) s -= uv[(size_t)row * n + c] * work[c];
+        work[row] = s / uv[(size_t)row * n + row];
       }
       if(zm) {
         double s = 0.0;
