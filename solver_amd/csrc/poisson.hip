@@ -1274,6 +1274,7 @@ struct TailBufs {
 };
 constexpr int kTailLdsBytes = 152 * 1024;
 constexpr int kCoarseStage = 16;  // coarsest systems of up to this many unknowns are read from LDS
+constexpr int kTailPre = 4;       // level-l0 values per thread read ahead of the fold (4096 cells)
 
 // Thread groups of the tail. Levels of more than kWaveCells cells run on the whole workgroup
 // (workgroup barriers, block-wide sums); the smaller ones and the coarsest solve run on its
@@ -1318,33 +1319,56 @@ __global__ void __launch_bounds__(kTailThreads)
   auto XA = [&](int l) -> T* { return LDS ? sT + B.oxa[l] : B.xa[l]; };
   auto XB = [&](int l) -> T* { return LDS ? sT + B.oxb[l] : B.xb[l]; };
   double* work = LDS ? sbuf + B.owork : gwork;
-  if(LDS) {
-    const int n0 = lvs[l0].n;
-    const T* src = B.b[l0];
-    T* dst = Bp(l0);
-    for(int id = threadIdx.x; id < n0; id += blockDim.x) dst[id] = src[id];
-    __syncthreads();
-  }
-  // a small coarsest system's row swaps, multipliers and U are staged in LDS at the start, so the
-  // serial elimination below reads no global memory (ordered by the barrier after the fold)
+  // The start's global reads (level l0's b when the levels are LDS-resident, a small coarsest
+  // system's row swaps, multipliers and U) are issued before the fold, so their latencies overlap
+  // with it; the barrier after the fold orders the LDS stores before any use. Staging the coarsest
+  // system lets the serial elimination below read no global memory.
   __shared__ double cfac[kCoarseStage * kCoarseStage], cU[kCoarseStage * kCoarseStage];
   __shared__ int cperm[kCoarseStage];
+  static_assert(kCoarseStage * kCoarseStage <= kTailThreads, "one staged entry per thread");
+  const int tid = threadIdx.x;
+  const int n0 = lvs[l0].n;
+  T bpre[kTailPre];
+  if(LDS) {
+#pragma unroll
+    for(int u = 0; u < kTailPre; ++u) {
+      const int id = tid + u * (int)blockDim.x;
+      bpre[u] = id < n0 ? B.b[l0][id] : T(0);
+    }
+  }
   const int ncs = lvs[nl - 1].n;
   const bool cstage = ncs <= kCoarseStage;
+  double fpre = 0.0, upre = 0.0;
+  int ppre = 0;
   if(cstage) {
-    for(int i = threadIdx.x; i < ncs * ncs; i += blockDim.x) {
-      cfac[i] = factor[i];
-      cU[i] = U[i];
+    if(tid < ncs * ncs) {
+      fpre = factor[tid];
+      upre = U[tid];
     }
-    for(int i = threadIdx.x; i < ncs; i += blockDim.x) cperm[i] = perm[i];
+    if(tid < ncs) ppre = perm[tid];
   }
   __shared__ double lm[kMaxLevels][4];  // per level: mb, m2, m4, m5
   __shared__ double lsbw[kMaxLevels];   // sum of raw b w/D
   __shared__ double rz_sh;
-  const int tid = threadIdx.x;
   const int s0 = P_LEVEL + 8 * l0;
   double mb0, sbw0;
-  down_fold(fold, ps, s0, (double)lvs[l0].n, zm, mb0, sbw0);
+  down_fold(fold, ps, s0, (double)n0, zm, mb0, sbw0);
+  if(LDS) {
+    T* dst = Bp(l0);
+#pragma unroll
+    for(int u = 0; u < kTailPre; ++u) {
+      const int id = tid + u * (int)blockDim.x;
+      if(id < n0) dst[id] = bpre[u];
+    }
+    for(int id = tid + kTailPre * (int)blockDim.x; id < n0; id += blockDim.x) dst[id] = B.b[l0][id];
+  }
+  if(cstage) {
+    if(tid < ncs * ncs) {
+      cfac[tid] = fpre;
+      cU[tid] = upre;
+    }
+    if(tid < ncs) cperm[tid] = ppre;
+  }
   if(tid == 0) {
     lm[l0][0] = mb0;
     lsbw[l0] = sbw0;
