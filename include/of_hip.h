@@ -127,6 +127,11 @@ typedef struct of_context of_context;
 /* No reference counterpart (the reference is single-threaded CPU code). Creates the device
  * context (kernel scratch, reduction buffers, stream) on HIP device `device`. */
 int of_context_create(int device, void* stream, of_context** out);
+/* Drops the caller's reference. Every stepper and adjoint run created on the context holds a
+ * reference of its own, so the native context (and its stream) is released with the last of
+ * them: handles may be destroyed in any order. A HIP failure while releasing is returned as
+ * OF_HIP_ERROR by the destroy call that ran it (and clears HIP's last error), never left for
+ * a later call. */
 int of_context_destroy(of_context* ctx);
 int of_context_synchronize(of_context* ctx);
 const char* of_last_error(void);

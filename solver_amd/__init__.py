@@ -13,7 +13,6 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
-import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsolver_amd.so")
@@ -248,20 +247,19 @@ class Context:
         if not torch.cuda.is_available():
             raise RuntimeError("solver_amd needs a HIP device")
         self.device = device
-        # objects whose native state refers to this context (steppers, adjoint runs); they are
-        # closed before the context, whatever order the garbage collector finalises them in
-        self._dependents = weakref.WeakSet()
         h = C.c_void_p()
         s = C.c_void_p(stream) if stream else C.c_void_p(None)
         call("of_context_create", device, s, C.byref(h))
         self.h = h
 
     def close(self):
+        """Drops this handle's reference to the native context. Steppers and adjoint runs
+        created on it hold references of their own in the library, so the native context lives
+        until the last of them is closed too, whatever order the handles are released in
+        (a garbage collector may finalise a context before the steppers that use it)."""
         if self.h:
-            for d in list(self._dependents):
-                d.close()
-            call("of_context_destroy", self.h)
-            self.h = None
+            h, self.h = self.h, None
+            call("of_context_destroy", h)
 
     def __del__(self):
         try:
@@ -302,13 +300,12 @@ class Stepper:
         h = C.c_void_p()
         call("of_stepper_create", ctx.h, C.byref(cfg), C.byref(h))
         self.h = h
-        ctx._dependents.add(self)
         self.nx, self.ny, self.nz = nx, ny, nz
 
     def close(self):
         if self.h:
-            call("of_stepper_destroy", self.h)
-            self.h = None
+            h, self.h = self.h, None
+            call("of_stepper_destroy", h)
 
     def __del__(self):
         try:
@@ -387,13 +384,12 @@ class Adjoint:
         h = C.c_void_p()
         call("of_adjoint_create", ctx.h, C.byref(self.cfg), int(checkpoint_every), C.byref(h))
         self.h = h
-        ctx._dependents.add(self)
         self.shape = (nx, ny, nz)
 
     def close(self):
         if self.h:
-            call("of_adjoint_destroy", self.h)
-            self.h = None
+            h, self.h = self.h, None
+            call("of_adjoint_destroy", h)
 
     def __del__(self):
         try:

@@ -91,6 +91,9 @@ DeviceField::DeviceField(const std::size_t count) : count_(count) {
   context();
   hip_check(hipMalloc(reinterpret_cast<void**>(&ptr_), count_ * sizeof(double)), "hipMalloc");
   hip_check(hipMemset(ptr_, 0, count_ * sizeof(double)), "hipMemset");
+  // the library's kernels run on a non-blocking stream, which the legacy stream's memset does
+  // not order against
+  hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
 }
 
 DeviceField::DeviceField(const StructuredField& host) : DeviceField(host.size()) { upload(host); }
@@ -101,7 +104,9 @@ DeviceField::DeviceField(DeviceField&& other) noexcept : ptr_(other.ptr_), count
 }
 
 DeviceField::~DeviceField() {
-  if(ptr_ != nullptr) (void)hipFree(ptr_);
+  // a destructor cannot report; clear HIP's last error so that a failed free does not surface
+  // as the error of a later, unrelated call
+  if(ptr_ != nullptr && hipFree(ptr_) != hipSuccess) (void)hipGetLastError();
 }
 
 void DeviceField::upload(const StructuredField& host) {

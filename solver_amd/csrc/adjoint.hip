@@ -436,7 +436,7 @@ void launch_component_bc_adjoint(Context& c, const of_grid& g, int comp, double*
   const Lay L = make_lay(g, OF_FACE_X + comp);
   for(int axis = 2; axis >= 0; --axis) {  // the forward fills axes 0, 1, 2 in turn
     const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
-    hipLaunchKernelGGL(k_vbc_periodic_adj, slab_grid2(L.a(o0), L.a(o1)), dim3(64, 4), 0, c.stream, f, L, comp, axis);
+    OFH_LAUNCH(k_vbc_periodic_adj, slab_grid2(L.a(o0), L.a(o1)), dim3(64, 4), 0, c.stream, f, L, comp, axis);
   }
 }
 
@@ -448,13 +448,13 @@ void launch_pressure_bc_adjoint(Context& c, const of_grid& g, double* p) {
   const Lay L = make_lay(g, OF_CELL);
   for(int axis = 2; axis >= 0; --axis) {
     const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
-    hipLaunchKernelGGL(k_pbc_periodic_adj, slab_grid2(L.a(o0), L.a(o1)), dim3(64, 4), 0, c.stream, p, L, axis);
+    OFH_LAUNCH(k_pbc_periodic_adj, slab_grid2(L.a(o0), L.a(o1)), dim3(64, 4), 0, c.stream, p, L, axis);
   }
 }
 
 void launch_gradient_adjoint(Context& c, const of_grid& g, double scale, const double* const* gbar, double* pbar) {
   const Lay Lc = make_lay(g, OF_CELL);
-  hipLaunchKernelGGL(k_grad_adj, stride_blocks(Lc.size()), 256, 0, c.stream, gbar[0], gbar[1], gbar[2],
+  OFH_LAUNCH(k_grad_adj, stride_blocks(Lc.size()), 256, 0, c.stream, gbar[0], gbar[1], gbar[2],
                      make_lay(g, OF_FACE_X), make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), Lc, g, scale, pbar);
 }
 
@@ -462,7 +462,7 @@ void launch_divergence_adjoint(Context& c, const of_grid& g, double scale, const
   const Lay Lc = make_lay(g, OF_CELL);
   for(int a = 0; a < 3; ++a) {
     const Lay L = make_lay(g, OF_FACE_X + a);
-    hipLaunchKernelGGL(k_div_adj, stride_blocks(L.size()), 256, 0, c.stream, dbar, Lc, L, a, spacing(g, a), scale,
+    OFH_LAUNCH(k_div_adj, stride_blocks(L.size()), 256, 0, c.stream, dbar, Lc, L, a, spacing(g, a), scale,
                        ubar[a]);
   }
 }
@@ -472,7 +472,7 @@ void launch_advection_tangent(Context& c, const of_grid& g, int scheme, const do
   const VP V = vp_of(g, vel[0], vel[1], vel[2]);
   const VP D = vp_of(g, dvel[0], dvel[1], dvel[2]);
   for(int comp = 0; comp < 3; ++comp)
-    hipLaunchKernelGGL(k_advection_tl, stride_blocks(V.L[comp].size()), 256, 0, c.stream, V, D, comp, g, scheme,
+    OFH_LAUNCH(k_advection_tl, stride_blocks(V.L[comp].size()), 256, 0, c.stream, V, D, comp, g, scheme,
                        out[comp]);
 }
 
@@ -480,7 +480,7 @@ void launch_advection_adjoint(Context& c, const of_grid& g, int scheme, const do
                               const double* const* abar, double* const* bar) {
   const VP V = vp_of(g, vel[0], vel[1], vel[2]);
   for(int comp = 0; comp < 3; ++comp)
-    hipLaunchKernelGGL(k_advection_adj, stride_blocks(V.L[comp].size()), 256, 0, c.stream, V, abar[comp], comp, g,
+    OFH_LAUNCH(k_advection_adj, stride_blocks(V.L[comp].size()), 256, 0, c.stream, V, abar[comp], comp, g,
                        scheme, bar[0], bar[1], bar[2]);
 }
 
@@ -536,8 +536,8 @@ struct AdjointRun {
     OFH_CHECK(hipStreamSynchronize(c.stream));
   }
   ~AdjointRun() {
-    (void)hipStreamSynchronize(c.stream);
-    for(double* p : owned) (void)hipFree(p);
+    OFH_RELEASE(hipStreamSynchronize(c.stream));
+    for(double* p : owned) OFH_RELEASE(hipFree(p));
   }
   double* alloc(size_t n) {
     double* p = nullptr;
@@ -553,7 +553,7 @@ struct AdjointRun {
   }
   void zero(double* p, size_t n) { OFH_CHECK(hipMemsetAsync(p, 0, n * sizeof(double), c.stream)); }
   void zero_inactive(double* p, const Lay& L) {
-    hipLaunchKernelGGL(k_zero_inactive, stride_blocks(L.size()), 256, 0, c.stream, p, L);
+    OFH_LAUNCH(k_zero_inactive, stride_blocks(L.size()), 256, 0, c.stream, p, L);
   }
   void axpy(const Lay& L, double a, const double* x, double* y) { launch_axpy_active(c, L, a, x, y); }
   double* state(int i) { return i < 3 ? fwd.vel[i] : i == 3 ? fwd.p_total : fwd.adv[i - 4]; }
@@ -691,7 +691,7 @@ struct AdjointRun {
       const Lay L = face(a);
       zero(ub[a], lay[1 + a]);
       axpy(L, 1.0, rb[a], ub[a]);
-      hipLaunchKernelGGL(k_lap_fc_adj, stride_blocks(L.size()), 256, 0, c.stream, rb[a], L, g, s_dif * nu, s_fc, alpha,
+      OFH_LAUNCH(k_lap_fc_adj, stride_blocks(L.size()), 256, 0, c.stream, rb[a], L, g, s_dif * nu, s_fc, alpha,
                          ub[a]);
     }
     const double* tbc[3] = {tb[0], tb[1], tb[2]};
@@ -709,7 +709,7 @@ struct AdjointRun {
         copy(ub[a], seed[a], lay[1 + a]);
       } else {
         const double invn = 1.0 / ((double)g.nx * g.ny * g.nz);
-        hipLaunchKernelGGL(k_ke_grad, stride_blocks(lay[1 + a]), 256, 0, c.stream, ufinal[a], face(a), cell(), a, invn,
+        OFH_LAUNCH(k_ke_grad, stride_blocks(lay[1 + a]), 256, 0, c.stream, ufinal[a], face(a), cell(), a, invn,
                            ub[a]);
       }
       zero(ab[a], lay[1 + a]);
@@ -833,7 +833,7 @@ struct AdjointRun {
       if(du_out && du_out[a]) copy(du_out[a], du[a], lay[1 + a]);
     if(dke) {
       const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz);
-      hipLaunchKernelGGL(k_ke_tangent, cl.grid, cl.block, 0, c.stream, fwd.vel[0], fwd.vel[1], fwd.vel[2], du[0], du[1],
+      OFH_LAUNCH(k_ke_tangent, cl.grid, cl.block, 0, c.stream, fwd.vel[0], fwd.vel[1], fwd.vel[2], du[0], du[1],
                          du[2], face(0), face(1), face(2), g, c.partials, cl.kz);
       launch_finalize(c.stream, c.partials, cl.grid, EpiStore<1>{red, 0u});
       double s = 0.0;

@@ -5,7 +5,9 @@
 #include "adjoint.hpp"
 #include "stepper.hpp"
 
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -37,15 +39,45 @@ void Context::prof_begin(const char*, hipEvent_t* end) {
 
 Context::~Context() {
   for(auto& e : prof_events) {
-    (void)hipEventDestroy(e.first);
-    (void)hipEventDestroy(e.second);
+    OFH_RELEASE(hipEventDestroy(e.first));
+    OFH_RELEASE(hipEventDestroy(e.second));
   }
   for(auto& s : scratch)
-    if(s.first) (void)hipFree(s.first);
-  if(partials) (void)hipFree(partials);
-  if(scal) (void)hipFree(scal);
-  if(host_scal) (void)hipHostFree(host_scal);
-  if(own_stream && stream) (void)hipStreamDestroy(stream);
+    if(s.first) OFH_RELEASE(hipFree(s.first));
+  if(partials) OFH_RELEASE(hipFree(partials));
+  if(scal) OFH_RELEASE(hipFree(scal));
+  if(host_scal) OFH_RELEASE(hipHostFree(host_scal));
+  if(own_stream && stream) OFH_RELEASE(hipStreamDestroy(stream));
+}
+
+namespace {
+thread_local std::string g_release_error;
+}
+
+void release_note(hipError_t e, const char* call) {
+  if(e != hipSuccess && g_release_error.empty())
+    g_release_error = std::string(call) + " failed: " + hipGetErrorString(e);
+}
+
+std::string release_take() {
+  (void)hipGetLastError();  // a failed release call is reported through the record, not later
+  std::string r;
+  r.swap(g_release_error);
+  return r;
+}
+
+int launch_check_level() {
+  static const int level = [] {
+    const char* v = std::getenv("OFH_CHECK_LAUNCHES");
+    return v ? std::atoi(v) : 0;
+  }();
+  return level;
+}
+
+void launch_check(const char* kernel) {
+  hipError_t e = hipGetLastError();
+  if(e == hipSuccess && launch_check_level() > 1) e = hipDeviceSynchronize();
+  if(e != hipSuccess) throw HipError(std::string("launch of ") + kernel + " failed: " + hipGetErrorString(e));
 }
 
 bool axis_periodic(const of_bc_set& b, int axis);
@@ -55,42 +87,72 @@ bool pressure_axis_periodic(const of_pressure_bc_set& b, int axis);
 
 using namespace ofh;
 
+// A context is shared by the caller's handle and every stepper or adjoint run created on it:
+// of_context_destroy drops the caller's reference and the native state goes with the last one,
+// so the order in which a caller (or a garbage collector) releases handles does not matter.
 struct of_context {
   Context ctx;
+  std::atomic<int> refs{1};
 };
+
+namespace {
+
+void context_retain(of_context* c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+
+// Drops one reference; the last one waits for the context stream and frees the context.
+void context_release(of_context* c) {
+  if(c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  OFH_RELEASE(hipSetDevice(c->ctx.device));
+  OFH_RELEASE(hipStreamSynchronize(c->ctx.stream));
+  delete c;
+}
+
+}  // namespace
 
 struct of_adjoint {
   AdjointRun* a;
+  of_context* owner;
   int device;
 };
 struct of_stepper {
   Stepper* s;
+  of_context* owner;
+  int device;
 };
 
 namespace {
 
 thread_local std::string g_error;
 
+// The body's own error is the one reported; a release failure during its unwinding is dropped.
+int failed(int code, const char* what) {
+  g_error = what;
+  (void)release_take();
+  return code;
+}
+
+// Runs an entry point's body: C++ exceptions become status codes plus the message, and a
+// release failure recorded while the body ran is reported as the call's HIP error.
 template <typename F>
 int guarded(F&& f) {
   try {
     f();
+    std::string rel = release_take();
+    if(!rel.empty()) {
+      g_error = rel;
+      return OF_HIP_ERROR;
+    }
     return OF_OK;
   } catch(const InvalidArgument& e) {
-    g_error = e.what();
-    return OF_INVALID_ARGUMENT;
+    return failed(OF_INVALID_ARGUMENT, e.what());
   } catch(const std::invalid_argument& e) {
-    g_error = e.what();
-    return OF_INVALID_ARGUMENT;
+    return failed(OF_INVALID_ARGUMENT, e.what());
   } catch(const HipError& e) {
-    g_error = e.what();
-    return OF_HIP_ERROR;
+    return failed(OF_HIP_ERROR, e.what());
   } catch(const std::exception& e) {
-    g_error = e.what();
-    return OF_RUNTIME_ERROR;
+    return failed(OF_RUNTIME_ERROR, e.what());
   } catch(...) {
-    g_error = "unknown error";
-    return OF_RUNTIME_ERROR;
+    return failed(OF_RUNTIME_ERROR, "unknown error");
   }
 }
 
@@ -157,7 +219,7 @@ int of_context_create(int device, void* stream, of_context** out) {
       }
       OFH_CHECK(hipMalloc(&x.partials, sizeof(double) * kMaxRed * kMaxPartials));
       OFH_CHECK(hipMalloc(&x.scal, sizeof(double) * kScalCount));
-      OFH_CHECK(hipMemset(x.scal, 0, sizeof(double) * kScalCount));
+      OFH_CHECK(hipMemsetAsync(x.scal, 0, sizeof(double) * kScalCount, x.stream));
       OFH_CHECK(hipHostMalloc(&x.host_scal, sizeof(double) * kScalCount, hipHostMallocDefault));
     } catch(...) {
       delete c;
@@ -170,9 +232,7 @@ int of_context_create(int device, void* stream, of_context** out) {
 int of_context_destroy(of_context* c) {
   return guarded([&] {
     if(c == nullptr) return;
-    (void)hipSetDevice(c->ctx.device);
-    (void)hipStreamSynchronize(c->ctx.stream);
-    delete c;
+    context_release(c);
   });
 }
 
@@ -570,13 +630,15 @@ int of_stepper_create(of_context* c, const of_stepper_config* cfg, of_stepper** 
   return guarded([&] {
     need(cfg, "config");
     need(out, "out");
-    auto* s = new of_stepper{nullptr};
+    Context& x = C(c);
+    auto* s = new of_stepper{nullptr, c, x.device};
     try {
-      s->s = new Stepper(C(c), *cfg);
+      s->s = new Stepper(x, *cfg);
     } catch(...) {
       delete s;
       throw;
     }
+    context_retain(c);
     *out = s;
   });
 }
@@ -584,8 +646,9 @@ int of_stepper_create(of_context* c, const of_stepper_config* cfg, of_stepper** 
 int of_stepper_destroy(of_stepper* s) {
   return guarded([&] {
     if(!s) return;
-    (void)hipSetDevice(s->s->c.device);
+    OFH_RELEASE(hipSetDevice(s->device));
     delete s->s;
+    context_release(s->owner);
     delete s;
   });
 }
@@ -593,7 +656,7 @@ int of_stepper_destroy(of_stepper* s) {
 int of_stepper_initialize(of_stepper* s, int convention) {
   return guarded([&] {
     need(s, "stepper");
-    OFH_CHECK(hipSetDevice(s->s->c.device));
+    OFH_CHECK(hipSetDevice(s->device));
     s->s->initialize(convention);
   });
 }
@@ -603,17 +666,22 @@ int of_stepper_upload(of_stepper* s, const double* u, const double* v, const dou
   return guarded([&] {
     need(s, "stepper");
     Stepper& S = *s->s;
-    OFH_CHECK(hipSetDevice(S.c.device));
+    OFH_CHECK(hipSetDevice(s->device));
     const double* vin[3] = {u, v, w};
     const double* ain[3] = {au, av, aw};
-    for(int a = 0; a < 3; ++a) {
-      need(vin[a], "velocity");
-      OFH_CHECK(hipMemcpy(S.vel[a], vin[a], fsize(S.g, OF_FACE_X + a), hipMemcpyHostToDevice));
-      if(ain[a]) OFH_CHECK(hipMemcpy(S.adv[a], ain[a], fsize(S.g, OF_FACE_X + a), hipMemcpyHostToDevice));
-      else OFH_CHECK(hipMemset(S.adv[a], 0, fsize(S.g, OF_FACE_X + a)));
-    }
+    for(int a = 0; a < 3; ++a) need(vin[a], "velocity");
     need(p, "p_total");
-    OFH_CHECK(hipMemcpy(S.p_total, p, fsize(S.g, OF_CELL), hipMemcpyHostToDevice));
+    // the copies go on the stepper's stream (non-blocking, so the legacy stream would not
+    // order them against its kernels) and the call returns when they are done
+    OFH_CHECK(hipStreamSynchronize(S.c.stream));
+    for(int a = 0; a < 3; ++a) {
+      const size_t n = fsize(S.g, OF_FACE_X + a);
+      OFH_CHECK(hipMemcpyAsync(S.vel[a], vin[a], n, hipMemcpyHostToDevice, S.c.stream));
+      if(ain[a]) OFH_CHECK(hipMemcpyAsync(S.adv[a], ain[a], n, hipMemcpyHostToDevice, S.c.stream));
+      else OFH_CHECK(hipMemsetAsync(S.adv[a], 0, n, S.c.stream));
+    }
+    OFH_CHECK(hipMemcpyAsync(S.p_total, p, fsize(S.g, OF_CELL), hipMemcpyHostToDevice, S.c.stream));
+    OFH_CHECK(hipStreamSynchronize(S.c.stream));
     S.has_prev = has_prev ? 1 : 0;
     if(m) S.metrics = *m;
     else {
@@ -628,7 +696,7 @@ int of_stepper_download(of_stepper* s, double* u, double* v, double* w, double* 
   return guarded([&] {
     need(s, "stepper");
     Stepper& S = *s->s;
-    OFH_CHECK(hipSetDevice(S.c.device));
+    OFH_CHECK(hipSetDevice(s->device));
     OFH_CHECK(hipStreamSynchronize(S.c.stream));
     double* vo[3] = {u, v, w};
     double* ao[3] = {au, av, aw};
@@ -662,7 +730,7 @@ int of_stepper_run(of_stepper* s, int steps, of_step_metrics* m) {
     need(s, "stepper");
     if(steps < 0) throw InvalidArgument("run_taylor_green_steps expects a non-negative step count");
     Stepper& S = *s->s;
-    OFH_CHECK(hipSetDevice(S.c.device));
+    OFH_CHECK(hipSetDevice(s->device));
     for(int i = 0; i < steps; ++i) S.step();
     if(m) *m = S.metrics;
   });
@@ -674,7 +742,7 @@ int of_stepper_kinetic_energy(of_stepper* s, double* out) {
   return guarded([&] {
     need(s, "stepper");
     need(out, "out");
-    OFH_CHECK(hipSetDevice(s->s->c.device));
+    OFH_CHECK(hipSetDevice(s->device));
     *out = s->s->kinetic_energy();
   });
 }
@@ -682,7 +750,7 @@ int of_stepper_kinetic_energy(of_stepper* s, double* out) {
 int of_stepper_set_forcing(of_stepper* s, const double* fu, const double* fv, const double* fw) {
   return guarded([&] {
     need(s, "stepper");
-    OFH_CHECK(hipSetDevice(s->s->c.device));
+    OFH_CHECK(hipSetDevice(s->device));
     const double* f[3] = {fu, fv, fw};
     s->s->set_forcing(fu || fv || fw ? f : nullptr);
   });
@@ -691,7 +759,7 @@ int of_stepper_set_forcing(of_stepper* s, const double* fu, const double* fv, co
 int of_stepper_set_predictor(of_stepper* s, int kind, double tol, int maxit) {
   return guarded([&] {
     need(s, "stepper");
-    OFH_CHECK(hipSetDevice(s->s->c.device));
+    OFH_CHECK(hipSetDevice(s->device));
     s->s->set_predictor(kind, tol, maxit);
   });
 }
@@ -801,13 +869,14 @@ int of_adjoint_create(of_context* c, const of_stepper_config* cfg, int every, of
     need(cfg, "config");
     need(out, "out");
     Context& x = C(c);
-    auto* h = new of_adjoint{nullptr, x.device};
+    auto* h = new of_adjoint{nullptr, c, x.device};
     try {
       h->a = adjoint_create(x, *cfg, every);
     } catch(...) {
       delete h;
       throw;
     }
+    context_retain(c);
     *out = h;
   });
 }
@@ -815,8 +884,9 @@ int of_adjoint_create(of_context* c, const of_stepper_config* cfg, int every, of
 int of_adjoint_destroy(of_adjoint* a) {
   return guarded([&] {
     if(!a) return;
-    (void)hipSetDevice(a->device);
+    OFH_RELEASE(hipSetDevice(a->device));
     adjoint_destroy(a->a);
+    context_release(a->owner);
     delete a;
   });
 }

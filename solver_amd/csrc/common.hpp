@@ -31,6 +31,28 @@ struct HipError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// Release paths (destructors and the *_destroy entry points) must not throw. They record the
+// first HIP call that failed with OFH_RELEASE and go on releasing; release_take() then clears
+// HIP's per-thread last error and hands the record to the entry point that ran them, which
+// reports it. A failed release therefore never surfaces later as the error of an unrelated
+// launch.
+void release_note(hipError_t e, const char* call);
+#define OFH_RELEASE(call) ::ofh::release_note((call), #call)
+// The first recorded release failure ("" if none); resets the record and HIP's last error.
+std::string release_take();
+
+// Every kernel launch goes through OFH_LAUNCH. With OFH_CHECK_LAUNCHES=1 in the environment
+// the launch status is read right after each launch, so a bad launch names its kernel; with
+// OFH_CHECK_LAUNCHES=2 the device is also synchronised after each launch, so a kernel that
+// fails while running is named too (slow: for finding a failure, not for timing).
+int launch_check_level();
+void launch_check(const char* kernel);
+#define OFH_LAUNCH(kernel, ...)                                                                           \
+  do {                                                                                                    \
+    hipLaunchKernelGGL(kernel, __VA_ARGS__);                                                              \
+    if(::ofh::launch_check_level() > 0) ::ofh::launch_check(#kernel);                                     \
+  } while(0)
+
 // ------------------------------------------------------------------ layout
 // Ghosted storage of core/field_layout.hpp:69-290 with one ghost layer.
 struct Lay {
@@ -171,7 +193,7 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_finalize(const double* __r
 
 template <class Epi>
 inline void launch_finalize(hipStream_t s, const double* partials, dim3 grid, const Epi& epi) {
-  hipLaunchKernelGGL(k_finalize<Epi>, dim3(1), dim3(kFinalizeThreads), 0, s, partials, grid.x * grid.y * grid.z,
+  OFH_LAUNCH(k_finalize<Epi>, dim3(1), dim3(kFinalizeThreads), 0, s, partials, grid.x * grid.y * grid.z,
                      epi);
 }
 

@@ -790,7 +790,7 @@ void launch_component_bc_mask(Context& c, const of_grid& g, const of_bc_set& b, 
     int e0, e1;
     slab_extents(g, axis, e0, e1);
     const int per = axis_periodic(b, axis) ? 1 : 0;
-    hipLaunchKernelGGL(k_velocity_bc_axis, slab_grid(e0, e1, block), block, 0, c.stream, u, v, w, Lu, Lv, Lw, axis,
+    OFH_LAUNCH(k_velocity_bc_axis, slab_grid(e0, e1, block), block, 0, c.stream, u, v, w, Lu, Lv, Lw, axis,
                        per, b.faces[2 * axis], b.faces[2 * axis + 1], mask);
   }
   OFH_CHECK(hipGetLastError());
@@ -811,7 +811,7 @@ void launch_pressure_bc(Context& c, const of_grid& g, const of_pressure_bc_set& 
   for(int axis = 0; axis < 3; ++axis) {
     const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
     const int per = pressure_axis_periodic(b, axis) ? 1 : 0;
-    hipLaunchKernelGGL(k_pressure_bc_axis, slab_grid(L.a(o0), L.a(o1), block), block, 0, c.stream, p, L, axis, per,
+    OFH_LAUNCH(k_pressure_bc_axis, slab_grid(L.a(o0), L.a(o1), block), block, 0, c.stream, p, L, axis, per,
                        b.faces[2 * axis], b.faces[2 * axis + 1], spacing(g, axis));
   }
   OFH_CHECK(hipGetLastError());
@@ -825,7 +825,7 @@ void launch_total_pressure_bc(Context& c, const of_grid& g, const of_bc_set& b, 
     const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
     const int per = axis_periodic(b, axis) ? 1 : 0;
     const double* src = axis == 0 ? su : axis == 1 ? sv : sw;
-    hipLaunchKernelGGL(k_total_pressure_bc_axis, slab_grid(L.a(o0), L.a(o1), block), block, 0, c.stream, p, L, src,
+    OFH_LAUNCH(k_total_pressure_bc_axis, slab_grid(L.a(o0), L.a(o1), block), block, 0, c.stream, p, L, src,
                        make_lay(g, OF_FACE_X + axis), axis, per, b.faces[2 * axis], b.faces[2 * axis + 1],
                        spacing(g, axis));
   }
@@ -838,7 +838,7 @@ void launch_gradient(Context& c, const of_grid& g, const double* p, double* gx, 
   for(int a = 0; a < 3; ++a) {
     const Lay L = make_lay(g, OF_FACE_X + a);
     const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
-    hipLaunchKernelGGL(k_gradient, cl.grid, cl.block, 0, c.stream, p, Lp, outs[a], L, a, spacing(g, a), cl.kz);
+    OFH_LAUNCH(k_gradient, cl.grid, cl.block, 0, c.stream, p, Lp, outs[a], L, a, spacing(g, a), cl.kz);
   }
   OFH_CHECK(hipGetLastError());
 }
@@ -848,7 +848,7 @@ void launch_divergence(Context& c, const of_grid& g, const double* u, const doub
   const ColumnLaunch cl = column_launch(Lc.ax, Lc.ay, Lc.az);
   {
     ProfScope prof_(c, "k_divergence", true);
-    hipLaunchKernelGGL(k_divergence, cl.grid, cl.block, 0, c.stream, u, v, w, make_lay(g, OF_FACE_X),
+    OFH_LAUNCH(k_divergence, cl.grid, cl.block, 0, c.stream, u, v, w, make_lay(g, OF_FACE_X),
                        make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), Lc, g, 1.0, 0, d, cl.kz);
   }
   OFH_CHECK(hipGetLastError());
@@ -860,7 +860,7 @@ void launch_divergence_scaled(Context& c, const of_grid& g, const double* u, con
   const ColumnLaunch cl = column_launch(Lc.ax, Lc.ay, Lc.az);
   {
     ProfScope prof_(c, "k_divergence", true);
-    hipLaunchKernelGGL(k_divergence, cl.grid, cl.block, 0, c.stream, u, v, w, make_lay(g, OF_FACE_X),
+    OFH_LAUNCH(k_divergence, cl.grid, cl.block, 0, c.stream, u, v, w, make_lay(g, OF_FACE_X),
                        make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), Lc, g, scale, 1, d, cl.kz);
   }
   OFH_CHECK(hipGetLastError());
@@ -869,7 +869,7 @@ void launch_divergence_scaled(Context& c, const of_grid& g, const double* u, con
 void launch_laplacian(Context& c, const of_grid& g, int loc, double scale, const double* in, double* out) {
   const Lay L = make_lay(g, loc);
   const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
-  hipLaunchKernelGGL(k_laplacian, cl.grid, cl.block, 0, c.stream, in, out, L, 1.0 / (g.dx * g.dx),
+  OFH_LAUNCH(k_laplacian, cl.grid, cl.block, 0, c.stream, in, out, L, 1.0 / (g.dx * g.dx),
                      1.0 / (g.dy * g.dy), 1.0 / (g.dz * g.dz), scale, scale != 1.0 ? 1 : 0, cl.kz);
   OFH_CHECK(hipGetLastError());
 }
@@ -881,7 +881,7 @@ void launch_advection(Context& c, const of_grid& g, int scheme, const double* u,
   for(int comp = 0; comp < 3; ++comp) {
     const Lay L = make_lay(g, OF_FACE_X + comp);
     const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
-    hipLaunchKernelGGL(k_advection, cl.grid, cl.block, 0, c.stream, V, comp, g, scheme, outs[comp], cl.kz);
+    OFH_LAUNCH(k_advection, cl.grid, cl.block, 0, c.stream, V, comp, g, scheme, outs[comp], cl.kz);
   }
   OFH_CHECK(hipGetLastError());
 }
@@ -889,7 +889,7 @@ void launch_advection(Context& c, const of_grid& g, int scheme, const double* u,
 void launch_cfl(Context& c, const of_grid& g, const double* u, const double* v, const double* w, double dt,
                 double* out5) {
   const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz);
-  hipLaunchKernelGGL(k_cfl, cl.grid, cl.block, 0, c.stream, u, v, w, make_lay(g, OF_FACE_X), make_lay(g, OF_FACE_Y),
+  OFH_LAUNCH(k_cfl, cl.grid, cl.block, 0, c.stream, u, v, w, make_lay(g, OF_FACE_X), make_lay(g, OF_FACE_Y),
                      make_lay(g, OF_FACE_Z), g, dt, c.partials, cl.kz);
   launch_finalize(c.stream, c.partials, cl.grid, EpiCfl{out5, dt});
   OFH_CHECK(hipGetLastError());
@@ -897,20 +897,20 @@ void launch_cfl(Context& c, const of_grid& g, const double* u, const double* v, 
 
 void launch_norms(Context& c, const Lay& L, const double* x, int slot) {
   const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
-  hipLaunchKernelGGL(k_norms, cl.grid, cl.block, 0, c.stream, x, L, c.partials, cl.kz);
+  OFH_LAUNCH(k_norms, cl.grid, cl.block, 0, c.stream, x, L, c.partials, cl.kz);
   launch_finalize(c.stream, c.partials, cl.grid, EpiStore<2>{c.scal + slot, 2u});
   OFH_CHECK(hipGetLastError());
 }
 
 void launch_scale_active(Context& c, const Lay& L, double scale, const double* in, double* out) {
   const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
-  hipLaunchKernelGGL(k_scale_active, cl.grid, cl.block, 0, c.stream, in, out, L, scale, cl.kz);
+  OFH_LAUNCH(k_scale_active, cl.grid, cl.block, 0, c.stream, in, out, L, scale, cl.kz);
   OFH_CHECK(hipGetLastError());
 }
 
 void launch_axpy_active(Context& c, const Lay& L, double a, const double* x, double* y) {
   const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
-  hipLaunchKernelGGL(k_axpy_active, cl.grid, cl.block, 0, c.stream, x, y, L, a, cl.kz);
+  OFH_LAUNCH(k_axpy_active, cl.grid, cl.block, 0, c.stream, x, y, L, a, cl.kz);
   OFH_CHECK(hipGetLastError());
 }
 
@@ -923,7 +923,7 @@ void launch_factorized_correction(Context& c, const of_grid& g, int variant, dou
     const Lay L = make_lay(g, OF_FACE_X + comp);
     OFH_CHECK(hipMemsetAsync(outs[comp], 0, L.size() * sizeof(double), c.stream));
     const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
-    hipLaunchKernelGGL(k_fc, cl.grid, cl.block, 0, c.stream, ins[comp], L, P, outs[comp], cl.kz);
+    OFH_LAUNCH(k_fc, cl.grid, cl.block, 0, c.stream, ins[comp], L, P, outs[comp], cl.kz);
   }
   OFH_CHECK(hipGetLastError());
 }
@@ -962,10 +962,10 @@ void launch_predictor_rhs(Context& c, const of_grid& g, int variant, int scheme,
     ProfScope prof_(c, "k_predictor_rhs", true);
     if(variant == 0) {
       const unsigned gx = div_up(L.ax, kRX), gy = div_up(L.ay, kRY), gz = div_up(L.az, kRKZ);
-      hipLaunchKernelGGL(k_predictor_rhs_tiled, dim3(gx * gy * gz), dim3(kRX, kRY), 0, c.stream, V, p,
+      OFH_LAUNCH(k_predictor_rhs_tiled, dim3(gx * gy * gz), dim3(kRX, kRY), 0, c.stream, V, p,
                          make_lay(g, OF_CELL), prev[comp], adv[comp], rhs[comp], P);
     } else {
-      hipLaunchKernelGGL(k_predictor_rhs, cl.grid, cl.block, 0, c.stream, V, p, make_lay(g, OF_CELL), prev[comp],
+      OFH_LAUNCH(k_predictor_rhs, cl.grid, cl.block, 0, c.stream, V, p, make_lay(g, OF_CELL), prev[comp],
                          adv[comp], rhs[comp], P, cl.kz);
     }
   }
@@ -982,7 +982,7 @@ void launch_correct(Context& c, const of_grid& g, const double* u, const double*
     const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
     {
       ProfScope prof_(c, "k_correct", true);
-      hipLaunchKernelGGL(k_correct, cl.grid, cl.block, 0, c.stream, ins[a], L, phi, Lp, a, spacing(g, a), scale,
+      OFH_LAUNCH(k_correct, cl.grid, cl.block, 0, c.stream, ins[a], L, phi, Lp, a, spacing(g, a), scale,
                          outs[a], cl.kz);
     }
   }

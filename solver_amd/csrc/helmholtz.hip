@@ -645,10 +645,10 @@ int launch_axis_sweeps(Context& c, const of_grid& g, const AdiPlan& P, int axis,
   for(int t = 0; t < B.ntasks; ++t) B.t[t].ck = ck + ck_off[t];
   if(axis == 0) {
     ProfScope prof_(c, "k_line_solve_x", true);
-    hipLaunchKernelGGL(k_line_solve_x, dim3(blocks), dim3(64), 0, c.stream, B);
+    OFH_LAUNCH(k_line_solve_x, dim3(blocks), dim3(64), 0, c.stream, B);
   } else {
     ProfScope prof_(c, "k_line_solve_s", true);
-    hipLaunchKernelGGL(k_line_solve_s, dim3(blocks), dim3(64, kLSY), 0, c.stream, B, axis);
+    OFH_LAUNCH(k_line_solve_s, dim3(blocks), dim3(64, kLSY), 0, c.stream, B, axis);
   }
   return mask;
 }
@@ -720,7 +720,7 @@ void predictor_adi_adjoint(Context& c, const of_grid& g, const of_bc_set& b, dou
       const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
       const dim3 block(64, 4, 1);
       const dim3 grid((unsigned)((L.a(o0) + 63) / 64), (unsigned)((L.a(o1) + 3) / 4), 1);
-      hipLaunchKernelGGL(k_dup_fold, grid, block, 0, c.stream, rbar[p.comp], L, axis, p.S.b, p.S.e);
+      OFH_LAUNCH(k_dup_fold, grid, block, 0, c.stream, rbar[p.comp], L, axis, p.S.b, p.S.e);
     }
     const int mask = launch_axis_sweeps(c, g, P, axis, out, true);
     for(int comp = 0; comp < 3; ++comp)

@@ -304,12 +304,12 @@ struct HelmholtzCG {
     for(auto& e : ev) OFH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   ~HelmholtzCG() {
-    (void)hipStreamSynchronize(c.stream);
+    OFH_RELEASE(hipStreamSynchronize(c.stream));
     for(double* p : {x, r, d, q, hs})
-      if(p) (void)hipFree(p);
-    if(host_flag) (void)hipHostFree(host_flag);
+      if(p) OFH_RELEASE(hipFree(p));
+    if(host_flag) OFH_RELEASE(hipHostFree(host_flag));
     for(auto& e : ev)
-      if(e) (void)hipEventDestroy(e);
+      if(e) OFH_RELEASE(hipEventDestroy(e));
   }
 };
 
@@ -346,28 +346,28 @@ int predictor_cg(HelmholtzCG& S, const of_grid& g, const of_bc_set& b, double al
   const unsigned tiles = (unsigned)(((H.nx + 63) / 64) * ((H.ny + 3) / 4) * ((H.nz + kHcgKZ - 1) / kHcgKZ));
   if(tiles * 3u > (unsigned)kMaxPartials) throw InvalidArgument("grid too large for the reduction buffer");
   const dim3 grid(tiles, 3, 1);
-  hipLaunchKernelGGL(k_hcg_gather, grid, block, 0, c.stream, ou, ov, ow, Lu, Lv, Lw, H, S.x);
+  OFH_LAUNCH(k_hcg_gather, grid, block, 0, c.stream, ou, ov, ow, Lu, Lv, Lw, H, S.x);
   OFH_CHECK(hipMemsetAsync(S.hs, 0, 3 * kHcgSlots * sizeof(double), c.stream));
   S.host_flag[0] = S.host_flag[1] = 0.0;
-  hipLaunchKernelGGL((k_hcg_apply<false>), grid, block, 0, c.stream, S.x, S.q, H, c.partials, S.hs);
-  hipLaunchKernelGGL(k_hcg_residual0, grid, block, 0, c.stream, S.x, S.q, S.r, S.d, H, c.partials);
+  OFH_LAUNCH((k_hcg_apply<false>), grid, block, 0, c.stream, S.x, S.q, H, c.partials, S.hs);
+  OFH_LAUNCH(k_hcg_residual0, grid, block, 0, c.stream, S.x, S.q, S.r, S.d, H, c.partials);
   launch_finalize(c.stream, c.partials, grid, EpiHcgInit{S.hs});
   for(int it = 0; it < max_iterations; ++it) {
     {
       ProfScope prof_(c, "k_hcg_apply");
-      hipLaunchKernelGGL((k_hcg_apply<true>), grid, block, 0, c.stream, S.d, S.q, H, c.partials, S.hs);
+      OFH_LAUNCH((k_hcg_apply<true>), grid, block, 0, c.stream, S.d, S.q, H, c.partials, S.hs);
     }
     launch_finalize(c.stream, c.partials, grid, EpiHcgAlpha{S.hs});
-    hipLaunchKernelGGL(k_hcg_update, grid, block, 0, c.stream, S.x, S.r, S.d, S.q, H, c.partials, S.hs);
+    OFH_LAUNCH(k_hcg_update, grid, block, 0, c.stream, S.x, S.r, S.d, S.q, H, c.partials, S.hs);
     launch_finalize(c.stream, c.partials, grid, EpiHcgBeta{S.hs, S.host_flag_dev + (it % 2), tol});
-    hipLaunchKernelGGL(k_hcg_direction, grid, block, 0, c.stream, S.r, S.d, H, S.hs);
+    OFH_LAUNCH(k_hcg_direction, grid, block, 0, c.stream, S.r, S.d, H, S.hs);
     OFH_CHECK(hipEventRecord(S.ev[it % 2], c.stream));
     if(it > 0) {  // one iteration of look-ahead: stop once the previous iteration stopped all three
       OFH_CHECK(hipEventSynchronize(S.ev[(it - 1) % 2]));
       if(reinterpret_cast<volatile double*>(S.host_flag)[(it - 1) % 2] != 0.0) break;
     }
   }
-  hipLaunchKernelGGL(k_hcg_scatter, grid, block, 0, c.stream, S.x, H, ou, ov, ow, Lu, Lv, Lw);
+  OFH_LAUNCH(k_hcg_scatter, grid, block, 0, c.stream, S.x, H, ou, ov, ow, Lu, Lv, Lw);
   launch_velocity_bc(c, g, b, ou, ov, ow);
   OFH_CHECK(hipGetLastError());
   double h[3 * kHcgSlots];

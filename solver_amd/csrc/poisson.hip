@@ -2011,14 +2011,14 @@ int resident_blocks(K kernel, int threads, int device) {
   do {                                                                                        \
     if((h).kz == 8) {                                                                         \
       if((h).uni)                                                                             \
-        hipLaunchKernelGGL((KERN, 8, true>), (h).grid, (h).block, 0, stream, __VA_ARGS__);    \
+        OFH_LAUNCH((KERN, 8, true>), (h).grid, (h).block, 0, stream, __VA_ARGS__);    \
       else                                                                                    \
-        hipLaunchKernelGGL((KERN, 8, false>), (h).grid, (h).block, 0, stream, __VA_ARGS__);   \
+        OFH_LAUNCH((KERN, 8, false>), (h).grid, (h).block, 0, stream, __VA_ARGS__);   \
     } else {                                                                                  \
       if((h).uni)                                                                             \
-        hipLaunchKernelGGL((KERN, 2, true>), (h).grid, (h).block, 0, stream, __VA_ARGS__);    \
+        OFH_LAUNCH((KERN, 2, true>), (h).grid, (h).block, 0, stream, __VA_ARGS__);    \
       else                                                                                    \
-        hipLaunchKernelGGL((KERN, 2, false>), (h).grid, (h).block, 0, stream, __VA_ARGS__);   \
+        OFH_LAUNCH((KERN, 2, false>), (h).grid, (h).block, 0, stream, __VA_ARGS__);   \
     }                                                                                         \
   } while(0)
 
@@ -2252,7 +2252,7 @@ void factor_coarse(const PLevel& L, bool zm, std::vector<int>& perm, std::vector
 
 PoissonSolver::~PoissonSolver() {
   auto fr = [](void* p) {
-    if(p) (void)hipFree(p);
+    if(p) OFH_RELEASE(hipFree(p));
   };
   fr(pc);
   fr(r);
@@ -2270,9 +2270,9 @@ PoissonSolver::~PoissonSolver() {
   fr(U);
   fr(work);
   fr(dlv);
-  if(host_ps) (void)hipHostFree(host_ps);
+  if(host_ps) OFH_RELEASE(hipHostFree(host_ps));
   for(auto& e : ev)
-    if(e) (void)hipEventDestroy(e);
+    if(e) OFH_RELEASE(hipEventDestroy(e));
 }
 
 void PoissonSolver::build() {
@@ -2392,10 +2392,10 @@ void PoissonSolver::vcycle() {
       {
         ProfScope prof_(c, "k_down_fused", l == 0);
         if(rows_down == 16)
-          hipLaunchKernelGGL((k_down_fused<T, 16>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<T*>(lxa[l]),
+          OFH_LAUNCH((k_down_fused<T, 16>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<T*>(lxa[l]),
                              P, C, static_cast<T*>(lb[l + 1]), out, ps, slot(l), zmi, f.kz, pend);
         else
-          hipLaunchKernelGGL((k_down_fused<T, 8>), f.grid, dim3(256), 0, c.stream, bptr(l), static_cast<T*>(lxa[l]),
+          OFH_LAUNCH((k_down_fused<T, 8>), f.grid, dim3(256), 0, c.stream, bptr(l), static_cast<T*>(lxa[l]),
                              P, C, static_cast<T*>(lb[l + 1]), out, ps, slot(l), zmi, f.kz, pend);
       }
       const bool consumer_folds = (l + 1 == l0 || fused[l + 1]) && foldable(f.grid);
@@ -2441,17 +2441,17 @@ void PoissonSolver::vcycle() {
     const bool tuni = lv[l0].per[0] && lv[l0].per[1] && lv[l0].per[2];
     if(in_lds) {
       if(tuni)
-        hipLaunchKernelGGL((k_vtail<T, true, true>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
+        OFH_LAUNCH((k_vtail<T, true, true>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
                            factor, U, work, ps, zmi, pend);
       else
-        hipLaunchKernelGGL((k_vtail<T, true, false>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
+        OFH_LAUNCH((k_vtail<T, true, false>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
                            factor, U, work, ps, zmi, pend);
     } else {
       if(tuni)
-        hipLaunchKernelGGL((k_vtail<T, false, true>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
+        OFH_LAUNCH((k_vtail<T, false, true>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb, perm,
                            factor, U, work, ps, zmi, pend);
       else
-        hipLaunchKernelGGL((k_vtail<T, false, false>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb,
+        OFH_LAUNCH((k_vtail<T, false, false>), dim3(1), dim3(kTailThreads), 0, c.stream, dlv, l0, nl, tb,
                            perm, factor, U, work, ps, zmi, pend);
     }
   }
@@ -2467,11 +2467,11 @@ void PoissonSolver::vcycle() {
       {
         ProfScope prof_(c, "k_up_fused", l == 0);
         if(rows_up == 16)
-          hipLaunchKernelGGL((k_up_fused<T, 16>), f.grid, dim3(512), 0, c.stream, bptr(l),
+          OFH_LAUNCH((k_up_fused<T, 16>), f.grid, dim3(512), 0, c.stream, bptr(l),
                              static_cast<const T*>(lxa[l]), xc, P, C, static_cast<T*>(lxb[l]), out, ps, slot(l),
                              slot(l + 1), f.kz, up_src, up_nb, zmi);
         else
-          hipLaunchKernelGGL((k_up_fused<T, 8>), f.grid, dim3(256), 0, c.stream, bptr(l),
+          OFH_LAUNCH((k_up_fused<T, 8>), f.grid, dim3(256), 0, c.stream, bptr(l),
                              static_cast<const T*>(lxa[l]), xc, P, C, static_cast<T*>(lxb[l]), out, ps, slot(l),
                              slot(l + 1), f.kz, up_src, up_nb, zmi);
       }
@@ -2523,17 +2523,17 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   const unsigned pgrid = std::min<unsigned>(div_up((size_t)F.n, 256 * kPointIlp), 8192u);
   OFH_CHECK(hipMemsetAsync(ps, 0, P_COUNT * sizeof(double), c.stream));
   if(zero_guess)
-    hipLaunchKernelGGL(k_setup1_zero, ll.grid, ll.block, 0, c.stream, rhs, Lc, F, r, pc, c.partials, ll.kz);
+    OFH_LAUNCH(k_setup1_zero, ll.grid, ll.block, 0, c.stream, rhs, Lc, F, r, pc, c.partials, ll.kz);
   else
-    hipLaunchKernelGGL(k_setup1, ll.grid, ll.block, 0, c.stream, rhs, p, Lc, F, r, pc, c.partials, ll.kz);
+    OFH_LAUNCH(k_setup1, ll.grid, ll.block, 0, c.stream, rhs, p, Lc, F, r, pc, c.partials, ll.kz);
   launch_finalize(c.stream, c.partials, ll.grid, EpiSetup1{ps, n0, zmi});
   if(zero_guess)
-    hipLaunchKernelGGL(k_setup2_zero, ll.grid, ll.block, 0, c.stream, r, F, c.partials, ps, ll.kz);
+    OFH_LAUNCH(k_setup2_zero, ll.grid, ll.block, 0, c.stream, r, F, c.partials, ps, ll.kz);
   else
-    hipLaunchKernelGGL(k_setup2, ll.grid, ll.block, 0, c.stream, r, pc, F, c.partials, ps, ll.kz);
+    OFH_LAUNCH(k_setup2, ll.grid, ll.block, 0, c.stream, r, pc, F, c.partials, ps, ll.kz);
   launch_finalize(c.stream, c.partials, ll.grid, EpiSetup2{ps, history, n0, tol, zmi});
   if(mixed)  // FP32 copy of the initial residual: the rhs of the finest FP32 level
-    hipLaunchKernelGGL(k_cast<T>, 2048, 256, 0, c.stream, r, static_cast<T*>(lb[0]), F.n, ps);
+    OFH_LAUNCH(k_cast<T>, 2048, 256, 0, c.stream, r, static_cast<T*>(lb[0]), F.n, ps);
   vcycle<T>();
   int it = 0;
   for(; it < maxit; ++it) {
@@ -2544,20 +2544,20 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
     } else {
       {
         ProfScope prof_(c, "k_direction");
-        hipLaunchKernelGGL(k_direction<T>, dim3(pgrid), dim3(256), 0, c.stream, static_cast<const T*>(x_final(0)),
+        OFH_LAUNCH(k_direction<T>, dim3(pgrid), dim3(256), 0, c.stream, static_cast<const T*>(x_final(0)),
                            d[(it + 1) % 2], d[it % 2], pc, F.n, ps, it == 0 ? 1 : 0);
       }
       ProfScope prof_(c, "k_apply_dot");
       if(hf.kz == 8) {
         if(hf.uni)
-          hipLaunchKernelGGL((k_apply_dot<8, true>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+          OFH_LAUNCH((k_apply_dot<8, true>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
         else
-          hipLaunchKernelGGL((k_apply_dot<8, false>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+          OFH_LAUNCH((k_apply_dot<8, false>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
       } else {
         if(hf.uni)
-          hipLaunchKernelGGL((k_apply_dot<2, true>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+          OFH_LAUNCH((k_apply_dot<2, true>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
         else
-          hipLaunchKernelGGL((k_apply_dot<2, false>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+          OFH_LAUNCH((k_apply_dot<2, false>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
       }
     }
     launch_finalize(c.stream, c.partials, hf.grid, EpiDir{ps, n0, zmi, it % 2});
@@ -2577,10 +2577,10 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   }
   {
     const dim3 pg(2048);
-    hipLaunchKernelGGL(k_pfinal, pg, dim3(256), 0, c.stream, pc, d[0], d[1], F.n, c.partials, ps);
+    OFH_LAUNCH(k_pfinal, pg, dim3(256), 0, c.stream, pc, d[0], d[1], F.n, c.partials, ps);
     launch_finalize(c.stream, c.partials, pg, EpiPfinal{ps, n0, zmi});
   }
-  hipLaunchKernelGGL(k_pout, ll.grid, ll.block, 0, c.stream, pc, F, Lc, p, ps, ll.kz);
+  OFH_LAUNCH(k_pout, ll.grid, ll.block, 0, c.stream, pc, F, Lc, p, ps, ll.kz);
   OFH_CHECK(hipGetLastError());
   std::vector<double> hps(P_COUNT);
   OFH_CHECK(hipMemcpyAsync(hps.data(), ps, P_COUNT * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -2632,7 +2632,7 @@ void launch_poisson_apply(Context& c, const of_grid& g, const of_pressure_bc_set
                           const double* in, double* out) {
   const PLevel L = make_level(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, b);
   const LevelLaunch ll = level_launch(L);
-  hipLaunchKernelGGL(k_poisson_apply_g, ll.grid, ll.block, 0, c.stream, in, out, make_lay(g, OF_CELL), L,
+  OFH_LAUNCH(k_poisson_apply_g, ll.grid, ll.block, 0, c.stream, in, out, make_lay(g, OF_CELL), L,
                      homogeneous ? 1 : 0, ll.kz);
   OFH_CHECK(hipGetLastError());
 }

@@ -247,7 +247,7 @@ Stepper::Stepper(Context& ctx, const of_stepper_config& cfg) : c(ctx), cfg(cfg) 
 Stepper::~Stepper() {
   if(poisson) poisson_destroy(poisson);
   if(hcg) helmholtz_cg_destroy(hcg);
-  for(double* p : owned) (void)hipFree(p);
+  for(double* p : owned) OFH_RELEASE(hipFree(p));
 }
 
 void Stepper::set_predictor(int kind, double tol, int maxit) {
@@ -301,7 +301,7 @@ void Stepper::initialize(int convention) {
       double* f = loc == OF_CELL ? p_total : vel[loc - 1];
       const dim3 block(64, 4, 1);
       const dim3 grid((unsigned)((L.ax + 63) / 64), (unsigned)((L.ay + 3) / 4), 1);
-      hipLaunchKernelGGL(k_tg_init, grid, block, 0, c.stream, f, L, loc, g, three_d, convention);
+      OFH_LAUNCH(k_tg_init, grid, block, 0, c.stream, f, L, loc, g, three_d, convention);
     }
   } else if(cfg.flow_case != OF_CASE_LID_DRIVEN_CAVITY) {
     const int pois = cfg.flow_case == OF_CASE_CHANNEL_POISEUILLE;
@@ -310,7 +310,7 @@ void Stepper::initialize(int convention) {
       double* f = loc == OF_CELL ? p_total : vel[0];
       const dim3 block(64, 4, 1);
       const dim3 grid((unsigned)((L.ax + 63) / 64), (unsigned)((L.ay + 3) / 4), 1);
-      hipLaunchKernelGGL(k_channel_init, grid, block, 0, c.stream, f, L, loc, g, pois, cfg.lid_velocity,
+      OFH_LAUNCH(k_channel_init, grid, block, 0, c.stream, f, L, loc, g, pois, cfg.lid_velocity,
                          cfg.pressure_drop, cfg.viscosity);
     }
   }
@@ -365,7 +365,7 @@ void Stepper::step() {
     const ColumnLaunch cl = column_launch(L.ax, L.ay, L.az);
     {
       ProfScope prof_(c, "k_pressure_update", true);
-      hipLaunchKernelGGL(k_pressure_update, cl.grid, cl.block, 0, c.stream, p_total, phi, prhs, L, -0.5 * nu * dt, cl.kz);
+      OFH_LAUNCH(k_pressure_update, cl.grid, cl.block, 0, c.stream, p_total, phi, prhs, L, -0.5 * nu * dt, cl.kz);
     }
   }
   total_pressure_bc(corr);
@@ -373,7 +373,7 @@ void Stepper::step() {
     const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz);
     {
       ProfScope prof_(c, "k_step_diag", true);
-      hipLaunchKernelGGL(k_step_diag, cl.grid, cl.block, 0, c.stream, corr[0], corr[1], corr[2], vel[0], vel[1], vel[2],
+      OFH_LAUNCH(k_step_diag, cl.grid, cl.block, 0, c.stream, corr[0], corr[1], corr[2], vel[0], vel[1], vel[2],
                          make_lay(g, OF_FACE_X), make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), g, dt, c.partials,
                          cl.kz);
     }
@@ -406,7 +406,7 @@ void Stepper::step() {
 
 double Stepper::kinetic_energy() {
   const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz);
-  hipLaunchKernelGGL(k_kinetic, cl.grid, cl.block, 0, c.stream, vel[0], vel[1], vel[2], make_lay(g, OF_FACE_X),
+  OFH_LAUNCH(k_kinetic, cl.grid, cl.block, 0, c.stream, vel[0], vel[1], vel[2], make_lay(g, OF_FACE_X),
                      make_lay(g, OF_FACE_Y), make_lay(g, OF_FACE_Z), g, c.partials, cl.kz);
   launch_finalize(c.stream, c.partials, cl.grid, EpiStore<1>{diag + 16, 0u});
   double e = 0.0;

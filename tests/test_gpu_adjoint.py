@@ -29,7 +29,9 @@ LOCS = (sa.FACE_X, sa.FACE_Y, sa.FACE_Z)
 
 @pytest.fixture(scope="module")
 def ctx():
-    return sa.Context(0)
+    c = sa.Context(0)
+    yield c
+    c.close()
 
 
 def vdot(a, b):

@@ -32,7 +32,9 @@ CASES = {"couette": sa.CHANNEL_COUETTE, "poiseuille": sa.CHANNEL_POISEUILLE}
 
 @pytest.fixture(scope="module")
 def ctx():
-    return sa.Context(0)
+    c = sa.Context(0)
+    yield c
+    c.close()
 
 
 def oracle_config(p, **over):

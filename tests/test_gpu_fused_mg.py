@@ -20,7 +20,9 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def ctx():
-    return sa.Context(0)
+    c = sa.Context(0)
+    yield c
+    c.close()
 
 
 def solve(ctx, g, rhs, precision, tol, unfused):

@@ -22,7 +22,9 @@ IT = 1e-10  # iterative solves
 
 @pytest.fixture(scope="module")
 def ctx():
-    return sa.Context(0)
+    c = sa.Context(0)
+    yield c
+    c.close()
 
 
 def vin(f, pre=""):
@@ -277,6 +279,7 @@ def test_taylor_green_nonconverged_message(ctx):
     st.initialize(0)
     with pytest.raises(sa.SolverError) as e:
         st.run(1)
+    st.close()
     ref = f.meta["error"]
     assert "Taylor-Green pressure solve did not converge at step 1 (iterations=1" in str(e.value)
     assert str(e.value).split(", relative_residual")[0] == ref.split(", relative_residual")[0]

@@ -24,7 +24,9 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def ctx():
-    return sa.Context(0)
+    c = sa.Context(0)
+    yield c
+    c.close()
 
 
 def tg_case(n, nz, viscosity, cfl, steps, tol):
