@@ -231,6 +231,22 @@ int of_project_velocity(of_context* ctx, const of_grid* grid, const of_bc_set* b
 int of_solve_pressure_poisson(of_context* ctx, const of_grid* grid, const of_pressure_bc_set* bcs,
                               const of_projection_options* opts, int precision, const double* rhs, double* p,
                               of_poisson_diagnostics* diag, double* residual_history, int history_capacity);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP stream capture, tight serving loops, harnesses that forbid hipMalloc): the solve of
+ * of_solve_pressure_poisson with every device array (multigrid hierarchy, CG vectors,
+ * reduction partials, scalars, residual history, coarse factors) carved from the library's
+ * fixed scratch block of OF_NOALLOC_SCRATCH_BYTES per device, so that the call allocates and
+ * frees no device or pinned host memory. It runs on `stream` (NULL: the null stream) of HIP
+ * device `device`, reads the stop flag back with a copy and a stream synchronisation after
+ * each iteration, and returns when the solve is done. Grids of more than OF_NOALLOC_MAX_CELLS
+ * cells, and solves whose arrays do not fit the block (e.g. a very large coarsest system),
+ * return OF_INVALID_ARGUMENT. Every call on one device uses that device's one block, which
+ * keeps the last call's contents: calls that overlap on one device race. */
+#define OF_NOALLOC_MAX_CELLS 262144
+#define OF_NOALLOC_SCRATCH_BYTES (32u << 20)
+int of_solve_pressure_poisson_noalloc(int device, void* stream, const of_grid* grid, const of_pressure_bc_set* bcs,
+                                      const of_projection_options* opts, int precision, const double* rhs, double* p,
+                                      of_poisson_diagnostics* diag);
 /* linsolve/poisson_solver.hpp:30-32 build_poisson_rhs_from_pressure */
 int of_build_poisson_rhs_from_pressure(of_context* ctx, const of_grid* grid, const of_pressure_bc_set* bcs,
                                        const double* p, double* rhs);

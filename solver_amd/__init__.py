@@ -108,7 +108,8 @@ EXPORTS = [
     "of_compute_diffusion_term", "of_compute_advective_cfl", "of_compute_factorized_correction",
     "of_assemble_predictor_rhs", "of_solve_predictor_adi", "of_solve_predictor_cg", "of_build_pressure_rhs",
     "of_correct_velocity",
-    "of_project_velocity", "of_solve_pressure_poisson", "of_build_poisson_rhs_from_pressure",
+    "of_project_velocity", "of_solve_pressure_poisson", "of_solve_pressure_poisson_noalloc",
+    "of_build_poisson_rhs_from_pressure",
     "of_apply_poisson_operator", "of_taylor_green_dt", "of_lid_driven_cavity_dt", "of_channel_flow_dt",
     "of_stepper_create",
     "of_stepper_destroy", "of_stepper_initialize", "of_stepper_upload", "of_stepper_download",

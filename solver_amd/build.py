@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsolver_amd.so")
 
-SOURCES = ["fields.hip", "helmholtz.hip", "helmholtz_cg.hip", "poisson.hip", "stepper.hip", "capi.hip", "adjoint.hip"]
+SOURCES = ["fields.hip", "helmholtz.hip", "helmholtz_cg.hip", "poisson.hip", "stepper.hip", "capi.hip", "adjoint.hip", "noalloc.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

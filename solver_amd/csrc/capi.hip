@@ -38,6 +38,7 @@ void Context::prof_begin(const char*, hipEvent_t* end) {
 }
 
 Context::~Context() {
+  if(borrowed) return;
   for(auto& e : prof_events) {
     OFH_RELEASE(hipEventDestroy(e.first));
     OFH_RELEASE(hipEventDestroy(e.second));
@@ -194,6 +195,8 @@ AdjointRun& A(of_adjoint* a) {
 }
 
 }  // namespace
+
+int ofh::guarded_call(const std::function<void()>& body) { return guarded(body); }
 
 extern "C" {
 

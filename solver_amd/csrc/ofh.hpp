@@ -3,16 +3,38 @@
 
 #include "common.hpp"
 
+#include <functional>
 #include <string>
 #include <vector>
 
 namespace ofh {
 
+// Runs the body of an extern "C" entry point: exceptions become status codes plus the message
+// of_last_error returns, and a release failure recorded meanwhile is reported (capi.hip).
+int guarded_call(const std::function<void()>& body);
+
 // ------------------------------------------------------------------ device context
+// Device memory handed out from one block the caller provides (nothing is allocated): 256-byte
+// aligned pieces, never freed one by one. take() throws InvalidArgument when the block is used up.
+struct DeviceArena {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  void* take(size_t bytes) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if(at + bytes > cap)
+      throw InvalidArgument("the solve needs more than the " + std::to_string(cap >> 20) +
+                            " MiB of no-allocation scratch");
+    used = at + bytes;
+    return base + at;
+  }
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  bool borrowed = false;  // partials and scal belong to someone else: the destructor frees nothing
   double* partials = nullptr;   // kMaxRed x kMaxPartials reduction partials
   double* scal = nullptr;       // device scalars
   double* host_scal = nullptr;  // pinned mirror for read-back
@@ -111,6 +133,10 @@ struct PoissonResult {
 };
 class PoissonSolver;  // MGPCG on the device, reused across solves of one grid/BC set
 PoissonSolver* poisson_create(Context& c, const of_grid& g, const of_pressure_bc_set& b, int precision);
+// The same, with every device array carved from `arena` and no allocation anywhere (the solve
+// reads its stop flag back by a copy and a stream synchronisation per iteration).
+PoissonSolver* poisson_create_in(Context& c, const of_grid& g, const of_pressure_bc_set& b, int precision,
+                                 DeviceArena& arena);
 void poisson_destroy(PoissonSolver* s);
 // Solves -lap(p) = rhs (reference sign conventions); rhs and p in ghosted cell storage.
 // p holds the warm start on entry (zero_guess: the warm start is zero and p is not read). Only

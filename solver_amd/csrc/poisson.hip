@@ -2054,9 +2054,18 @@ class PoissonSolver {
   double *factor = nullptr, *U = nullptr, *work = nullptr;
   int hist_cap = 0;
   double omega = kOmega;  // Jacobi weight of every level's first pre-smoothing sweep
+  // Device memory source: null -> hipMalloc per array (freed by the destructor); otherwise every
+  // array is carved from the arena and nothing is allocated or freed. With an arena the host
+  // reads the stop flag from device memory after each iteration (no mapped pinned mirror and no
+  // events), so the solve allocates nothing at all.
+  DeviceArena* arena = nullptr;
+  void dev_alloc(void** out, size_t bytes) {
+    if(arena) *out = arena->take(bytes);
+    else OFH_CHECK(hipMalloc(out, bytes));
+  }
 
-  PoissonSolver(Context& ctx, const of_grid& grid, const of_pressure_bc_set& b, int prec)
-      : c(ctx), g(grid), bcs(b), precision(prec) {}
+  PoissonSolver(Context& ctx, const of_grid& grid, const of_pressure_bc_set& b, int prec, DeviceArena* a = nullptr)
+      : c(ctx), g(grid), bcs(b), precision(prec), arena(a) {}
   ~PoissonSolver();
   void build();
   template <typename T>
@@ -2251,6 +2260,7 @@ void factor_coarse(const PLevel& L, bool zm, std::vector<int>& perm, std::vector
 }  // namespace
 
 PoissonSolver::~PoissonSolver() {
+  if(arena) return;  // the arena's owner keeps its memory
   auto fr = [](void* p) {
     if(p) OFH_RELEASE(hipFree(p));
   };
@@ -2328,43 +2338,50 @@ void PoissonSolver::build() {
                                 : resident_blocks(k_up_fused<double, 8>, 256, c.device);
   }
   const size_t n0 = (size_t)lv[0].n;
-  OFH_CHECK(hipMalloc(&pc, n0 * sizeof(double)));
-  OFH_CHECK(hipMalloc(&r, n0 * sizeof(double)));
-  OFH_CHECK(hipMalloc(&d[0], n0 * sizeof(double)));
-  OFH_CHECK(hipMalloc(&d[1], n0 * sizeof(double)));
-  OFH_CHECK(hipMalloc(&q, n0 * sizeof(double)));
+  dev_alloc(reinterpret_cast<void**>(&pc), n0 * sizeof(double));
+  dev_alloc(reinterpret_cast<void**>(&r), n0 * sizeof(double));
+  dev_alloc(reinterpret_cast<void**>(&d[0]), n0 * sizeof(double));
+  dev_alloc(reinterpret_cast<void**>(&d[1]), n0 * sizeof(double));
+  dev_alloc(reinterpret_cast<void**>(&q), n0 * sizeof(double));
   const size_t es = precision == OF_PRECISION_MIXED ? sizeof(float) : sizeof(double);
   for(size_t l = 0; l < lv.size(); ++l) {
     void *b = nullptr, *xa = nullptr, *xb = nullptr;
     const size_t n = (size_t)lv[l].n;
     // the finest FP64 level reads the CG residual directly as its rhs
-    if(!(l == 0 && precision == OF_PRECISION_FP64)) OFH_CHECK(hipMalloc(&b, n * es));
-    OFH_CHECK(hipMalloc(&xa, n * es));
-    OFH_CHECK(hipMalloc(&xb, n * es));
+    if(!(l == 0 && precision == OF_PRECISION_FP64)) dev_alloc(&b, n * es);
+    dev_alloc(&xa, n * es);
+    dev_alloc(&xb, n * es);
     lb.push_back(b);
     lxa.push_back(xa);
     lxb.push_back(xb);
   }
-  OFH_CHECK(hipMalloc(&ps, P_COUNT * sizeof(double)));
+  dev_alloc(reinterpret_cast<void**>(&ps), P_COUNT * sizeof(double));
   OFH_CHECK(hipMemsetAsync(ps, 0, P_COUNT * sizeof(double), c.stream));
-  OFH_CHECK(hipMalloc(&partials2, (size_t)kMaxRed * kMaxPartials * sizeof(double)));
-  OFH_CHECK(hipHostMalloc(&host_ps, 2 * kHostMirror * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
-  OFH_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&host_ps_dev), host_ps, 0));
-  for(int q = 0; q < 2 * kHostMirror; ++q) host_ps[q] = 0.0;
-  OFH_CHECK(hipMalloc(&dlv, lv.size() * sizeof(PLevel)));
-  OFH_CHECK(hipMemcpy(dlv, lv.data(), lv.size() * sizeof(PLevel), hipMemcpyHostToDevice));
-  for(auto& e : ev) OFH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  dev_alloc(reinterpret_cast<void**>(&partials2), (size_t)kMaxRed * kMaxPartials * sizeof(double));
+  if(arena) {  // the stop flag stays in device memory; the host copies it back after each iteration
+    host_ps_dev = static_cast<double*>(arena->take(2 * kHostMirror * sizeof(double)));
+    OFH_CHECK(hipMemsetAsync(host_ps_dev, 0, 2 * kHostMirror * sizeof(double), c.stream));
+  } else {
+    OFH_CHECK(hipHostMalloc(&host_ps, 2 * kHostMirror * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    OFH_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&host_ps_dev), host_ps, 0));
+    for(int q = 0; q < 2 * kHostMirror; ++q) host_ps[q] = 0.0;
+    for(auto& e : ev) OFH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  dev_alloc(reinterpret_cast<void**>(&dlv), lv.size() * sizeof(PLevel));
+  OFH_CHECK(hipMemcpyAsync(dlv, lv.data(), lv.size() * sizeof(PLevel), hipMemcpyHostToDevice, c.stream));
   std::vector<int> hperm;
   std::vector<double> hf, hU;
   factor_coarse(lv.back(), zm, hperm, hf, hU);
   nc = lv.back().n;
-  OFH_CHECK(hipMalloc(&perm, nc * sizeof(int)));
-  OFH_CHECK(hipMalloc(&factor, (size_t)nc * nc * sizeof(double)));
-  OFH_CHECK(hipMalloc(&U, (size_t)nc * nc * sizeof(double)));
-  OFH_CHECK(hipMalloc(&work, (size_t)nc * sizeof(double)));
-  OFH_CHECK(hipMemcpy(perm, hperm.data(), nc * sizeof(int), hipMemcpyHostToDevice));
-  OFH_CHECK(hipMemcpy(factor, hf.data(), hf.size() * sizeof(double), hipMemcpyHostToDevice));
-  OFH_CHECK(hipMemcpy(U, hU.data(), hU.size() * sizeof(double), hipMemcpyHostToDevice));
+  dev_alloc(reinterpret_cast<void**>(&perm), nc * sizeof(int));
+  dev_alloc(reinterpret_cast<void**>(&factor), (size_t)nc * nc * sizeof(double));
+  dev_alloc(reinterpret_cast<void**>(&U), (size_t)nc * nc * sizeof(double));
+  dev_alloc(reinterpret_cast<void**>(&work), (size_t)nc * sizeof(double));
+  // copies on the solver's stream, waited for here: the host vectors go out of scope
+  OFH_CHECK(hipMemcpyAsync(perm, hperm.data(), nc * sizeof(int), hipMemcpyHostToDevice, c.stream));
+  OFH_CHECK(hipMemcpyAsync(factor, hf.data(), hf.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  OFH_CHECK(hipMemcpyAsync(U, hU.data(), hU.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  OFH_CHECK(hipStreamSynchronize(c.stream));
 }
 
 template <typename T>
@@ -2508,8 +2525,8 @@ template <typename T>
 PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, double tol, bool want_history,
                                      bool zero_guess) {
   if(hist_cap < maxit + 1) {
-    if(history) OFH_CHECK(hipFree(history));
-    OFH_CHECK(hipMalloc(&history, (size_t)(maxit + 1) * sizeof(double)));
+    if(history && !arena) OFH_CHECK(hipFree(history));
+    dev_alloc(reinterpret_cast<void**>(&history), (size_t)(maxit + 1) * sizeof(double));
     hist_cap = maxit + 1;
   }
   const PLevel& F = lv[0];
@@ -2568,10 +2585,18 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
     }
     launch_finalize(c.stream, c.partials, hf.grid,
                     EpiUpdate{ps, history, host_ps_dev + (it % 2) * kHostMirror, n0, tol, it, zmi});
-    OFH_CHECK(hipEventRecord(ev[it % 2], c.stream));
-    if(it > 0) {
-      OFH_CHECK(hipEventSynchronize(ev[(it - 1) % 2]));
-      if(reinterpret_cast<volatile double*>(host_ps)[((it - 1) % 2) * kHostMirror + P_DONE] != 0.0) break;
+    if(arena) {
+      double done = 0.0;
+      OFH_CHECK(hipMemcpyAsync(&done, host_ps_dev + (it % 2) * kHostMirror + P_DONE, sizeof(double),
+                               hipMemcpyDeviceToHost, c.stream));
+      OFH_CHECK(hipStreamSynchronize(c.stream));
+      if(done != 0.0) break;
+    } else {
+      OFH_CHECK(hipEventRecord(ev[it % 2], c.stream));
+      if(it > 0) {
+        OFH_CHECK(hipEventSynchronize(ev[(it - 1) % 2]));
+        if(reinterpret_cast<volatile double*>(host_ps)[((it - 1) % 2) * kHostMirror + P_DONE] != 0.0) break;
+      }
     }
     vcycle<T>();
   }
@@ -2609,6 +2634,19 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
 PoissonSolver* poisson_create(Context& c, const of_grid& g, const of_pressure_bc_set& b, int precision) {
   if(precision != OF_PRECISION_FP64 && precision != OF_PRECISION_MIXED) throw InvalidArgument("unknown precision");
   auto* s = new PoissonSolver(c, g, b, precision);
+  try {
+    s->build();
+  } catch(...) {
+    delete s;
+    throw;
+  }
+  return s;
+}
+
+PoissonSolver* poisson_create_in(Context& c, const of_grid& g, const of_pressure_bc_set& b, int precision,
+                                 DeviceArena& arena) {
+  if(precision != OF_PRECISION_FP64 && precision != OF_PRECISION_MIXED) throw InvalidArgument("unknown precision");
+  auto* s = new PoissonSolver(c, g, b, precision, &arena);
   try {
     s->build();
   } catch(...) {
