@@ -44,7 +44,11 @@ def parse():
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--profile-kernel", default="k_dir_apply")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=128, help="grid of the bounded CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="steps of the benchmark case the reference CPU path is timed for (one is about 50 s)")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 2D 128^2 (C1) run")
+    ap.add_argument("--fp64-chebyshev-steps", type=int, default=20,
+                    help="FP64 steps timed with the mixed mode's Chebyshev sweep weights (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--adjoint", choices=["c4", "c5", "none"], default="c4",
                     help="adjoint workload timed beside the forward step (c5: 512^3, about 160 GB of HBM)")
@@ -190,14 +194,27 @@ def kernel_bytes(name, n_cells, mixed):
 
 
 def step_bytes_model(n, iters, mixed):
-    """Algorithmic HBM bytes of one step (passes over an n^3 FP64 array):
-    predictor RHS 3 x 7, ADI staging copies 3 x 2 + sweeps 9 x 2, pressure RHS 4, Poisson
-    setup/finish 12 (+1.5 FP32 cast), correction 9, pressure update 4, diagnostics 6; per MGPCG
-    iteration: fused direction/operator 6 (z, d_old, p read; d, p, q written; z in FP32 and an
-    FP32 copy of r in mixed: 5.5 + 3.5) and residual update 3, and the V-cycle 10.25 x 8/7
-    (hierarchy) in the multigrid precision (x 1/2 for FP32)."""
+    """Algorithmic HBM bytes of one step by the stage model of the path (passes over an n^3
+    FP64 array, each stage counted as its distinct arrays read + written once):
+    predictor RHS with the three components fused 13 (u, v, w, p, A_prev x 3 read; A x 3 and
+    rhs x 3 written); ADI 6 per component (rhs read, line written, staged once) = 18; pressure
+    RHS 4 + its mean 1; correction with max|du| and the divergence 13; pressure update 4;
+    diagnostics 6. Per MGPCG iteration: CG vector work 11 and the V-cycle 15.1 (13.25 on the
+    finest level x 8/7 for the hierarchy), the V-cycle halved when it runs in FP32."""
+    fixed = 13 + 18 + 5 + 13 + 4 + 6
+    per_it = 11.0 + 15.1 * (0.5 if mixed else 1.0)
+    return (fixed + iters * per_it) * n ** 3 * 8
+
+
+def step_kernel_bytes(n, iters, mixed):
+    """HBM bytes the launches of one step read and write by their own arrays (each kernel's
+    distinct arrays read + written once, the fused kernels counted as fused): predictor RHS
+    3 x 7, ADI staging copies 3 x 2 + sweeps 9 x 2, pressure RHS 4, Poisson setup/finish 12
+    (+1.5 FP32 cast), correction 9, pressure update 4, diagnostics 6; per MGPCG iteration the
+    fused direction/operator 6 (5.5 mixed) and residual update 3 (3.5 mixed), and the fused
+    V-cycle levels 2.125 (down) + 3.125 (up) x 8/7 in the multigrid precision."""
     fixed = 21 + 6 + 18 + 4 + 12 + (1.5 if mixed else 0.0) + 9 + 4 + 6
-    per_it = 9.0 + 10.25 * 8.0 / 7.0 * (0.5 if mixed else 1.0)
+    per_it = 9.0 + 5.25 * 8.0 / 7.0 * (0.5 if mixed else 1.0)
     return (fixed + iters * per_it) * n ** 3 * 8
 
 
@@ -234,19 +251,20 @@ def host_cpu():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(n_target, n, nu, cfl):
+def cpu_baseline(n, nu, cfl, steps):
     """The reference's own CPU path (oracle/_ref/ref_golden, built from the reference sources,
-    -O3, no fast-math) on this host, one thread, for one step of the same case at n^3; the
-    steps/s is scaled to n_target^3 by the cell ratio. Falls back to the oracle restatement."""
+    -O3, no fast-math) on this host, one thread, for `steps` steps of the benchmark case itself
+    (n^3, the same dt, tolerance and initial field convention as the GPU run). Falls back to
+    the oracle restatement where the reference build is absent."""
     from oracle import refrun
 
     L = 2 * math.pi
     final_time = 100 * (cfl / (3 * n / L)) * (1 - 1e-14)
-    scale = (n / n_target) ** 3
     if refrun.available():
+        print(f"# cpu baseline: the reference CPU path, {steps} step(s) at {n}^3", flush=True)
         t0 = time.perf_counter()
         _, meta = refrun.run("tg_steps", (), (), timeout=1800, nx=n, ny=n, nz=n, viscosity=nu, cfl=cfl,
-                             final_time=final_time, tol=1e-10, maxit=400, steps=1)
+                             final_time=final_time, tol=1e-10, maxit=400, steps=steps)
         wall = time.perf_counter() - t0
         el = float(meta.get("elapsed_seconds", wall))
         kind, what = "reference", "the reference CPU sources built -O3 without fast-math"
@@ -255,23 +273,72 @@ def cpu_baseline(n_target, n, nu, cfl):
         from oracle import ofref
 
         c = ofref.tg_config(n, n, n, nu, cfl, final_time, 400, 1e-10)
-        s = ofref.tg_init(c, convention=1)
+        s = ofref.tg_init(c, convention=0)
         t0 = time.perf_counter()
-        ofref.tg_steps(c, s, 1)
+        ofref.tg_steps(c, s, steps)
         el = time.perf_counter() - t0
         kind, what = "port", "the oracle's FP64 restatement of the reference"
         iters = s.metrics.pressure_iterations
-    return {"value": scale / el, "unit": "steps/s", "cores": 1, "kind": kind,
-            "sample": f"1 step of the same Taylor-Green case at {n}^3 ({n ** 3} cells, FP64, MGPCG tol 1e-10, "
-                      f"{iters} iterations) with {what}, 1 thread: {el:.2f} s; value = 1/{el:.2f} s scaled to "
-                      f"{n_target}^3 by the cell ratio {scale:g}",
-            "host_cpu": host_cpu(), "host_logical_cpus": os.cpu_count()}
+    return {"value": steps / el, "unit": "steps/s", "cores": 1, "kind": kind,
+            "sample": f"{steps} step(s) of the benchmark case itself ({n}^3 = {n ** 3} cells, FP64, MGPCG tol "
+                      f"1e-10, {iters} iterations in the last step, the reference's own initial field) with "
+                      f"{what}, 1 thread: {el:.2f} s",
+            "seconds": el, "host_cpu": host_cpu(), "host_logical_cpus": os.cpu_count()}
 
 
-def run_precision(sa, ctx, n, nu, dt, precision, warmup, steps, prof_kernel, predictor_cg=False):
+C1 = dict(n=128, nu=0.01, cfl=0.5, steps=200, tol=1e-10, maxit=200)
+
+
+def c1_run(sa, ctx, with_cpu):
+    """The 2D Taylor-Green benchmark configuration (128^2, viscosity 0.01, CFL 0.5, 200 steps,
+    MGPCG tolerance 1e-10, shifted initial field): the whole run on the device in FP64, and the
+    reference CPU path over the same 200 steps on this host (one thread)."""
     import torch
 
-    st = sa.Stepper(ctx, sa.TAYLOR_GREEN, n, n, n, nu, dt, 400, 1e-10, precision)
+    n, steps = C1["n"], C1["steps"]
+    L = 2 * math.pi
+    final_time = steps * (C1["cfl"] / (2 * n / L)) * (1 - 1e-12)
+    dt, ns = sa.taylor_green_dt(n, n, 1, C1["cfl"], final_time)
+    assert ns == steps
+    st = sa.Stepper(ctx, sa.TAYLOR_GREEN, n, n, 1, C1["nu"], dt, C1["maxit"], C1["tol"], sa.FP64)
+    st.initialize(convention=1)
+    st.run(1)  # warm-up of the kernels' first launches
+    st.initialize(convention=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m = st.run(steps)
+    ctx.sync()
+    gpu_s = time.perf_counter() - t0
+    st.close()
+    out = {"workload": "2D periodic Taylor-Green 128^2, viscosity 0.01, CFL 0.5, 200 steps, MGPCG tol 1e-10, fp64",
+           "gpu_seconds": gpu_s, "gpu_steps_per_s": steps / gpu_s, "max_div_linf": m.max_divergence_linf}
+    if with_cpu:
+        from oracle import refrun, ofref
+
+        if refrun.available():
+            c = ofref.tg_config(n, n, 1, C1["nu"], C1["cfl"], final_time, C1["maxit"], C1["tol"])
+            s0 = ofref.tg_init(c, convention=1)
+            shp = [ofref.storage_shape(n, n, 1, loc) for loc in (1, 2, 3, 0, 1, 2, 3)]
+            print("# cpu baseline: the reference CPU path, C1 200 steps", flush=True)
+            _, meta = refrun.run("tg_steps", s0.vel + [s0.p] + s0.adv, shp, timeout=1800, nx=n, ny=n, nz=1,
+                                 viscosity=C1["nu"], cfl=C1["cfl"], final_time=final_time, tol=C1["tol"],
+                                 maxit=C1["maxit"], steps=steps, has_prev=0)
+            el = float(meta["elapsed_seconds"])
+            out["cpu_reference_seconds"] = el
+            out["cpu_reference_steps_per_s"] = steps / el
+            out["gpu_vs_cpu"] = el / gpu_s
+    return out
+
+
+def run_precision(sa, ctx, n, nu, dt, precision, warmup, steps, prof_kernel, predictor_cg=False, chebyshev=False):
+    import torch
+
+    if chebyshev:
+        os.environ["SOLVER_AMD_FP64_CHEBYSHEV"] = "1"
+    try:
+        st = sa.Stepper(ctx, sa.TAYLOR_GREEN, n, n, n, nu, dt, 400, 1e-10, precision)
+    finally:
+        os.environ.pop("SOLVER_AMD_FP64_CHEBYSHEV", None)
     if predictor_cg:  # the Metal backend's predictor at its FP64 tolerance
         st.set_predictor(sa.PREDICTOR_CG, 1e-12, 80)
     st.initialize(convention=1)
@@ -291,12 +358,12 @@ def run_precision(sa, ctx, n, nu, dt, precision, warmup, steps, prof_kernel, pre
     ctx.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof = ctx.profile_read()
+    durs = ctx.profile_durations()
     ctx.profile_kernel("")
     vel, p, _, _, _ = st.download()
     st.close()
     return {"seconds": t1 - t0, "iters": iters, "div_linf": div_inf, "div_l2": m.divergence_l2,
-            "prof": prof, "fields": vel + [p], "max_cfl": m.max_cfl}
+            "durations": durs, "fields": vel + [p], "max_cfl": m.max_cfl}
 
 
 def rel_l2(a, b):
@@ -361,14 +428,23 @@ def main():
     def step_hbm(rr, mixed):
         it = sum(rr["iters"]) / len(rr["iters"])
         b = step_bytes_model(n, it, mixed)
-        gbs = b / (rr["seconds"] / a.steps) / 1e9
+        kb = step_kernel_bytes(n, it, mixed)
+        sec = rr["seconds"] / a.steps
+        gbs = b / sec / 1e9
         return {"algorithmic_bytes_per_step": b, "achieved_GBs": gbs, "peak_GBs": HBM_PEAK_GBS,
-                "frac": gbs / HBM_PEAK_GBS}
+                "frac": gbs / HBM_PEAK_GBS, "model": "stage passes (fused predictor, 6-pass ADI, 11 + 15.1 per "
+                "MGPCG iteration)", "kernel_bytes_per_step": kb, "kernel_bytes_GBs": kb / sec / 1e9,
+                "kernel_bytes_frac": kb / sec / 1e9 / HBM_PEAK_GBS}
 
-    prof = r["prof"]
-    avg_ms = prof["total_ms"] / max(prof["launches"], 1)
+    # the launches that did work: after convergence the host has already queued one more
+    # iteration whose kernels return at entry (a few microseconds); they are not the kernel's
+    # duration and are left out of the average
+    durs = r["durations"]
+    med = sorted(durs)[len(durs) // 2] if durs else 0.0
+    work = [d for d in durs if d > 0.25 * med]
+    avg_ms = sum(work) / len(work) if work else 0.0
     nbytes = kernel_bytes(a.profile_kernel, n_cells, main_prec == sa.MIXED)
-    achieved = nbytes / (avg_ms * 1e-3) / 1e9 if prof["launches"] else None
+    achieved = nbytes / (avg_ms * 1e-3) / 1e9 if work else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -390,8 +466,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": a.profile_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": measured_traffic(a.profile_kernel, main_prec == sa.MIXED),
-                     "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": avg_ms,
-                     "launches": prof["launches"]},
+                     "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": avg_ms, "median_launch_ms": med,
+                     "launches": len(work), "early_exit_launches": len(durs) - len(work)},
         "adjoint_s_per_step": None,
     }
     if sa.FP64 in res:
@@ -403,6 +479,24 @@ def main():
             out["mixed_speedup_vs_fp64"] = value / out["fp64"]["steps_per_s"]
             out["mixed_vs_fp64_rel_l2"] = {k: rel_l2(x, y) for k, x, y in
                                            zip("uvwp", res[sa.MIXED]["fields"], res[sa.FP64]["fields"])}
+    if a.fp64_chebyshev_steps > 0 and sa.FP64 in res and sa.MIXED in res:
+        # like for like: FP64 with the mixed mode's sweep weights separates the two sources of
+        # the mixed speed-up (smoother weights vs FP32 arithmetic); not the reference's smoother
+        k = min(a.fp64_chebyshev_steps, a.steps)
+        rch = run_precision(sa, ctx, n, nu, dt, sa.FP64, a.warmup, k, "", chebyshev=True)
+        rf = run_precision(sa, ctx, n, nu, dt, sa.FP64, a.warmup, k, "")
+        rm = run_precision(sa, ctx, n, nu, dt, sa.MIXED, a.warmup, k, "")
+        cheb = k / rch["seconds"]
+        out["fp64_chebyshev_weights"] = {
+            "steps": k, "steps_per_s": cheb, "ms_per_step": 1e3 * rch["seconds"] / k,
+            "mgpcg_iterations_per_step": sum(rch["iters"]) / k, "max_div_linf": rch["div_linf"],
+            "fp64_reference_weights_steps_per_s": k / rf["seconds"], "mixed_steps_per_s": k / rm["seconds"],
+            "speedup_from_weights": cheb / (k / rf["seconds"]),
+            "speedup_from_fp32": (k / rm["seconds"]) / cheb,
+            "vs_fp64_rel_l2": {key: rel_l2(x, y) for key, x, y in zip("uvwp", rch["fields"], rf["fields"])}}
+        print(f"# fp64 with Chebyshev weights: {cheb:.2f} steps/s, {sum(rch['iters']) / k:.1f} iterations/step; "
+              f"speed-up from the weights {out['fp64_chebyshev_weights']['speedup_from_weights']:.2f}x, "
+              f"from FP32 {out['fp64_chebyshev_weights']['speedup_from_fp32']:.2f}x", flush=True)
     if a.predictor_cg_steps > 0 and sa.FP64 in res:
         # the CG Helmholtz predictor in place of the line solves, FP64, on the same first steps
         k = min(a.predictor_cg_steps, a.steps)
@@ -422,9 +516,13 @@ def main():
               f"({r_adj['reverse_s_per_step'] * 1e3:.1f} ms/step)", flush=True)
         out["adjoint_s_per_step"] = r_adj["reverse_s_per_step"]
         out["adjoint"] = r_adj
+    if not a.no_c1:
+        out["c1"] = c1_run(sa, ctx, with_cpu=not a.no_cpu_baseline)
+        print(f"# c1 (2D 128^2, 200 steps): {out['c1']['gpu_steps_per_s']:.0f} steps/s on the device", flush=True)
     if not a.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(n, a.cpu_n, nu, cfl)
+            out["cpu_baseline"] = cpu_baseline(n, nu, cfl, a.cpu_steps)
+            out["cpu_baseline"]["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
         except Exception as e:  # reported, never fatal to the GPU measurement
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(out))

@@ -143,6 +143,9 @@ const char* of_build_info(void);
  * and restarts the record. NULL or "" switches timing off. */
 int of_profile_kernel(of_context* ctx, const char* kernel_name);
 int of_profile_read(of_context* ctx, int* launches, double* total_ms, double* min_ms, double* max_ms);
+/* The same record launch by launch: the first `capacity` durations in launch order (out_ms may
+ * be NULL), the count in *launches; restarts the record. */
+int of_profile_durations(of_context* ctx, double* out_ms, int capacity, int* launches);
 
 /* ---------------------------------------------------------------- boundary conditions */
 /* solver/projection.cpp:966-971 apply_velocity_boundary_conditions */

@@ -252,6 +252,46 @@ int run(const std::string& cmd, const Args& a) {
     w.put(s.advection_previous);
     return 0;
   }
+  if(cmd == "cavity_run") {  // to steady state (or max_steps); out: u v w p_total A_prev(u v w)
+    const solver::LidDrivenCavityConfig c = cavity_config_from(a);
+    solver::LidDrivenCavityState s = solver::initialize_lid_driven_cavity_state(c);
+    w.kv("dt", solver::lid_driven_cavity_dt(c));
+    bool steady = false;
+    try {
+      // one step per call: the public API's step loop with the steady test of run_lid_driven_cavity
+      while(s.metrics.step < c.max_steps) {
+        solver::run_lid_driven_cavity_steps(c, 1, s);
+        if(s.metrics.step >= c.min_steps && s.metrics.max_velocity_change <= c.steady_tolerance) {
+          steady = true;
+          break;
+        }
+      }
+    } catch(const std::exception& e) {
+      // one step per call numbers the failing step 1 in the message; run_lid_driven_cavity's
+      // single loop numbers it by the state's step count, which is restored here
+      std::string msg = e.what();
+      const std::string local = "at step 1 (";
+      const size_t at = msg.find(local);
+      if(at != std::string::npos)
+        msg.replace(at, local.size(), "at step " + std::to_string(s.metrics.step + 1) + " (");
+      w.kv("error", msg);
+    }
+    w.kv("steady", steady ? 1 : 0);
+    put_metrics(w, s.metrics);
+    const solver::LidDrivenCavityResult r = solver::finalize_lid_driven_cavity_result(c, s);
+    w.kv("u_vertical_max", r.extrema.u_vertical_max);
+    w.kv("u_vertical_min", r.extrema.u_vertical_min);
+    w.kv("v_horizontal_max", r.extrema.v_horizontal_max);
+    w.kv("v_horizontal_min", r.extrema.v_horizontal_min);
+    w.kv("validation_pass", r.validation.pass ? 1 : 0);
+    w.kv("validation_max_relative_error", r.validation.max_relative_error);
+    for(size_t k = 0; k < r.validation.points.size(); ++k)
+      w.kv(("sample_" + std::to_string(k)).c_str(), r.validation.points[k].sample_value);
+    w.put(s.velocity);
+    w.put(s.pressure_total);
+    w.put(s.advection_previous);
+    return 0;
+  }
   if(cmd == "channel") {  // out: streamwise profile (ny values)
     solver::ChannelFlowConfig c;
     c.case_kind = need(a, "case") == "poiseuille" ? solver::ChannelFlowCase::poiseuille

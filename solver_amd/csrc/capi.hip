@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 namespace ofh {
 
@@ -264,6 +265,20 @@ int of_profile_read(of_context* c, int* launches, double* total_ms, double* min_
     if(total_ms) *total_ms = tot;
     if(min_ms) *min_ms = x.prof_used ? mn : 0.0;
     if(max_ms) *max_ms = mx;
+    x.prof_used = 0;
+  });
+}
+
+int of_profile_durations(of_context* c, double* out_ms, int capacity, int* launches) {
+  return guarded([&] {
+    Context& x = C(c);
+    OFH_CHECK(hipStreamSynchronize(x.stream));
+    for(size_t i = 0; i < x.prof_used; ++i) {
+      float ms = 0.0f;
+      OFH_CHECK(hipEventElapsedTime(&ms, x.prof_events[i].first, x.prof_events[i].second));
+      if(out_ms && (int)i < capacity) out_ms[i] = ms;
+    }
+    if(launches) *launches = (int)x.prof_used;
     x.prof_used = 0;
   });
 }
@@ -677,11 +692,28 @@ int of_stepper_upload(of_stepper* s, const double* u, const double* v, const dou
     // the copies go on the stepper's stream (non-blocking, so the legacy stream would not
     // order them against its kernels) and the call returns when they are done
     OFH_CHECK(hipStreamSynchronize(S.c.stream));
+    std::vector<double> active_only;
     for(int a = 0; a < 3; ++a) {
       const size_t n = fsize(S.g, OF_FACE_X + a);
       OFH_CHECK(hipMemcpyAsync(S.vel[a], vin[a], n, hipMemcpyHostToDevice, S.c.stream));
-      if(ain[a]) OFH_CHECK(hipMemcpyAsync(S.adv[a], ain[a], n, hipMemcpyHostToDevice, S.c.stream));
-      else OFH_CHECK(hipMemsetAsync(S.adv[a], 0, n, S.c.stream));
+      // The reference carries the state's edge and corner ghosts unchanged from step to step
+      // (every copy in run_taylor_green_steps takes them along and no BC fill writes them, so
+      // the factorisation correction keeps reading the injected values). The step writes its new
+      // velocity into the second buffer set and swaps, so both sets start with the same storage.
+      OFH_CHECK(hipMemcpyAsync(S.corr[a], S.vel[a], n, hipMemcpyDeviceToDevice, S.c.stream));
+      if(ain[a]) {
+        // the reference's A_prev after a step carries the ghosts of its advection work array,
+        // which are never written (zero); the step never reads them, so they are zero here too
+        const Lay L = make_lay(S.g, OF_FACE_X + a);
+        active_only.assign(L.size(), 0.0);
+        for(int k = 1; k <= L.az; ++k)
+          for(int j = 1; j <= L.ay; ++j)
+            for(int i = 1; i <= L.ax; ++i) active_only[L.at(i, j, k)] = ain[a][L.at(i, j, k)];
+        OFH_CHECK(hipMemcpyAsync(S.adv[a], active_only.data(), n, hipMemcpyHostToDevice, S.c.stream));
+        OFH_CHECK(hipStreamSynchronize(S.c.stream));  // active_only is refilled for the next component
+      } else {
+        OFH_CHECK(hipMemsetAsync(S.adv[a], 0, n, S.c.stream));
+      }
     }
     OFH_CHECK(hipMemcpyAsync(S.p_total, p, fsize(S.g, OF_CELL), hipMemcpyHostToDevice, S.c.stream));
     OFH_CHECK(hipStreamSynchronize(S.c.stream));

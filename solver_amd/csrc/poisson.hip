@@ -2112,6 +2112,15 @@ void mixed_omega(int dims, double& w1, double& w2) {
     throw InvalidArgument("SOLVER_AMD_MIXED_OMEGA must give a convergent smoother");
 }
 
+// SOLVER_AMD_FP64_CHEBYSHEV=1, read when a solver is built, gives an FP64 solve the MIXED mode's
+// Chebyshev sweep weights (or SOLVER_AMD_MIXED_OMEGA's). Not the reference's smoother, so FP64
+// iteration counts then differ from the reference's: it exists to measure how much of the
+// mixed mode's speed comes from the weights and how much from FP32 arithmetic.
+bool fp64_chebyshev() {
+  const char* e = std::getenv("SOLVER_AMD_FP64_CHEBYSHEV");
+  return e != nullptr && e[0] == '1';
+}
+
 // Tile rows of the fused level kernels per precision, overridden by SOLVER_AMD_FUSED_ROWS =
 // "down64,up64,down32,up32" (each 8 or 16) when a solver is built. 16 rows (512 threads) is the
 // default: the halo overhead falls from 1.59x to 1.33x of the tile and the FP64 down kernel
@@ -2287,7 +2296,8 @@ PoissonSolver::~PoissonSolver() {
 
 void PoissonSolver::build() {
   double w2 = omega;
-  if(precision == OF_PRECISION_MIXED) mixed_omega((g.nx > 1) + (g.ny > 1) + (g.nz > 1), omega, w2);
+  if(precision == OF_PRECISION_MIXED || fp64_chebyshev())
+    mixed_omega((g.nx > 1) + (g.ny > 1) + (g.nz > 1), omega, w2);
   for(int f = 0; f < 6; ++f)
     if(bcs.faces[f].type == OF_PRESSURE_DIRICHLET) zm = false;
   // hierarchy (linsolve/poisson_solver.cpp:316-384)

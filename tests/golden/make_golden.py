@@ -305,5 +305,65 @@ def validation_fixture():
     save("validation_latest", {"source": "validation/latest"}, {}, {}, {"tables": tables})
 
 
+def long_run_fixtures():
+    """Reference runs the small fixtures above cannot stand for:
+    * tg3d_edges: two steps from a state whose edge and corner ghosts are nonzero, the values
+      the 27-point factorisation correction reads (solver/taylor_green.cpp:145-225) and a BC
+      fill never writes;
+    * tg2d_128_c1: the 2D Taylor-Green benchmark configuration at full length (128^2,
+      viscosity 0.01, CFL 0.5, 200 steps, MGPCG tolerance 1e-10) from the shifted initial
+      field u = sin x cos y, v = -cos x sin y, injected through TaylorGreenState;
+    * cavity_c2_128: the lid-driven cavity (128^2, Re 100) run to steady state by the rule of
+      run_lid_driven_cavity at max|du| <= 1e-8 with Poisson tolerance 1e-10, and at the
+      reference's own benchmark setting (1e-7, 1e-9): final metrics, centreline validation and
+      fields."""
+    rng = np.random.default_rng(20261204)
+    tp = tg_params(10, 8, 6, 0.02, 0.4, 2)
+    g = dict(nx=10, ny=8, nz=6)
+    shp = shapes(g, VEL + [CELL] + VEL)
+    ins = [0.5 * rng.standard_normal(s) for s in shp]
+    outs, meta = refrun.run("tg_steps", ins, shp, steps=2, has_prev=1, **tp)
+    save("tg3d_edges_steps", dict(tp, steps=2, has_prev=1),
+         dict(u=ins[0], v=ins[1], w=ins[2], p=ins[3], au=ins[4], av=ins[5], aw=ins[6]),
+         dict(u=outs[0], v=outs[1], w=outs[2], p=outs[3], au=outs[4], av=outs[5], aw=outs[6]), meta)
+
+    tp = tg_params(128, 128, 1, 0.01, 0.5, 200, tol=1e-10, maxit=200)
+    c = ofref.tg_config(128, 128, 1, tp["viscosity"], tp["cfl"], tp["final_time"], tp["maxit"], tp["tol"])
+    s0 = ofref.tg_init(c, convention=1)
+    g = dict(nx=128, ny=128, nz=1)
+    shp = shapes(g, VEL + [CELL] + VEL)
+    ins = s0.vel + [s0.p] + s0.adv
+    outs, meta = refrun.run("tg_steps", ins, shp, steps=200, has_prev=0, timeout=1800, **tp)
+    save("tg2d_128_c1", dict(tp, steps=200, has_prev=0), dict(u=ins[0], v=ins[1], p=ins[3]),
+         dict(u=outs[0], v=outs[1], p=outs[3]), meta)
+
+    cavity_fixtures()
+
+
+def cavity_fixtures():
+    import concurrent.futures as cf
+
+    g = dict(nx=128, ny=128, nz=1)
+    shp = shapes(g, VEL + [CELL] + VEL)
+    cases = [("cavity_c2_128", 1e-8, 1e-10), ("cavity_bench_128", 1e-7, 1e-9)]
+
+    def one(case):
+        name, steady, tol = case
+        cp = dict(nx=128, ny=128, reynolds=100.0, lid=1.0, cfl=0.5, max_steps=20000, min_steps=400,
+                  steady_tolerance=steady, tol=tol, maxit=200, validate=1)
+        outs, meta = refrun.run("cavity_run", (), shp, timeout=3600, **cp)
+        return name, cp, outs, meta
+
+    with cf.ThreadPoolExecutor(len(cases)) as ex:  # independent reference runs of several minutes
+        for name, cp, outs, meta in ex.map(one, cases):
+            save(name, cp, {}, dict(u=outs[0], v=outs[1], p=outs[3]), meta)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["long"]:
+        long_run_fixtures()
+    elif sys.argv[1:] == ["cavity"]:
+        cavity_fixtures()
+    else:
+        main()
+        long_run_fixtures()

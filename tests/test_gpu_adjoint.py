@@ -275,3 +275,36 @@ def test_adjoint_rejects_unsupported_cases(ctx):
     adj = sa.Adjoint(ctx, 8, 8, 8, 0.01, 0.01, 2)
     with pytest.raises(sa.InvalidArgument, match="forward pass"):
         adj.backward()
+
+
+def test_forced_mixed_gradient_wrt_forcing(ctx):
+    """The C5 workload's shape at a small size: mixed precision (FP32 V-cycle inside FP64 CG),
+    Re 1600, the seeded forcing f = (A sin(kx x) sin(ky y), 0, 0), checkpoints every 2 of 6
+    steps; the gradient of the final kinetic energy w.r.t. f against the device tangent and
+    against central differences of the forward run along seeded forcing directions."""
+    n, steps = 16, 6
+    c, dt, vel, p = tg_setup(n, steps, nu=1 / 1600, amp=0.05, tol=1e-12)
+    base = inputs.forcing(n, SEED, 1e-2)
+    adj = make_adjoint(ctx, c, dt, 2, sa.MIXED)
+    adj.set_forcing([dev(a) for a in base])
+    u0 = [dev(a) for a in vel]
+    adj.forward(u0, dev(p), steps)
+    _, _, gf = adj.backward(want_pressure=False, want_forcing=True)
+    g = [host(t) for t in gf]
+    assert np.max(np.abs(g[0])) > 0
+    for d in range(1, 3):
+        df = inputs.forcing(n, SEED + d, 1.0)
+        pred = vdot(g, df)
+        _, dke = adj.tangent(dforce=[dev(a) for a in df])
+        assert abs(dke - pred) <= 1e-9 * abs(pred), (d, dke, pred)
+        errs = []
+        for eps in (1e-4, 1e-5, 1e-6):
+            ks = []
+            for sgn in (1.0, -1.0):
+                adj.set_forcing([dev(b + sgn * eps * x) for b, x in zip(base, df)])
+                ks.append(adj.forward(u0, dev(p), steps)[0])
+            errs.append(abs((ks[0] - ks[1]) / (2 * eps) - pred) / abs(pred))
+        adj.set_forcing([dev(a) for a in base])
+        adj.forward(u0, dev(p), steps)
+        assert min(errs) <= 1e-6, (d, errs)
+    adj.close()

@@ -126,7 +126,8 @@ def test_poisson_periodic_64_fft_and_oracle(ctx, precision):
     ([2, 2, 2, 2, 2, 2], (12, 10, 8)),
     ([2, 2, 0, 0, 1, 1], (18, 12, 8)),
 ])
-def test_poisson_bc_families_vs_oracle(ctx, pbc, shape):
+@pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
+def test_poisson_bc_families_vs_oracle(ctx, pbc, shape, precision):
     nx, ny, nz = shape
     h = (1.0 / nx, 1.3 / ny, 0.9 / nz if nz > 1 else 1.0)
     g = sa.grid(nx, ny, nz, *h)
@@ -141,11 +142,13 @@ def test_poisson_bc_families_vs_oracle(ctx, pbc, shape):
     sol = zeros(g, sa.CELL)
     d = sa.PoissonDiagnostics()
     rhs_d = dev(rhs)
-    ctx.run("of_solve_pressure_poisson", C.byref(g), C.byref(pb), C.byref(o), sa.FP64, sa.ptr(rhs_d), sa.ptr(sol),
+    ctx.run("of_solve_pressure_poisson", C.byref(g), C.byref(pb), C.byref(o), precision, sa.ptr(rhs_d), sa.ptr(sol),
             C.byref(d), None, 0)
     ref, rd, _ = ofref.poisson(og, ofref.pbcset(pbc, vals, grads), ofref.opts(1.0, 1.0, 600, tol), rhs.copy())
     assert d.converged == rd.converged == 1
-    assert d.iterations == rd.iterations
+    assert d.relative_residual <= tol
+    if precision == sa.FP64:
+        assert d.iterations == rd.iterations
     assert rel_max(host(sol)[1:-1, 1:-1, 1:-1], ref[1:-1, 1:-1, 1:-1]) <= 1e-10
 
 
@@ -326,3 +329,50 @@ def test_taylor_green_cg_predictor_vs_oracle(ctx, precision):
         if not np.any(b[1:-1, 1:-1, 1:-1]):
             continue
         assert rel_l2_active(a, b, mean_removed=(k == "p")) <= 1e-10, k
+
+
+@pytest.fixture(scope="module")
+def tg_production_oracle():
+    """The oracle's trajectory of a 3D periodic Taylor-Green run at a production tile shape
+    (128 x 64 x 64: full 64-lane rows, several fused multigrid levels, x tile boundaries),
+    Re 1600, CFL 0.5, 3 steps at tolerance 1e-12; computed once for both precisions."""
+    nx, ny, nz = 128, 64, 64
+    L = 2 * math.pi
+    final_time = 3 * (0.5 / (nx / L + ny / L + nz / L)) * (1 - 1e-12)
+    c = ofref.tg_config(nx, ny, nz, 1 / 1600, 0.5, final_time, 400, 1e-12)
+    s = ofref.tg_init(c, convention=1)
+    init = s.copy()
+    iters = []
+    for _ in range(3):
+        ofref.tg_steps(c, s, 1)
+        iters.append(s.metrics.pressure_iterations)
+    return c, init, s, iters
+
+
+@pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
+def test_taylor_green_production_shape_vs_oracle(ctx, tg_production_oracle, precision):
+    c, init, ref, ref_iters = tg_production_oracle
+    dt, nsteps = sa.taylor_green_dt(c.nx, c.ny, c.nz, 0.5, c.final_time)
+    assert nsteps == 3
+    st = sa.Stepper(ctx, sa.TAYLOR_GREEN, c.nx, c.ny, c.nz, c.viscosity, dt, 400, 1e-12, precision)
+    st.upload(init.vel[0], init.vel[1], init.vel[2], init.p)
+    iters = []
+    for _ in range(3):
+        m = st.run(1)
+        iters.append(m.pressure_iterations)
+        assert m.divergence_linf <= 1e-10
+    vel, p, adv, _, _ = st.download()
+    st.close()
+    if precision == sa.FP64:
+        # On this grid (spacing 1:2:2) the preconditioned CG amplifies rounding: the device's
+        # and the oracle's residual histories agree to 1e-15 in the first iterations, and the
+        # difference grows about tenfold per iteration from there, because the two sum their
+        # dot products in different orders. The stopping iteration can move by one or two; the
+        # fields, which both solve to the tolerance, agree to 1e-10 (checked below).
+        assert all(abs(a - b) <= 2 for a, b in zip(iters, ref_iters)), (iters, ref_iters)
+    for a, b, k in zip(vel + [p] + adv, ref.vel + [ref.p] + ref.adv, ["u", "v", "w", "p", "au", "av", "aw"]):
+        if not np.any(b[1:-1, 1:-1, 1:-1]):
+            assert np.max(np.abs(a[1:-1, 1:-1, 1:-1])) <= 1e-14, k
+            continue
+        err = rel_l2_active(a, b, mean_removed=(k == "p"))
+        assert err <= 1e-10, f"{k}: rel L2 {err:.3e}"
