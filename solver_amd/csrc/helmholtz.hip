@@ -397,6 +397,170 @@ __global__ void __launch_bounds__(64) k_line_solve_x(LineBatch B) {
   if(S.copy_dup && lane < nr) f[my_off + (S.e - S.b)] = first;
 }
 
+// ------------------------------------------------------------------ periodic lines as convolutions
+// On a periodic axis every line matrix of a sweep is the same circulant tridiag(-b, 1 + 2b, -b)
+// of order n (the cells along the axis; the normal component's duplicated face is not an
+// unknown). Its inverse is the circulant with first row
+//   g_k = (rho^k + rho^(n-k)) / ((1 - rho^n) (b/rho - b rho)),  k = 0 .. n-1,
+// where rho = 2b / ((1 + 2b) + sqrt((1 + 2b)^2 - 4 b^2)) is the smaller root of
+// b r^2 - (1 + 2b) r + b = 0, so g decays like rho^|k| (rho ~ b for small b: 2e-3 on the 256^3
+// Taylor-Green step). Where the taps beyond K weigh less than 1e-18 of g_0 the sweep is the
+// (2K+1)-point convolution x_t = sum_{|m|<=K} g_|m| r_{t+m mod n}: the same solution as the
+// cyclic Thomas / Sherman-Morrison recurrences of solver/projection.cpp:561-617 to rounding
+// (about 1e-16 relative), computed with one read and one write per element and no serial
+// recurrence. Sweeps with more than kConvMaxTaps taps, or lines shorter than 2K+1, keep the
+// line solves.
+constexpr int kConvMaxTaps = 16;
+struct ConvTaps {
+  double g[kConvMaxTaps + 1];  // g_0 .. g_K (zero beyond K)
+  int k;
+};
+
+bool conv_taps(int n, double beta, ConvTaps& t) {
+  if(!(beta > 0.0) || n < 3) return false;
+  const long double b = beta;
+  const long double d = 1.0L + 2.0L * b;
+  const long double rho = 2.0L * b / (d + std::sqrt(d * d - 4.0L * b * b));
+  const long double scale = 1.0L / ((1.0L - std::pow(rho, (long double)n)) * (b / rho - b * rho));
+  std::vector<long double> g((size_t)n / 2 + 1);
+  for(int k = 0; k <= n / 2; ++k) g[k] = scale * (std::pow(rho, (long double)k) + std::pow(rho, (long double)(n - k)));
+  int K = -1;
+  for(int k = 1; k <= std::min(kConvMaxTaps, n / 2); ++k) {
+    long double tail = 0.0L;  // weight of the taps the convolution leaves out, both sides
+    for(int m = k + 1; m <= n / 2; ++m) tail += (2 * m == n ? 1.0L : 2.0L) * g[m];
+    if(tail <= 1e-18L * g[0]) {
+      K = k;
+      break;
+    }
+  }
+  if(K < 0 || 2 * K + 1 > n) return false;
+  for(int m = 0; m <= kConvMaxTaps; ++m) t.g[m] = m <= K ? (double)g[m] : 0.0;
+  t.k = K;
+  return true;
+}
+
+// x sweep of one component: one wave per grid row. The row's n unknowns (the first sweep's
+// input, so the normal component's duplicated face pair is averaged as the periodic ghost fill
+// does before the first sweep, solver/projection.cpp:233-253) go to an LDS buffer with K
+// wrapped elements on each side; each lane then forms outputs i = lane, lane + 64, ... from
+// the symmetric pairs, smallest taps first.
+template <int K>
+__global__ void __launch_bounds__(256) k_helm_conv_x(const double* __restrict__ in, double* __restrict__ out, Lay L,
+                                                     int comp, int n, int nrows, ConvTaps taps) {
+  extern __shared__ double conv_row[];
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  const int r = (int)blockIdx.x * 4 + wave;
+  double* const buf = conv_row + (size_t)wave * (n + 2 * K);
+  if(r < nrows) {
+    const int j = 1 + r % L.ay, k = 1 + r / L.ay;
+    const double* const row = in + L.at(0, j, k);
+    // partner row of the averaged duplicate pair (components 1 and 2), or null
+    const double* prow = nullptr;
+    if(comp == 1 && (j == 1 || j == L.ay)) prow = in + L.at(0, j == 1 ? L.ay : 1, k);
+    if(comp == 2 && (k == 1 || k == L.az)) prow = in + L.at(0, j, k == 1 ? L.az : 1);
+    for(int m = lane; m < n + 2 * K; m += 64) {
+      int t = m - K;
+      t = t < 0 ? t + n : (t >= n ? t - n : t);
+      double v = row[1 + t];
+      if(comp == 0 && t == 0) v = 0.5 * (v + row[1 + n]);
+      if(prow != nullptr) v = 0.5 * (v + prow[1 + t]);
+      buf[m] = v;
+    }
+  }
+  __syncthreads();
+  if(r >= nrows) return;
+  const int j = 1 + r % L.ay, k = 1 + r / L.ay;
+  double* const orow = out + L.at(0, j, k);
+  for(int i = lane; i < n; i += 64) {
+    const double* c = buf + i + K;
+    double acc = 0.0;
+#pragma unroll
+    for(int m = K; m >= 1; --m) acc = fma(taps.g[m], c[-m] + c[m], acc);
+    acc = fma(taps.g[0], c[0], acc);
+    orow[1 + i] = acc;
+    if(comp == 0 && i == 0) orow[1 + n] = acc;  // the duplicated face
+  }
+}
+
+// y or z sweep of one component: lanes along i (coalesced rows), each thread convolving a
+// chunk of S consecutive elements of one line through a register window of 2K+1 inputs. Blocks
+// are ordered chunk-fastest and grouped per XCD, so neighbouring chunks of a line, which share
+// 2K halo elements, tend to run on one XCD and find those in its L2.
+constexpr int kConvChunk = 32;
+template <int K>
+__global__ void __launch_bounds__(256) k_helm_conv_s(const double* __restrict__ in, double* __restrict__ out, Lay L,
+                                                     int comp, int axis, int n, int nchunks, int ngx, ConvTaps taps) {
+  constexpr int S = kConvChunk;
+  const int o1 = axis == 2 ? 1 : 2;  // the other axis of the line's position
+  unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int chunk = (int)(bid % (unsigned)nchunks);
+  bid /= (unsigned)nchunks;
+  const int bx = (int)(bid % (unsigned)ngx), by = (int)(bid / (unsigned)ngx);
+  const int i = bx * 64 + (int)threadIdx.x;
+  const int o = by * 4 + (int)threadIdx.y;
+  if(i >= L.ax || o >= L.a(o1)) return;
+  const size_t st = axis == 1 ? (size_t)L.sx : (size_t)L.sx * L.sy;
+  const size_t base = axis == 1 ? L.at(1 + i, 1, 1 + o) : L.at(1 + i, 1 + o, 1);  // element t at base + t st
+  const double* const src = in + base;
+  double* const dst = out + base;
+  const int t0 = chunk * S;
+  auto wrap = [&](int t) { return t < 0 ? t + n : (t >= n ? t - n : t); };
+  double v[S + 2 * K];
+#pragma unroll
+  for(int m = 0; m < S + 2 * K; ++m) v[m] = src[(size_t)wrap(min(t0 - K + m, n + K - 1)) * st];
+#pragma unroll
+  for(int s = 0; s < S; ++s) {
+    const int t = t0 + s;
+    if(t >= n) break;
+    const double* c = v + s + K;
+    double acc = 0.0;
+#pragma unroll
+    for(int m = K; m >= 1; --m) acc = fma(taps.g[m], c[-m] + c[m], acc);
+    acc = fma(taps.g[0], c[0], acc);
+    dst[(size_t)t * st] = acc;
+    if(comp == axis && t == 0) dst[(size_t)n * st] = acc;  // the duplicated face
+  }
+}
+
+template <int K>
+void launch_conv_sweep(Context& c, const of_grid& g, int comp, int axis, const double* in, double* out,
+                       const ConvTaps& taps) {
+  const Lay L = make_lay(g, OF_FACE_X + comp);
+  const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
+  if(axis == 0) {
+    const int nrows = L.ay * L.az;
+    const size_t lds = 4 * (size_t)(n + 2 * K) * sizeof(double);
+    ProfScope prof_(c, "k_helm_conv_x", true);
+    OFH_LAUNCH(k_helm_conv_x<K>, dim3((unsigned)div_up((size_t)nrows, 4)), dim3(64, 4), lds, c.stream, in, out, L,
+               comp, n, nrows, taps);
+  } else {
+    const int o1 = axis == 2 ? 1 : 2;
+    const int nchunks = (n + kConvChunk - 1) / kConvChunk;
+    const int ngx = (L.ax + 63) / 64, ngy = (L.a(o1) + 3) / 4;
+    ProfScope prof_(c, "k_helm_conv_s", true);
+    OFH_LAUNCH(k_helm_conv_s<K>, dim3((unsigned)(nchunks * ngx * ngy)), dim3(64, 4), 0, c.stream, in, out, L, comp,
+               axis, n, nchunks, ngx, taps);
+  }
+}
+
+void conv_sweep(Context& c, const of_grid& g, int comp, int axis, const double* in, double* out, const ConvTaps& t) {
+  if(t.k <= 8) launch_conv_sweep<8>(c, g, comp, axis, in, out, t);
+  else launch_conv_sweep<16>(c, g, comp, axis, in, out, t);
+}
+
+// The convolution taps of the three axes when every axis is periodic and every sweep qualifies.
+bool conv_plan(const of_grid& g, const of_bc_set& b, double alpha, ConvTaps (&t)[3]) {
+  if(alpha <= 0.0) return false;
+  const int n[3] = {g.nx, g.ny, g.nz};
+  for(int a = 0; a < 3; ++a) {
+    if(!axis_periodic(b, a)) return false;
+    const double h = spacing(g, a);
+    if(!conv_taps(n[a], alpha / (h * h), t[a])) return false;
+    if(a == 0 && 4 * (size_t)(n[a] + 2 * kConvMaxTaps) * sizeof(double) > 64 * 1024) return false;
+  }
+  return true;
+}
+
 inline bool dirichlet_velocity(const of_bc& c) {  // solver/projection.cpp:45-49
   return c.type == OF_NO_SLIP_WALL || c.type == OF_PRESCRIBED_VELOCITY || c.type == OF_SYMMETRY;
 }
@@ -674,11 +838,32 @@ __global__ void k_dup_fold(double* f, Lay L, int axis, int b, int e) {
 
 // Returns the reference's line_solves count.
 int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* ru, const double* rv,
-                  const double* rw, double* ou, double* ov, double* ow) {
+                  const double* rw, double* ou, double* ov, double* ow, double* const* scratch, bool keep_storage) {
   if(alpha < 0.0) throw InvalidArgument("solve_predictor_adi expects a non-negative alpha");
   const double* rhs[3] = {ru, rv, rw};
   double* out[3] = {ou, ov, ow};
   const AdiPlan P = plan_adi(c, g, b, alpha);
+  ConvTaps taps[3];
+  if(conv_plan(g, b, alpha, taps)) {
+    // every axis periodic: x sweep rhs -> out, y sweep out -> tmp, z sweep tmp -> out, then
+    // the ghost fill; the sweeps read no ghosts, and the duplicated faces are equal after the
+    // first sweep, so the reference's fills between sweeps change nothing they read
+    double* tmp[3];
+    for(int comp = 0; comp < 3; ++comp) {
+      const Lay L = make_lay(g, OF_FACE_X + comp);
+      if(keep_storage)  // the reference's result starts as a copy of the whole rhs storage
+        OFH_CHECK(hipMemcpyAsync(out[comp], rhs[comp], L.size() * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+      tmp[comp] = scratch != nullptr ? scratch[comp]
+                                     : static_cast<double*>(c.scratch_get(kAdiCheckpointSlot + 1 + comp,
+                                                                          L.size() * sizeof(double)));
+    }
+    for(int comp = 0; comp < 3; ++comp) conv_sweep(c, g, comp, 0, rhs[comp], out[comp], taps[0]);
+    for(int comp = 0; comp < 3; ++comp) conv_sweep(c, g, comp, 1, out[comp], tmp[comp], taps[1]);
+    for(int comp = 0; comp < 3; ++comp) conv_sweep(c, g, comp, 2, tmp[comp], out[comp], taps[2]);
+    launch_velocity_bc(c, g, b, out[0], out[1], out[2]);
+    OFH_CHECK(hipGetLastError());
+    return P.line_solves;
+  }
   for(int comp = 0; comp < 3; ++comp) {
     const Lay L = make_lay(g, OF_FACE_X + comp);
     OFH_CHECK(hipMemcpyAsync(out[comp], rhs[comp], L.size() * sizeof(double), hipMemcpyDeviceToDevice, c.stream));

@@ -110,8 +110,13 @@ void launch_scale_active(Context& c, const Lay& L, double scale, const double* i
 void launch_axpy_active(Context& c, const Lay& L, double a, const double* x, double* y);
 
 // ------------------------------------------------------------------ helmholtz.hip
+// scratch: three work fields of the velocity layout the solve may overwrite (null: context
+// scratch); they may be the rhs arrays themselves when the caller no longer needs them.
+// keep_storage: the output starts as a copy of the whole rhs storage (edge ghosts included), as
+// the reference's does; a caller that never reads the output's edge ghosts passes false.
 int predictor_adi(Context& c, const of_grid& g, const of_bc_set& b, double alpha, const double* ru,
-                  const double* rv, const double* rw, double* ou, double* ov, double* ow);
+                  const double* rv, const double* rw, double* ou, double* ov, double* ow,
+                  double* const* scratch = nullptr, bool keep_storage = true);
 // Conjugate-gradient alternative to predictor_adi for grids periodic on every axis (work arrays
 // of one grid in a HelmholtzCG). Returns the largest iteration count of the three components;
 // throws when one did not reach tol within max_iterations. worst_rel (may be null) receives the

@@ -342,7 +342,7 @@ void Stepper::step() {
     predictor_iterations = predictor_cg(*hcg, g, bcs, 0.5 * nu * dt, rhs[0], rhs[1], rhs[2], pred[0], pred[1],
                                         pred[2], predictor_tol, predictor_maxit, nullptr);
   else
-    predictor_adi(c, g, bcs, 0.5 * nu * dt, rhs[0], rhs[1], rhs[2], pred[0], pred[1], pred[2]);
+    predictor_adi(c, g, bcs, 0.5 * nu * dt, rhs[0], rhs[1], rhs[2], pred[0], pred[1], pred[2], rhs, false);
   // projection (solver/projection.cpp:1097-1131); pred already carries its BC ghosts
   launch_divergence_scaled(c, g, pred[0], pred[1], pred[2], 1.0 / dt, prhs);
   // phi starts from zero (solver/taylor_green.cpp:596): the zero-warm-start solve neither reads
