@@ -147,6 +147,11 @@ int of_profile_read(of_context* ctx, int* launches, double* total_ms, double* mi
  * be NULL), the count in *launches; restarts the record. */
 int of_profile_durations(of_context* ctx, double* out_ms, int capacity, int* launches);
 
+/* No reference counterpart: sqrt(mean((a - b)^2)) over the active entries of two fields of one
+ * location (a deterministic device reduction; the manufactured-solution check uses it). */
+int of_active_rms_difference(of_context* ctx, const of_grid* grid, int location, const double* a, const double* b,
+                             double* out);
+
 /* ---------------------------------------------------------------- boundary conditions */
 /* solver/projection.cpp:966-971 apply_velocity_boundary_conditions */
 int of_apply_velocity_boundary_conditions(of_context* ctx, const of_grid* grid, const of_bc_set* bcs, double* u,

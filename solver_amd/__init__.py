@@ -101,7 +101,7 @@ _lib = None
 # exported C ABI (include/of_hip.h)
 EXPORTS = [
     "of_context_create", "of_context_destroy", "of_context_synchronize", "of_last_error", "of_build_info",
-    "of_profile_kernel", "of_profile_read", "of_profile_durations",
+    "of_profile_kernel", "of_profile_read", "of_profile_durations", "of_active_rms_difference",
     "of_apply_velocity_boundary_conditions", "of_apply_pressure_boundary_conditions",
     "of_apply_total_pressure_boundary_conditions", "of_derive_pressure_correction_boundary_conditions",
     "of_compute_gradient", "of_compute_divergence", "of_compute_laplacian", "of_compute_advection_term",

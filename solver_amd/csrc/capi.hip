@@ -283,6 +283,24 @@ int of_profile_durations(of_context* c, double* out_ms, int capacity, int* launc
   });
 }
 
+int of_active_rms_difference(of_context* c, const of_grid* g, int location, const double* a, const double* b,
+                             double* out) {
+  return guarded([&] {
+    validate_grid(g);
+    if(location < OF_CELL || location > OF_FACE_Z) throw InvalidArgument("unknown field location");
+    need(a, "a"), need(b, "b"), need(out, "out");
+    Context& x = C(c);
+    const Lay L = make_lay(*g, location);
+    double* diff = static_cast<double*>(x.scratch_get(15, L.size() * sizeof(double)));
+    launch_scale_active(x, L, 1.0, a, diff);
+    launch_axpy_active(x, L, -1.0, b, diff);
+    launch_norms(x, L, diff, kScalRed0);
+    OFH_CHECK(hipMemcpyAsync(x.host_scal, x.scal + kScalRed0, sizeof(double), hipMemcpyDeviceToHost, x.stream));
+    OFH_CHECK(hipStreamSynchronize(x.stream));
+    *out = std::sqrt(x.host_scal[0] / ((double)L.ax * (double)L.ay * (double)L.az));
+  });
+}
+
 int of_context_synchronize(of_context* c) {
   return guarded([&] { OFH_CHECK(hipStreamSynchronize(C(c).stream)); });
 }
