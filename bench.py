@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "3D TG 256³ steps/s mixed vs FP64 at ‖div u‖∞≤1e-10; HBM GB/s; adjoint s/step"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+PROFILE_STRIDE = 8  # live kernel timing samples every 8th launch of the kernel
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "pmc_traffic_tg256.md")
 
 
@@ -349,6 +350,9 @@ def run_precision(sa, ctx, n, nu, dt, precision, warmup, steps, prof_kernel, pre
     torch.cuda.synchronize()
     ctx.sync()
     ctx.profile_kernel(prof_kernel)
+    # every 8th launch is bracketed with events: an event record idles the device for a few
+    # microseconds, which timing every launch would add to the steps being measured
+    ctx.profile_stride(PROFILE_STRIDE)
     iters = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -436,9 +440,9 @@ def main():
                 "MGPCG iteration)", "kernel_bytes_per_step": kb, "kernel_bytes_GBs": kb / sec / 1e9,
                 "kernel_bytes_frac": kb / sec / 1e9 / HBM_PEAK_GBS}
 
-    # the launches that did work: after convergence the host has already queued one more
-    # iteration whose kernels return at entry (a few microseconds); they are not the kernel's
-    # duration and are left out of the average
+    # sampled launches (every PROFILE_STRIDE-th) that did work: after convergence the host has
+    # already queued one more iteration whose kernels return at entry (a few microseconds);
+    # they are not the kernel's duration and are left out of the average
     durs = r["durations"]
     med = sorted(durs)[len(durs) // 2] if durs else 0.0
     work = [d for d in durs if d > 0.25 * med]

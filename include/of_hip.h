@@ -146,6 +146,9 @@ int of_profile_read(of_context* ctx, int* launches, double* total_ms, double* mi
 /* The same record launch by launch: the first `capacity` durations in launch order (out_ms may
  * be NULL), the count in *launches; restarts the record. */
 int of_profile_durations(of_context* ctx, double* out_ms, int capacity, int* launches);
+/* Brackets only every stride-th launch of the timed kernel (default 1): each bracketing event
+ * record leaves the device idle for a few microseconds, so a throughput run samples. */
+int of_profile_stride(of_context* ctx, int stride);
 
 /* No reference counterpart: sqrt(mean((a - b)^2)) over the active entries of two fields of one
  * location (a deterministic device reduction; the manufactured-solution check uses it). */

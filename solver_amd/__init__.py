@@ -101,7 +101,8 @@ _lib = None
 # exported C ABI (include/of_hip.h)
 EXPORTS = [
     "of_context_create", "of_context_destroy", "of_context_synchronize", "of_last_error", "of_build_info",
-    "of_profile_kernel", "of_profile_read", "of_profile_durations", "of_active_rms_difference",
+    "of_profile_kernel", "of_profile_read", "of_profile_durations", "of_profile_stride",
+    "of_active_rms_difference",
     "of_apply_velocity_boundary_conditions", "of_apply_pressure_boundary_conditions",
     "of_apply_total_pressure_boundary_conditions", "of_derive_pressure_correction_boundary_conditions",
     "of_compute_gradient", "of_compute_divergence", "of_compute_laplacian", "of_compute_advection_term",
@@ -278,6 +279,10 @@ class Context:
         n, tot, mn, mx = C.c_int(), C.c_double(), C.c_double(), C.c_double()
         call("of_profile_read", self.h, C.byref(n), C.byref(tot), C.byref(mn), C.byref(mx))
         return dict(launches=n.value, total_ms=tot.value, min_ms=mn.value, max_ms=mx.value)
+
+    def profile_stride(self, stride):
+        """Time only every stride-th launch of the profiled kernel."""
+        call("of_profile_stride", self.h, int(stride))
 
     def profile_durations(self, capacity=1 << 16):
         """Per-launch durations (ms) of the profiled kernel since the last read, in launch order."""

@@ -246,6 +246,14 @@ int of_profile_kernel(of_context* c, const char* name) {
     OFH_CHECK(hipStreamSynchronize(x.stream));
     x.prof_name = name ? name : "";
     x.prof_used = 0;
+    x.prof_seen = 0;
+  });
+}
+
+int of_profile_stride(of_context* c, int stride) {
+  return guarded([&] {
+    if(stride < 1) throw InvalidArgument("the profiling stride must be at least 1");
+    C(c).prof_stride = stride;
   });
 }
 

@@ -46,7 +46,11 @@ struct Context {
   std::string prof_name;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
   size_t prof_used = 0;
-  bool prof_on(const char* name) const { return !prof_name.empty() && prof_name == name; }
+  int prof_stride = 1;    // bracket every prof_stride-th launch of the kernel
+  size_t prof_seen = 0;   // launches of the kernel since timing was switched on
+  // true for the launches to time: an event record leaves the device idle for a few
+  // microseconds, so a timed run samples every prof_stride-th launch
+  bool prof_on(const char* name) { return !prof_name.empty() && prof_name == name && prof_seen++ % prof_stride == 0; }
   void prof_begin(const char* name, hipEvent_t* end);
   ~Context();
 };

@@ -32,6 +32,7 @@
 #include "ofh.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -87,6 +88,7 @@ enum PS : int {
   P_COUNT = 32 + 8 * 24
 };
 constexpr int kHostMirror = 8;  // host mirror slot stride; P_DONE is published after every residual update
+constexpr int kHostSeq = 1;     // mirror slot of the sequence number (iteration + 1) of that update
 enum LS : int { L_MB = 0, L_SBW, L_MX2, L_MX4, L_MX5, L_SUMB, L_SPARE1, L_SPARE2 };
 
 struct PLevel {
@@ -443,8 +445,10 @@ struct EpiUpdate {
         ps[P_DONE] = 1.0;
       }
     }
-    // the host only polls the stop flag; the other scalars are copied back once at the end
+    // the host only polls the stop flag and this update's sequence number (stored after the
+    // flag, with release order); the other scalars are copied back once at the end
     __hip_atomic_store(host_out + P_DONE, ps[P_DONE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(host_out + kHostSeq, (double)(it + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 };
 
@@ -1233,6 +1237,304 @@ __global__ void __launch_bounds__(32 * ROWS) k_up_fused(const T* __restrict__ b,
   if(s + 1 <= last) step(s + 1, pb, pa, std::false_type{});
   if(s + 2 <= last) step(s + 2, pc, pb, std::false_type{});
   if(s + 3 <= last) step(s + 3, pa, pc, std::false_type{});
+  const int ops[2] = {kSum, kSum};
+  store_partials<2>(v, ops, partials);
+}
+
+// ---------------------------------------------------------------- small fused levels in one shot
+// On the small all-periodic levels (64^3 and 32^3 at 256^3) each block of the fused kernels
+// owns two planes and its march is six dependent plane steps (four of them halo planes) with
+// two barriers each, so those launches cost about 13 us for very little work. These variants
+// compute the same stages with the same per-cell expressions, in the same order, without the
+// march: every load of the six planes is issued at once, and the stages run one after another
+// over all their planes (three barrier-separated phases). LDS holds all six planes (141 KB in
+// FP64), so a block owns a compute unit; these levels have at most a few hundred blocks.
+template <typename T>
+__global__ void __launch_bounds__(512) k_down_small(const T* __restrict__ b, T* __restrict__ x2g, PLevel L, PLevel Cl,
+                                                    T* __restrict__ bc, double* partials, double* ps, int lslot,
+                                                    int zm, DownFold fold) {
+  if(stopped(ps)) return;
+  constexpr int ROWS = 16, kThreads = 32 * ROWS;
+  constexpr int BW = kFX + 4, BH = ROWS + 4, BP = BW * BH;  // x1, halo 2
+  constexpr int XW = kFX + 2, XH = ROWS + 2, XP = XW * XH;  // b - mb and x2, halo 1
+  constexpr int NB = (BP + kThreads - 1) / kThreads;
+  constexpr int NH = (XP + kThreads - 1) / kThreads;
+  constexpr int P1 = 6, P2 = 4;  // x1 on planes k0-2 .. k0+3; x2 and b - mb on k0-1 .. k0+2
+  __shared__ T sb[P1][BP];
+  __shared__ T sbt[P2][XP];
+  __shared__ T sx[P2][XP];
+  const int tid = threadIdx.x;
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const size_t plane = (size_t)nx * ny;
+  const size_t cplane = (size_t)Cl.nx * Cl.ny;
+  double mbd, sbw;
+  down_fold(fold, ps, lslot, (double)L.n, zm, mbd, sbw);
+  const T mb = T(mbd);
+  const T m1 = zm ? T((sbw - mbd * L.winv_sum) / (double)L.n) : T(0);
+  const T w = T(L.winv[0]);
+  const T w2 = w * T(L.wr);
+  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+  const T inv_children = T(1.0 / 8.0);
+  const FusedTile t = fused_tile<ROWS>(L, 2);
+  const int tx = tid & 63, ty = tid >> 6;
+  const int gi = t.i0 + tx, gj0 = t.j0 + 2 * ty;
+  const bool own0 = gi < nx && gj0 < ny, own1 = gi < nx && gj0 + 1 < ny;
+  size_t off[NB];
+  bool okb[NB];
+  int e1[NB];
+#pragma unroll
+  for(int m = 0; m < NB; ++m) {
+    const int e = tid + m * kThreads;
+    const int ex = e % BW, ey = e / BW;
+    const int gx = t.i0 + ex - 2, gy = t.j0 + ey - 2;
+    okb[m] = e < BP && gx < nx + 2 && gy < ny + 2;
+    off[m] = okb[m] ? (size_t)wrap_index(gx, nx) + (size_t)nx * wrap_index(gy, ny) : 0;
+    e1[m] = e < BP && ex >= 1 && ex <= XW && ey >= 1 && ey <= XH ? (ex - 1) + XW * (ey - 1) : -1;
+  }
+  int hb[NH];
+  long long hg[NH];
+#pragma unroll
+  for(int m = 0; m < NH; ++m) {
+    const int e = tid + m * kThreads;
+    const int xi = e % XW, xj = e / XW;
+    hb[m] = (xi + 1) + BW * (xj + 1);
+    const int ci = t.i0 + xi - 1, cj = t.j0 + xj - 1;
+    hg[m] = e < XP && xi >= 1 && xi <= kFX && xj >= 1 && xj <= ROWS && ci < nx && cj < ny
+                ? (long long)ci + (long long)nx * cj
+                : -1;
+  }
+  const size_t own_off0 = own0 ? (size_t)gi + (size_t)nx * gj0 : 0, own_off1 = own1 ? own_off0 + nx : 0;
+  // phase A: every load of the six planes, then x1 = (b - mb) w - m1 and b - mb into LDS
+  T raw[P1][NB], tb[2][2];
+#pragma unroll
+  for(int p = 0; p < P1; ++p) {
+    const size_t kz = plane * (size_t)wrap_index(t.k0 - 2 + p, nz);
+#pragma unroll
+    for(int m = 0; m < NB; ++m) raw[p][m] = b[off[m] + kz];
+  }
+#pragma unroll
+  for(int r = 0; r < 2; ++r) {
+    const size_t kz = plane * (size_t)(t.k0 + r);
+    tb[r][0] = b[own_off0 + kz];
+    tb[r][1] = b[own_off1 + kz];
+  }
+#pragma unroll
+  for(int p = 0; p < P1; ++p) {
+#pragma unroll
+    for(int m = 0; m < NB; ++m) {
+      const int e = tid + m * kThreads;
+      const T bt = okb[m] ? raw[p][m] - mb : T(0);
+      if(e < BP) sb[p][e] = bt * w - m1;
+      if(p >= 1 && p <= P2 && e1[m] >= 0) sbt[p - 1][e1[m]] = bt;
+    }
+  }
+  __syncthreads();
+  // phase B: x2 on planes k0-1 .. k0+2, tile + halo 1 (stored for the block's own planes)
+  double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for(int q = 0; q < P2; ++q) {
+    const int kq = t.k0 - 1 + q;
+    const T* x1m = sb[q];
+    const T* x1c = sb[q + 1];
+    const T* x1p = sb[q + 2];
+    const T* btc = sbt[q];
+    T* dst = sx[q];
+    const bool own = q == 1 || q == 2;
+    const size_t pq = plane * (size_t)(own ? kq : 0);
+#pragma unroll
+    for(int m = 0; m < NH; ++m) {
+      const int e = tid + m * kThreads;
+      if(e < XP) {
+        const int be = hb[m];
+        const T c = x1c[be];
+        const T ax = lap_neg<T>(c, x1c[be - 1], x1c[be + 1], x1c[be - BW], x1c[be + BW], x1m[be], x1p[be], ihx2, ihy2,
+                                ihz2);
+        const T out = c + w2 * (btc[e] + T(-1) * ax);
+        dst[e] = out;
+        if(own && hg[m] >= 0) {
+          x2g[(size_t)hg[m] + pq] = out;
+          v[0] += (double)out;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // phase C: residual on planes k0 and k0+1 (tile), restricted 2x2x2
+  T acc = T(0);
+#pragma unroll
+  for(int r = 0; r < 2; ++r) {
+    const T* sm = sx[r];
+    const T* sc = sx[r + 1];
+    const T* sp = sx[r + 2];
+    T xs[2];
+#pragma unroll
+    for(int rr = 0; rr < 2; ++rr) {
+      const int xe = (tx + 1) + XW * (2 * ty + rr + 1);
+      const T c = sc[xe];
+      const T ax = lap_neg<T>(c, sc[xe - 1], sc[xe + 1], sc[xe - XW], sc[xe + XW], sm[xe], sp[xe], ihx2, ihy2, ihz2);
+      const T bq = (rr == 0 ? own0 : own1) ? tb[r][rr] - mb : T(0);
+      T res = bq + T(-1) * ax;
+      res += __shfl_xor(res, 1, 64);  // x pair
+      xs[rr] = res;
+    }
+    const T ysum = xs[0] + xs[1];  // y pair
+    const T tot = r == 1 ? acc + ysum : ysum;
+    acc = tot;
+    if(r == 1 && (tx & 1) == 0 && own0) {
+      const int ic = gi >> 1, jc = gj0 >> 1, kc = t.k0 >> 1;
+      const T val = inv_children * tot;
+      bc[(size_t)ic + (size_t)Cl.nx * jc + cplane * kc] = val;
+      v[1] += (double)val;
+      v[2] += (double)val * Cl.winv[0];
+    }
+  }
+  const int ops[3] = {kSum, kSum, kSum};
+  store_partials<3>(v, ops, partials);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512) k_up_small(const T* __restrict__ b, const T* __restrict__ x2,
+                                                  const T* __restrict__ xc, PLevel L, PLevel Cl,
+                                                  T* __restrict__ x5g, double* partials, double* ps, int lslot,
+                                                  int cslot, const double* fold_src, unsigned fold_nb, int zm) {
+  if(stopped(ps)) return;
+  constexpr int ROWS = 16, kThreads = 32 * ROWS;
+  constexpr int AW = kFX + 4, AH = ROWS + 4, AP = AW * AH;  // x3, halo 2
+  constexpr int XW = kFX + 2, XH = ROWS + 2, XP = XW * XH;  // x4 and b - mb, halo 1
+  constexpr int NA = (AP + kThreads - 1) / kThreads, NX = (XP + kThreads - 1) / kThreads;
+  constexpr int P1 = 6, P2 = 4;  // x3 on planes k0-2 .. k0+3; x4 and b - mb on k0-1 .. k0+2
+  __shared__ T s3[P1][AP];
+  __shared__ T s4[P2][XP];
+  __shared__ T sbt[P2][XP];
+  const int tid = threadIdx.x;
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const size_t plane = (size_t)nx * ny;
+  const size_t cplane = (size_t)Cl.nx * Cl.ny;
+  const T mb = T(ps[lslot + L_MB]);
+  const T m2 = T(ps[lslot + L_MX2]);
+  double mcd;
+  if(fold_src != nullptr) {
+    double r[2];
+    fold_all<2>(fold_src, fold_nb, r);
+    mcd = zm ? r[0] / (double)Cl.n : 0.0;
+    if(blockIdx.x == 0 && tid == 0) ps[cslot + L_MX5] = mcd;
+  } else {
+    mcd = ps[cslot + L_MX5];
+  }
+  const T mc = T(mcd);
+  const T w = T(L.winv[0]);
+  const T w2 = w * T(L.wr);
+  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+  const FusedTile t = fused_tile<ROWS>(L, 2);
+  const int tx = tid & 63, ty = tid >> 6;
+  const int gi = t.i0 + tx, gj0 = t.j0 + 2 * ty;
+  const bool own0 = gi < nx && gj0 < ny, own1 = gi < nx && gj0 + 1 < ny;
+  unsigned offa[NA], offc[NA], offx[NX];
+  bool oka[NA], okx[NX];
+  int ha[NX];
+#pragma unroll
+  for(int m = 0; m < NA; ++m) {
+    const int e = tid + m * kThreads;
+    const int gx = t.i0 + (e % AW) - 2, gy = t.j0 + (e / AW) - 2;
+    oka[m] = e < AP && gx < nx + 2 && gy < ny + 2;
+    const int ii = oka[m] ? wrap_index(gx, nx) : 0, jj = oka[m] ? wrap_index(gy, ny) : 0;
+    offa[m] = (unsigned)ii + (unsigned)nx * (unsigned)jj;
+    offc[m] = (unsigned)(ii >> 1) + (unsigned)Cl.nx * (unsigned)(jj >> 1);
+  }
+#pragma unroll
+  for(int m = 0; m < NX; ++m) {
+    const int e = tid + m * kThreads;
+    const int gx = t.i0 + (e % XW) - 1, gy = t.j0 + (e / XW) - 1;
+    okx[m] = e < XP && gx < nx + 1 && gy < ny + 1;
+    offx[m] = okx[m] ? (unsigned)wrap_index(gx, nx) + (unsigned)nx * (unsigned)wrap_index(gy, ny) : 0u;
+    ha[m] = (e % XW + 1) + AW * (e / XW + 1);
+  }
+  const size_t own_off0 = own0 ? (size_t)gi + (size_t)nx * gj0 : 0, own_off1 = own1 ? own_off0 + nx : 0;
+  // phase A: every load, then x3 = (x2 - m2) + (xc - mc)[parent] and b - mb into LDS
+  T ra[P1][NA], rc[P1][NA], rx[P2][NX], tb[2][2];
+#pragma unroll
+  for(int p = 0; p < P1; ++p) {
+    const int kw = wrap_index(t.k0 - 2 + p, nz);
+    const size_t kz = plane * (size_t)kw, cz = cplane * (size_t)(kw >> 1);
+#pragma unroll
+    for(int m = 0; m < NA; ++m) {
+      ra[p][m] = x2[offa[m] + kz];
+      rc[p][m] = xc[offc[m] + cz];
+    }
+  }
+#pragma unroll
+  for(int q = 0; q < P2; ++q) {
+    const size_t kz = plane * (size_t)wrap_index(t.k0 - 1 + q, nz);
+#pragma unroll
+    for(int m = 0; m < NX; ++m) rx[q][m] = b[offx[m] + kz];
+  }
+#pragma unroll
+  for(int r = 0; r < 2; ++r) {
+    const size_t kz = plane * (size_t)(t.k0 + r);
+    tb[r][0] = b[own_off0 + kz];
+    tb[r][1] = b[own_off1 + kz];
+  }
+#pragma unroll
+  for(int p = 0; p < P1; ++p) {
+#pragma unroll
+    for(int m = 0; m < NA; ++m) {
+      const int e = tid + m * kThreads;
+      if(e < AP) s3[p][e] = oka[m] ? (ra[p][m] - m2) + (rc[p][m] - mc) : T(0);
+    }
+  }
+#pragma unroll
+  for(int q = 0; q < P2; ++q) {
+#pragma unroll
+    for(int m = 0; m < NX; ++m) {
+      const int e = tid + m * kThreads;
+      if(e < XP) sbt[q][e] = okx[m] ? rx[q][m] - mb : T(0);
+    }
+  }
+  __syncthreads();
+  // phase B: x4 on planes k0-1 .. k0+2, tile + halo 1
+#pragma unroll
+  for(int q = 0; q < P2; ++q) {
+    const T* x3m = s3[q];
+    const T* x3c = s3[q + 1];
+    const T* x3p = s3[q + 2];
+    const T* bq = sbt[q];
+    T* dst = s4[q];
+#pragma unroll
+    for(int m = 0; m < NX; ++m) {
+      const int e = tid + m * kThreads;
+      if(e < XP) {
+        const int ae = ha[m];
+        const T c = x3c[ae];
+        const T ax =
+            lap_neg<T>(c, x3c[ae - 1], x3c[ae + 1], x3c[ae - AW], x3c[ae + AW], x3m[ae], x3p[ae], ihx2, ihy2, ihz2);
+        dst[e] = c + w2 * (bq[e] + T(-1) * ax);
+      }
+    }
+  }
+  __syncthreads();
+  // phase C: x5 on planes k0 and k0+1 (tile)
+  double v[2] = {0.0, 0.0};
+#pragma unroll
+  for(int r = 0; r < 2; ++r) {
+    const int kr = t.k0 + r;
+    const T* sm = s4[r];
+    const T* sc = s4[r + 1];
+    const T* sp = s4[r + 2];
+#pragma unroll
+    for(int rr = 0; rr < 2; ++rr) {
+      const int xe = (tx + 1) + XW * (2 * ty + rr + 1);
+      const T c = sc[xe];
+      const T ax = lap_neg<T>(c, sc[xe - 1], sc[xe + 1], sc[xe - XW], sc[xe + XW], sm[xe], sp[xe], ihx2, ihy2, ihz2);
+      const bool own = rr == 0 ? own0 : own1;
+      const T bt = own ? tb[r][rr] - mb : T(0);
+      const T out = c + w * (bt + T(-1) * ax);
+      if(own) {
+        x5g[(rr == 0 ? own_off0 : own_off1) + plane * kr] = out;
+        v[0] += (double)out;
+        v[1] += (double)bt * (double)out;
+      }
+    }
+  }
   const int ops[2] = {kSum, kSum};
   store_partials<2>(v, ops, partials);
 }
@@ -2041,7 +2343,6 @@ class PoissonSolver {
   double* history = nullptr;  // residual history (device)
   double* host_ps = nullptr;  // mapped pinned mirror of the leading scalars (2 slots)
   double* host_ps_dev = nullptr;
-  hipEvent_t ev[2]{};
   PLevel* dlv = nullptr;      // level descriptors for the tail kernel
   int l0 = 0;                 // first level handled by the one-workgroup tail
   std::vector<char> fused;    // per level: the fused all-periodic kernels run it
@@ -2056,8 +2357,8 @@ class PoissonSolver {
   double omega = kOmega;  // Jacobi weight of every level's first pre-smoothing sweep
   // Device memory source: null -> hipMalloc per array (freed by the destructor); otherwise every
   // array is carved from the arena and nothing is allocated or freed. With an arena the host
-  // reads the stop flag from device memory after each iteration (no mapped pinned mirror and no
-  // events), so the solve allocates nothing at all.
+  // reads the stop flag from device memory after each iteration (no mapped pinned mirror), so
+  // the solve allocates nothing at all.
   DeviceArena* arena = nullptr;
   void dev_alloc(void** out, size_t bytes) {
     if(arena) *out = arena->take(bytes);
@@ -2075,6 +2376,20 @@ class PoissonSolver {
   // the array holding a level's smoothed correction after the up leg: the fused up kernel
   // writes it beside x2 (its halo is still being read by neighbouring tiles)
   void* x_final(int l) const { return l < l0 && fused[l] ? lxb[l] : lxa[l]; }
+  // Waits until the update of iteration `it` has published its stop flag in the mapped mirror
+  // and returns the flag. The host spins on the coherent mapping; after about a second without
+  // the sequence number it falls back to waiting for the whole stream (the update has then run).
+  double wait_mirror(int it) const {
+    const volatile double* hm = host_ps + (it % 2) * kHostMirror;
+    for(long spin = 0; hm[kHostSeq] < (double)(it + 1); ++spin) {
+      if(spin == (1L << 26)) {
+        OFH_CHECK(hipStreamSynchronize(c.stream));
+        if(hm[kHostSeq] < (double)(it + 1)) throw HipError("the residual update's stop flag did not arrive");
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return hm[P_DONE];
+  }
 };
 
 namespace {
@@ -2084,6 +2399,13 @@ namespace {
 bool fused_disabled() {
   const char* e = std::getenv("SOLVER_AMD_MG_UNFUSED");
   return e != nullptr && e[0] == '1';
+}
+
+// SOLVER_AMD_MG_SMALL=0 keeps the marching fused kernels on the small levels (two planes per
+// block) instead of the one-shot variants; read when a V-cycle is queued.
+bool small_levels() {
+  const char* e = std::getenv("SOLVER_AMD_MG_SMALL");
+  return e == nullptr || e[0] != '0';
 }
 
 // SOLVER_AMD_CG_UNFUSED=1 runs the search direction and the operator as two kernels
@@ -2290,8 +2612,6 @@ PoissonSolver::~PoissonSolver() {
   fr(work);
   fr(dlv);
   if(host_ps) OFH_RELEASE(hipHostFree(host_ps));
-  for(auto& e : ev)
-    if(e) OFH_RELEASE(hipEventDestroy(e));
 }
 
 void PoissonSolver::build() {
@@ -2375,7 +2695,6 @@ void PoissonSolver::build() {
     OFH_CHECK(hipHostMalloc(&host_ps, 2 * kHostMirror * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
     OFH_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&host_ps_dev), host_ps, 0));
     for(int q = 0; q < 2 * kHostMirror; ++q) host_ps[q] = 0.0;
-    for(auto& e : ev) OFH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   dev_alloc(reinterpret_cast<void**>(&dlv), lv.size() * sizeof(PLevel));
   OFH_CHECK(hipMemcpyAsync(dlv, lv.data(), lv.size() * sizeof(PLevel), hipMemcpyHostToDevice, c.stream));
@@ -2418,7 +2737,10 @@ void PoissonSolver::vcycle() {
       double* out = pbuf[which];
       {
         ProfScope prof_(c, "k_down_fused", l == 0);
-        if(rows_down == 16)
+        if(f.kz == 2 && small_levels())
+          OFH_LAUNCH((k_down_small<T>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<T*>(lxa[l]), P, C,
+                     static_cast<T*>(lb[l + 1]), out, ps, slot(l), zmi, pend);
+        else if(rows_down == 16)
           OFH_LAUNCH((k_down_fused<T, 16>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<T*>(lxa[l]),
                              P, C, static_cast<T*>(lb[l + 1]), out, ps, slot(l), zmi, f.kz, pend);
         else
@@ -2493,7 +2815,10 @@ void PoissonSolver::vcycle() {
       double* out = pbuf[which];
       {
         ProfScope prof_(c, "k_up_fused", l == 0);
-        if(rows_up == 16)
+        if(f.kz == 2 && small_levels())
+          OFH_LAUNCH((k_up_small<T>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<const T*>(lxa[l]), xc, P,
+                     C, static_cast<T*>(lxb[l]), out, ps, slot(l), slot(l + 1), up_src, up_nb, zmi);
+        else if(rows_up == 16)
           OFH_LAUNCH((k_up_fused<T, 16>), f.grid, dim3(512), 0, c.stream, bptr(l),
                              static_cast<const T*>(lxa[l]), xc, P, C, static_cast<T*>(lxb[l]), out, ps, slot(l),
                              slot(l + 1), f.kz, up_src, up_nb, zmi);
@@ -2549,6 +2874,10 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   const bool cg_fused = !cg_fused_disabled();
   const unsigned pgrid = std::min<unsigned>(div_up((size_t)F.n, 256 * kPointIlp), 8192u);
   OFH_CHECK(hipMemsetAsync(ps, 0, P_COUNT * sizeof(double), c.stream));
+  if(host_ps) {  // the previous solve has finished (its end synchronised the stream)
+    volatile double* hm = host_ps;
+    for(int q = 0; q < 2 * kHostMirror; ++q) hm[q] = 0.0;
+  }
   if(zero_guess)
     OFH_LAUNCH(k_setup1_zero, ll.grid, ll.block, 0, c.stream, rhs, Lc, F, r, pc, c.partials, ll.kz);
   else
@@ -2601,12 +2930,11 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
                                hipMemcpyDeviceToHost, c.stream));
       OFH_CHECK(hipStreamSynchronize(c.stream));
       if(done != 0.0) break;
-    } else {
-      OFH_CHECK(hipEventRecord(ev[it % 2], c.stream));
-      if(it > 0) {
-        OFH_CHECK(hipEventSynchronize(ev[(it - 1) % 2]));
-        if(reinterpret_cast<volatile double*>(host_ps)[((it - 1) % 2) * kHostMirror + P_DONE] != 0.0) break;
-      }
+    } else if(it > 0) {
+      // one iteration of look-ahead: the previous iteration's update has published its stop flag
+      // once its sequence number shows in the mapped mirror (no event in the queue: an event
+      // record left the device idle for about 5 us per iteration)
+      if(wait_mirror(it - 1) != 0.0) break;
     }
     vcycle<T>();
   }
