@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--adjoint-tl", type=int, default=0,
                     help="adjoint-versus-tangent-linear checks along this many seeded directions")
     ap.add_argument("--adjoint-only", action="store_true", help="skip the forward benchmark")
+    ap.add_argument("--adjoint-steps", type=int, default=0,
+                    help="steps of the adjoint workload instead of its own count (short profiling runs)")
     ap.add_argument("--predictor-cg-steps", type=int, default=20,
                     help="FP64 steps timed with the CG Helmholtz predictor beside the line solves (0: skip)")
     return ap.parse_args()
@@ -73,7 +75,7 @@ ADJOINT_CASES = {
 }
 
 
-def run_adjoint(sa, ctx, name, fd_dirs, tl_dirs=0):
+def run_adjoint(sa, ctx, name, fd_dirs, tl_dirs=0, steps_override=0):
     """Forward run with checkpoints, then the reverse sweep (window recompute + adjoint steps).
     Initial state: the Taylor-Green field plus the seeded divergence-free perturbation of RMS
     1e-2. Optional central finite differences of the final kinetic energy along seeded
@@ -84,6 +86,8 @@ def run_adjoint(sa, ctx, name, fd_dirs, tl_dirs=0):
 
     cs = ADJOINT_CASES[name]
     n, nu, steps, every = cs["n"], cs["nu"], cs["steps"], cs["every"]
+    if steps_override:
+        steps = steps_override
     prec = sa.MIXED if cs["mixed"] else sa.FP64
     L = 2 * math.pi
     dt, ns = sa.taylor_green_dt(n, n, n, 0.5, steps * (0.5 / (3 * n / L)) * (1 - 1e-14))
@@ -412,7 +416,8 @@ def main():
     assert nsteps == 100, nsteps
     ctx = sa.Context(0)
     if a.adjoint_only:
-        r = run_adjoint(sa, ctx, a.adjoint if a.adjoint != "none" else "c4", a.adjoint_fd, a.adjoint_tl)
+        r = run_adjoint(sa, ctx, a.adjoint if a.adjoint != "none" else "c4", a.adjoint_fd, a.adjoint_tl,
+                        a.adjoint_steps)
         print(json.dumps({"metric": "adjoint s/step", "value": r["reverse_s_per_step"], "unit": "s/step",
                           "higher_is_better": False, "adjoint": r}))
         return 0
@@ -515,7 +520,7 @@ def main():
     if a.adjoint != "none":
         for k in list(res):
             res[k].pop("fields", None)
-        r_adj = run_adjoint(sa, ctx, a.adjoint, a.adjoint_fd, a.adjoint_tl)
+        r_adj = run_adjoint(sa, ctx, a.adjoint, a.adjoint_fd, a.adjoint_tl, a.adjoint_steps)
         print(f"# adjoint {a.adjoint}: forward {r_adj['forward_s']:.2f} s, reverse {r_adj['reverse_s']:.2f} s "
               f"({r_adj['reverse_s_per_step'] * 1e3:.1f} ms/step)", flush=True)
         out["adjoint_s_per_step"] = r_adj["reverse_s_per_step"]
