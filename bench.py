@@ -181,10 +181,10 @@ def run_adjoint(sa, ctx, name, fd_dirs, tl_dirs=0, steps_override=0):
 def kernel_bytes(name, n_cells, mixed):
     f64, t = 8, (4 if mixed else 8)
     table = {
-        "k_dir_apply": t + 5 * f64,                     # z, d_old, p read; d, p, q written
+        "k_dir_apply": t + 2 * f64,                     # z, d_old read; d written
         "k_apply_dot": 2 * f64,                         # d read; q written
-        "k_direction": t + 2 * f64 + 2 * f64,           # z, d_old, p read; d_new, p written
-        "k_update": 3 * f64 + (4 if mixed else 0),      # r, q read; r (+ FP32 copy) written
+        "k_direction": t + 2 * f64,                     # z, d_old read; d written
+        "k_update": 3 * f64 + (4 if mixed else 0),      # r, d read; r (+ FP32 copy) written
         "k_down_fused": 2 * t + t / 8,                  # b read; x2, coarse b written
         "k_up_fused": 3 * t + t / 8,                    # b, x2, coarse x read; x5 written
         "k_smooth12": 2 * t,                            # b read; x2 written (unfused levels)
@@ -194,6 +194,8 @@ def kernel_bytes(name, n_cells, mixed):
         "k_predictor_rhs": 7 * f64,                     # q, 2 other components, p, A_prev; A, rhs
         "k_line_solve_x": 3 * 2 * f64,                  # 3 components: line read + written once
         "k_line_solve_s": 3 * 2 * f64,
+        "k_helm_conv_x": 2 * f64,                       # one component: line read + written once
+        "k_helm_conv_s": 2 * f64,
     }
     return table[name] * n_cells
 
@@ -214,12 +216,13 @@ def step_bytes_model(n, iters, mixed):
 def step_kernel_bytes(n, iters, mixed):
     """HBM bytes the launches of one step read and write by their own arrays (each kernel's
     distinct arrays read + written once, the fused kernels counted as fused): predictor RHS
-    3 x 7, ADI staging copies 3 x 2 + sweeps 9 x 2, pressure RHS 4, Poisson setup/finish 12
-    (+1.5 FP32 cast), correction 9, pressure update 4, diagnostics 6; per MGPCG iteration the
-    fused direction/operator 6 (5.5 mixed) and residual update 3 (3.5 mixed), and the fused
-    V-cycle levels 2.125 (down) + 3.125 (up) x 8/7 in the multigrid precision."""
-    fixed = 21 + 6 + 18 + 4 + 12 + (1.5 if mixed else 0.0) + 9 + 4 + 6
-    per_it = 9.0 + 5.25 * 8.0 / 7.0 * (0.5 if mixed else 1.0)
+    3 x 7, periodic line sweeps 9 x 2, pressure RHS 4, Poisson set-up 4 (+1.5 FP32 cast), the
+    pressure pass's write 1 and mean removal 2, correction 9, pressure update 4, diagnostics 6;
+    per MGPCG iteration the fused direction/operator 3 (2.5 mixed), the residual update with the
+    operator recomputed 3 (3.5 mixed), the pressure pass's read of the iteration's direction 1,
+    and the fused V-cycle levels 2.125 (down) + 3.125 (up) x 8/7 in the multigrid precision."""
+    fixed = 21 + 18 + 4 + 4 + (1.5 if mixed else 0.0) + 1 + 2 + 9 + 4 + 6
+    per_it = 7.0 + 5.25 * 8.0 / 7.0 * (0.5 if mixed else 1.0)
     return (fixed + iters * per_it) * n ** 3 * 8
 
 

@@ -511,41 +511,52 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
   const double* w = V.w;
   const Lay &Lu = V.Lu, &Lv = V.Lv, &Lw = V.Lw;
   // point values of plane k: the two face-velocity pairs of the other components (a0..a3,
-  // b0..b3, in the order of advect_point), p at the two cells of the gradient, A_prev
+  // b0..b3, in the order of advect_point), p at the two cells of the gradient, A_prev. Each is
+  // read at a per-thread offset within its plane plus (k + dz) planes, a uniform product (the
+  // 64-bit index products of Lay::at per point were a large share of the kernel's VALU work).
+  const double* A = c == 0 ? v : u;  // a0..a3
+  const double* B = c == 2 ? v : w;  // b0..b3
+  const Lay& LA = c == 0 ? Lv : Lu;
+  const Lay& LB = c == 2 ? Lv : Lw;
+  const size_t plA = (size_t)LA.sx * LA.sy, plB = (size_t)LB.sx * LB.sy, plP = (size_t)Lp.sx * Lp.sy;
+  size_t oa[4], ob[4];
+  int dza[4] = {0, 0, 0, 0}, dzb[4] = {0, 0, 1, 1};
+  if(c == 0) {
+    oa[0] = LA.at(ic - 1, jc, 0), oa[1] = LA.at(ic, jc, 0), oa[2] = LA.at(ic - 1, jc + 1, 0), oa[3] = LA.at(ic, jc + 1, 0);
+    ob[0] = LB.at(ic - 1, jc, 0), ob[1] = LB.at(ic, jc, 0), ob[2] = LB.at(ic - 1, jc, 0), ob[3] = LB.at(ic, jc, 0);
+  } else if(c == 1) {
+    oa[0] = LA.at(ic, jc - 1, 0), oa[1] = LA.at(ic, jc, 0), oa[2] = LA.at(ic + 1, jc - 1, 0), oa[3] = LA.at(ic + 1, jc, 0);
+    ob[0] = LB.at(ic, jc - 1, 0), ob[1] = LB.at(ic, jc, 0), ob[2] = LB.at(ic, jc - 1, 0), ob[3] = LB.at(ic, jc, 0);
+  } else {
+    oa[0] = LA.at(ic, jc, 0), oa[1] = LA.at(ic, jc, 0), oa[2] = LA.at(ic + 1, jc, 0), oa[3] = LA.at(ic + 1, jc, 0);
+    ob[0] = LB.at(ic, jc, 0), ob[1] = LB.at(ic, jc, 0), ob[2] = LB.at(ic, jc + 1, 0), ob[3] = LB.at(ic, jc + 1, 0);
+    dza[0] = dza[2] = dzb[0] = dzb[2] = -1;
+    dzb[1] = dzb[3] = 0;
+  }
+  const size_t op0 = Lp.at(ic, jc, 0), op1 = Lp.at(ic - di, jc - dj, 0), oq = L.at(ic, jc, 0);
+  const size_t ost = L.at(min(i, L.sx - 1), min(j, L.sy - 1), 0);  // store offset (threads in range)
   double a0, a1, a2, a3, b0, b1, b2, b3, p0, p1, pa;
   auto point_loads = [&](int k) {
-    if(c == 0) {
-      a0 = v[Lv.at(ic - 1, jc, k)];
-      a1 = v[Lv.at(ic, jc, k)];
-      a2 = v[Lv.at(ic - 1, jc + 1, k)];
-      a3 = v[Lv.at(ic, jc + 1, k)];
-      b0 = w[Lw.at(ic - 1, jc, k)];
-      b1 = w[Lw.at(ic, jc, k)];
-      b2 = w[Lw.at(ic - 1, jc, k + 1)];
-      b3 = w[Lw.at(ic, jc, k + 1)];
-    } else if(c == 1) {
-      a0 = u[Lu.at(ic, jc - 1, k)];
-      a1 = u[Lu.at(ic, jc, k)];
-      a2 = u[Lu.at(ic + 1, jc - 1, k)];
-      a3 = u[Lu.at(ic + 1, jc, k)];
-      b0 = w[Lw.at(ic, jc - 1, k)];
-      b1 = w[Lw.at(ic, jc, k)];
-      b2 = w[Lw.at(ic, jc - 1, k + 1)];
-      b3 = w[Lw.at(ic, jc, k + 1)];
-    } else {
-      a0 = u[Lu.at(ic, jc, k - 1)];
-      a1 = u[Lu.at(ic, jc, k)];
-      a2 = u[Lu.at(ic + 1, jc, k - 1)];
-      a3 = u[Lu.at(ic + 1, jc, k)];
-      b0 = v[Lv.at(ic, jc, k - 1)];
-      b1 = v[Lv.at(ic, jc, k)];
-      b2 = v[Lv.at(ic, jc + 1, k - 1)];
-      b3 = v[Lv.at(ic, jc + 1, k)];
-    }
-    p0 = p[Lp.at(ic, jc, k)];
-    p1 = p[Lp.at(ic - di, jc - dj, k - dk)];
-    pa = P.has_prev ? prev[L.at(ic, jc, k)] : 0.0;
+    a0 = A[oa[0] + plA * (size_t)(k + dza[0])];
+    a1 = A[oa[1] + plA * (size_t)(k + dza[1])];
+    a2 = A[oa[2] + plA * (size_t)(k + dza[2])];
+    a3 = A[oa[3] + plA * (size_t)(k + dza[3])];
+    b0 = B[ob[0] + plB * (size_t)(k + dzb[0])];
+    b1 = B[ob[1] + plB * (size_t)(k + dzb[1])];
+    b2 = B[ob[2] + plB * (size_t)(k + dzb[2])];
+    b3 = B[ob[3] + plB * (size_t)(k + dzb[3])];
+    p0 = p[op0 + plP * (size_t)k];
+    p1 = p[op1 + plP * (size_t)(k - dk)];
+    pa = P.has_prev ? prev[oq + plane * (size_t)k] : 0.0;
   };
+  // the per-plane LDS phases: each thread's elements of the (kRY + 2) x (kRX + 2) and
+  // kRY x (kRX + 2) patches (at most two each), decomposed once
+  constexpr int kE1 = (kRY + 2) * (kRX + 2), kE2 = kRY * (kRX + 2), kThreads = kRX * kRY;
+  static_assert(kE1 <= 2 * kThreads && kE2 <= 2 * kThreads, "two patch elements per thread");
+  const int e1b = tid + kThreads, e2b = tid + kThreads;
+  const int y1a = tid / (kRX + 2), x1a = tid - y1a * (kRX + 2), y1b = e1b / (kRX + 2), x1b = e1b - y1b * (kRX + 2);
+  const int y2a = y1a, x2a = x1a, y2b = y1b, x2b = x1b;  // the same decomposition (row length kRX + 2)
+  const bool has1b = e1b < kE1, has2a = tid < kE2, has2b = e2b < kE2;
   for(int kk = kb - 2; kk < kb + 2; ++kk) {
     fetch(kk);
     commit(kk);
@@ -559,20 +570,23 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
     __syncthreads();
     const int s0 = slot(k), sp = slot(k + 1), sm = slot(k - 1);
     // delta_z q over x/y halo 1, delta_y q over x halo 1 (sd_axis)
-    for(int e = tid; e < (kRY + 2) * (kRX + 2); e += kRX * kRY) {
-      const int hy = e / (kRX + 2), hx = e - hy * (kRX + 2);
+    auto dz_at = [&](int hy, int hx) {
       ssz[hy][hx] = (sq[sp][hy + 1][hx + 1] - 2.0 * sq[s0][hy + 1][hx + 1] + sq[sm][hy + 1][hx + 1]) * inv2;
-    }
-    for(int e = tid; e < kRY * (kRX + 2); e += kRX * kRY) {
-      const int hy = e / (kRX + 2), hx = e - hy * (kRX + 2);
+    };
+    auto dy_at = [&](int hy, int hx) {
       ssy[hy][hx] = (sq[s0][hy + 3][hx + 1] - 2.0 * sq[s0][hy + 2][hx + 1] + sq[s0][hy + 1][hx + 1]) * inv1;
-    }
+    };
+    dz_at(y1a, x1a);
+    if(has1b) dz_at(y1b, x1b);
+    if(has2a) dy_at(y2a, x2a);
+    if(has2b) dy_at(y2b, x2b);
     __syncthreads();
     // delta_y delta_z q over x halo 1 (mixed_sd with outer = y, inner = z)
-    for(int e = tid; e < kRY * (kRX + 2); e += kRX * kRY) {
-      const int hy = e / (kRX + 2), hx = e - hy * (kRX + 2);
+    auto dyz_at = [&](int hy, int hx) {
       sm12[hy][hx] = (ssz[hy + 2][hx] - 2.0 * ssz[hy + 1][hx] + ssz[hy][hx]) * inv1;
-    }
+    };
+    if(has2a) dyz_at(y2a, x2a);
+    if(has2b) dyz_at(y2b, x2b);
     __syncthreads();
     if(in) {
       const int cy = ty + 2, cx = tx + 2;
@@ -629,7 +643,7 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
       ff_prev = ff;
       const double a = (ef - wf) * P.idx + (nf - sf) * P.idy + (ff - bf) * P.idz;
       const double grad = (p0 - p1) * ih;
-      const size_t at = L.at(i, j, k);
+      const size_t at = ost + plane * (size_t)k;
       double r = qc;
       r += P.s_grad * grad;
       r += P.s_dif * dif;

@@ -20,7 +20,12 @@
 //  * Fusions: the first two pre-smoothing sweeps (the first starts from x = 0 and is pointwise);
 //    residual + restriction (2x2x2 children combined with lane shuffles, LDS and the k-march);
 //    prolongation + first post-smoothing sweep; the search-direction update + operator + dot
-//    product (+ the deferred pressure update of the previous iteration).
+//    product; the residual update recomputes q = A d from the stored direction instead of
+//    reading q back.
+//  * Pressure accumulation: every search direction of a solve is kept (a ring of buffers) with
+//    its alpha and mean, and one pass at the end adds them to p in iteration order. Each cell
+//    sees exactly the operations of a per-iteration update, so p is bitwise the same, but it is
+//    read and written once per solve instead of once per iteration.
 //  * All scalars (alpha, beta, means, norms, convergence flags) stay on the device. Kernels of
 //    iterations past convergence return immediately (flag check), so the host queues the next
 //    iteration before it has read the previous one's residual.
@@ -82,8 +87,6 @@ enum PS : int {
   P_MP,            // mean of the warm start p
   P_MSYS,          // removed rhs mean
   P_PFINAL,        // final mean of p
-  P_PEND,          // 1 while p += alpha d of the last residual update is still to be applied
-  P_DSEL,          // which direction buffer holds the current d
   P_LEVEL = 32,    // per level: 8 slots (see LS)
   P_COUNT = 32 + 8 * 24
 };
@@ -334,7 +337,6 @@ struct EpiSetup2 {  // initial residual, early exit (linsolve/poisson_solver.cpp
     ps[P_RMS] = rms;
     ps[P_REL] = rms > 0.0 ? 1.0 : 0.0;
     ps[P_ITER] = 0.0;
-    ps[P_PEND] = 0.0;
     history[0] = rms;
     if(rms <= tol) {
       ps[P_DONE] = 1.0;
@@ -395,7 +397,7 @@ struct EpiDir {  // means of d and q, d.q, alpha or breakdown (linsolve/poisson_
   static constexpr int NV = 3;
   double* ps;
   double n;
-  int zm, dsel;
+  int zm;
   __device__ int op(int) const { return kSum; }
   __device__ bool active() const { return !stopped(ps); }
   __device__ void operator()(const double (&r)[3]) const {
@@ -403,8 +405,6 @@ struct EpiDir {  // means of d and q, d.q, alpha or breakdown (linsolve/poisson_
     const double mq = zm ? r[1] / n : 0.0;
     ps[P_MD] = mdn;
     ps[P_MQ] = mq;
-    ps[P_PEND] = 0.0;
-    ps[P_DSEL] = (double)dsel;
     const double dq = r[2] - mdn * r[1];
     ps[P_DQ] = dq;
     if(fabs(dq) < kSmallNumber) {
@@ -417,11 +417,13 @@ struct EpiDir {  // means of d and q, d.q, alpha or breakdown (linsolve/poisson_
 
 // Residual RMS and convergence test (linsolve/poisson_solver.cpp:764-776). Always runs: it
 // also publishes the stop flag to the mapped host mirror, from which the host learns that the
-// solve has stopped.
+// solve has stopped. An update that runs records its iteration's alpha and direction mean for
+// the pressure accumulation (coef[2 it], coef[2 it + 1]).
 struct EpiUpdate {
   static constexpr int NV = 3;
   double* ps;
   double* history;
+  double* coef;
   double* host_out;
   double n, tol;
   int it, zm;
@@ -436,7 +438,8 @@ struct EpiUpdate {
       const double rms = sqrt(fmax(r[1] - n * m * m, 0.0) / n);
       ps[P_ITER] = (double)(it + 1);
       ps[P_RMS] = rms;
-      ps[P_PEND] = 1.0;
+      coef[2 * it] = ps[P_ALPHA];
+      coef[2 * it + 1] = ps[P_MD];
       const double rel = ps[P_R0] > 0.0 ? rms / ps[P_R0] : 0.0;
       ps[P_REL] = rel;
       history[it + 1] = rms;
@@ -461,7 +464,6 @@ struct EpiPfinal {  // final mean of p
   __device__ bool active() const { return true; }
   __device__ void operator()(const double (&r)[1]) const {
     ps[P_PFINAL] = zm ? r[0] / n : 0.0;
-    ps[P_PEND] = 0.0;
   }
 };
 
@@ -487,9 +489,10 @@ __global__ void k_setup1(const double* __restrict__ rhs_g, const double* __restr
   store_partials<2>(v, ops, partials);
 }
 
-// k_setup1 for a zero warm start: p_c = 0 without reading p.
+// k_setup1 for a zero warm start: p is neither read nor copied (the pressure accumulation
+// starts from zero).
 __global__ void k_setup1_zero(const double* __restrict__ rhs_g, Lay Lc, PLevel L, double* __restrict__ sys,
-                              double* __restrict__ pc, double* partials, int kz) {
+                              double* partials, int kz) {
   const Tile t = tile(L, kz);
   double v[2] = {0.0, 0.0};
   if(t.i < L.nx && t.j < L.ny) {
@@ -497,7 +500,6 @@ __global__ void k_setup1_zero(const double* __restrict__ rhs_g, Lay Lc, PLevel L
       const size_t id = (size_t)t.i + (size_t)L.nx * ((size_t)t.j + (size_t)L.ny * k);
       const double s = -rhs_g[Lc.at(t.i + 1, t.j + 1, k + 1)] + -1.0 * boundary_source(L, t.i, t.j, k);
       sys[id] = s;
-      pc[id] = 0.0;
       v[0] += s;
     }
   }
@@ -1864,38 +1866,31 @@ __global__ void __launch_bounds__(kTailThreads)
 }
 
 // ---------------------------------------------------------------- PCG vector kernels
-// Search direction (pointwise): d = (z - mz) + beta (d_old - md) (first: d = z - mz), and the
-// deferred p += alpha_prev (d_old - md) of the previous residual update.
+// Search direction (pointwise): d = (z - mz) + beta (d_old - md) (first: d = z - mz).
 constexpr int kPointIlp = 4;
 template <typename T>
 __global__ void __launch_bounds__(256) k_direction(const T* __restrict__ z, const double* __restrict__ d_old,
-                                                   double* __restrict__ d_new, double* __restrict__ p, int n,
-                                                   const double* ps, int first) {
+                                                   double* __restrict__ d_new, int n, const double* ps, int first) {
   if(stopped(ps)) return;
   const double mz = ps[P_LEVEL + L_MX5];
   const double md = ps[P_MD];
   const double beta = first ? 0.0 : ps[P_BETA];
-  const double alpha_prev = ps[P_ALPHA];
-  const int pend = !first && ps[P_PEND] != 0.0;
   const int stride = gridDim.x * 256 * kPointIlp;
   for(int base = blockIdx.x * 256 * kPointIlp + threadIdx.x; base < n; base += stride) {
-    double zt[kPointIlp], dv[kPointIlp], pv[kPointIlp];
+    double zt[kPointIlp], dv[kPointIlp];
     if(base + (kPointIlp - 1) * 256 < n) {
-      // whole chunk in range: unconditional loads (d_old and p are valid memory even when
-      // `first` or `pend` leaves them unused), so they are issued together
+      // whole chunk in range: unconditional loads (d_old is valid memory even when `first`
+      // leaves it unused), so they are issued together
 #pragma unroll
       for(int e = 0; e < kPointIlp; ++e) {
         const int id = base + e * 256;
         zt[e] = (double)z[id];
         dv[e] = d_old[id];
-        pv[e] = p[id];
       }
 #pragma unroll
       for(int e = 0; e < kPointIlp; ++e) {
-        const int id = base + e * 256;
         const double zc = zt[e] - mz;
-        d_new[id] = first ? zc : zc + beta * (dv[e] - md);
-        if(pend) p[id] = pv[e] + alpha_prev * (dv[e] - md);
+        d_new[base + e * 256] = first ? zc : zc + beta * (dv[e] - md);
       }
       continue;
     }
@@ -1903,17 +1898,8 @@ __global__ void __launch_bounds__(256) k_direction(const T* __restrict__ z, cons
     for(int e = 0; e < kPointIlp; ++e) {
       const int id = base + e * 256;
       if(id < n) {
-        zt[e] = (double)z[id] - mz;
-        dv[e] = first ? 0.0 : d_old[id];
-        pv[e] = pend ? p[id] : 0.0;
-      }
-    }
-#pragma unroll
-    for(int e = 0; e < kPointIlp; ++e) {
-      const int id = base + e * 256;
-      if(id < n) {
-        d_new[id] = first ? zt[e] : zt[e] + beta * (dv[e] - md);
-        if(pend) p[id] = pv[e] + alpha_prev * (dv[e] - md);
+        const double zc = (double)z[id] - mz;
+        d_new[id] = first ? zc : zc + beta * (d_old[id] - md);
       }
     }
   }
@@ -2008,13 +1994,13 @@ __global__ void __launch_bounds__(256) k_apply_dot(const double* __restrict__ d,
 // Search direction, operator and dot products in one pass over the grid (k_direction followed
 // by k_apply_dot without the round trip of d through HBM): d = (z - mz) + beta (d_old - md)
 // (first: z - mz) is formed wherever the stencil needs it (the cell, the next plane and the
-// lateral neighbours, the latter from z and d_old read through the cache), stored once, and
-// q = A d with the sums of d, q and d.q follows; the deferred p += alpha_prev (d_old - md) of
-// the previous residual update is applied to the cell. Every d is the expression of
-// k_direction, so d and q are the values of the two separate kernels.
+// lateral neighbours, the latter from z and d_old read through the cache) and stored once;
+// q = A d is formed for the sums of d, q and d.q and not stored (the residual update
+// recomputes it from the stored d). Every d is the expression of k_direction, so d and the sums
+// are the values of the two separate kernels.
 struct DirCoef {
-  double mz, md, beta, alpha_prev;
-  int first, pend;
+  double mz, md, beta;
+  int first;
   template <typename T>
   __device__ double d(T zv, double dold) const {
     const double zc = (double)zv - mz;
@@ -2024,8 +2010,7 @@ struct DirCoef {
 
 template <typename T, int KZ, bool UNI, bool FULL, bool SHFL>
 __device__ __forceinline__ void dir_apply_march(const T* __restrict__ z, const double* __restrict__ d_old,
-                                                double* __restrict__ d_new, double* __restrict__ p,
-                                                double* __restrict__ q, const PLevel& L, const HotTile& t,
+                                                double* __restrict__ d_new, const PLevel& L, const HotTile& t,
                                                 const DirCoef& cf, double (&v)[3]) {
   const ZWin<UNI> zw(L);
   const size_t plane = (size_t)L.nx * L.ny;
@@ -2055,12 +2040,12 @@ __device__ __forceinline__ void dir_apply_march(const T* __restrict__ z, const d
   const size_t kp0 = plane * (size_t)zw.kprev(t.k0);
   double prev = cf.d(z[col + kp0], d_old[col + kp0]);
   T zc = z[col + plane * t.k0];
-  double dc = d_old[col + plane * t.k0], pc = p[col + plane * t.k0];
+  double dc = d_old[col + plane * t.k0];
   Lat la;
   lat(plane * t.k0, la);
   const size_t kn0 = plane * (size_t)zw.knext(t.k0);
   T zn = z[col + kn0];
-  double dn = d_old[col + kn0], pn = p[col + kn0];
+  double dn = d_old[col + kn0];
 #pragma unroll
   for(int kk = 0; kk < KZ; ++kk) {
     const int k = t.k0 + kk;
@@ -2068,13 +2053,12 @@ __device__ __forceinline__ void dir_apply_march(const T* __restrict__ z, const d
       // software pipeline: the lateral inputs of plane k + 1 and the centre of plane k + 2
       Lat nla{};
       T nzn = T(0);
-      double ndn = 0.0, npn = 0.0;
+      double ndn = 0.0;
       if(kk + 1 < KZ && (FULL || k + 1 < t.k1)) {
         lat(plane * (size_t)(k + 1), nla);
         const size_t kn = plane * (size_t)zw.knext(k + 1);
         nzn = z[col + kn];
         ndn = d_old[col + kn];
-        npn = p[col + kn];
       }
       const double cur = cf.d(zc, dc);
       const double next = cf.d(zn, dn);
@@ -2091,20 +2075,15 @@ __device__ __forceinline__ void dir_apply_march(const T* __restrict__ z, const d
       const double sv = sgn(y.sm, cf.d(la.zs, la.ds)), nv = sgn(y.sp, cf.d(la.zn, la.dn));
       const double qv = lap_neg<double>(cur, wv, ev, sv, nv, zw.below(k, prev, cur), zw.above(k, next, cur), L.ihx2,
                                         L.ihy2, L.ihz2);
-      const size_t id = col + plane * k;
-      d_new[id] = cur;
-      q[id] = qv;
-      if(cf.pend) p[id] = pc + cf.alpha_prev * (dc - cf.md);
+      d_new[col + plane * k] = cur;
       v[0] += cur;
       v[1] += qv;
       v[2] += cur * qv;
       prev = cur;
       zc = zn;
       dc = dn;
-      pc = pn;
       zn = nzn;
       dn = ndn;
-      pn = npn;
       la = nla;
     }
   }
@@ -2112,35 +2091,33 @@ __device__ __forceinline__ void dir_apply_march(const T* __restrict__ z, const d
 
 template <typename T, int KZ, bool UNI>
 __global__ void __launch_bounds__(256) k_dir_apply(const T* __restrict__ z, const double* __restrict__ d_old,
-                                                   double* __restrict__ d_new, double* __restrict__ p,
-                                                   double* __restrict__ q, PLevel L, double* partials,
+                                                   double* __restrict__ d_new, PLevel L, double* partials,
                                                    const double* ps, int first) {
   if(stopped(ps)) return;
   DirCoef cf;
   cf.mz = ps[P_LEVEL + L_MX5];
   cf.md = ps[P_MD];
   cf.beta = first ? 0.0 : ps[P_BETA];
-  cf.alpha_prev = ps[P_ALPHA];
   cf.first = first;
-  cf.pend = !first && ps[P_PEND] != 0.0;
   const HotTile t = hot_tile<KZ>(L);
   double v[3] = {0.0, 0.0, 0.0};
   if(t.in) {
     const bool rowfull = t.i - (int)threadIdx.x + 64 <= L.nx;  // uniform over the wave
     if(t.k1 - t.k0 == KZ) {
-      if(rowfull) dir_apply_march<T, KZ, UNI, true, true>(z, d_old, d_new, p, q, L, t, cf, v);
-      else dir_apply_march<T, KZ, UNI, true, false>(z, d_old, d_new, p, q, L, t, cf, v);
+      if(rowfull) dir_apply_march<T, KZ, UNI, true, true>(z, d_old, d_new, L, t, cf, v);
+      else dir_apply_march<T, KZ, UNI, true, false>(z, d_old, d_new, L, t, cf, v);
     } else {
-      if(rowfull) dir_apply_march<T, KZ, UNI, false, true>(z, d_old, d_new, p, q, L, t, cf, v);
-      else dir_apply_march<T, KZ, UNI, false, false>(z, d_old, d_new, p, q, L, t, cf, v);
+      if(rowfull) dir_apply_march<T, KZ, UNI, false, true>(z, d_old, d_new, L, t, cf, v);
+      else dir_apply_march<T, KZ, UNI, false, false>(z, d_old, d_new, L, t, cf, v);
     }
   }
   const int ops[3] = {kSum, kSum, kSum};
   store_partials<3>(v, ops, partials);
 }
 
-// r = (r - mr) - alpha (q - mq); sums of r, r^2, r w/D; the matching p update is deferred to
-// the next direction kernel (or the final update).
+// r = (r - mr) - alpha (q - mq); sums of r, r^2, r w/D (q stored by k_apply_dot; the path
+// with separate direction and operator kernels). The matching p update is left to the pressure
+// accumulation.
 template <typename T, int KZ, bool UNI>
 __global__ void __launch_bounds__(256) k_update(double* __restrict__ r, const double* __restrict__ q,
                                                 T* __restrict__ b_lo, PLevel L, double* partials, const double* ps,
@@ -2176,38 +2153,157 @@ __global__ void __launch_bounds__(256) k_update(double* __restrict__ r, const do
   store_partials<3>(v, ops, partials);
 }
 
+// The residual update of the fused path: q = A d is recomputed from the stored direction by
+// the march of k_apply_dot (the same neighbour values in the same expression, so q is bitwise
+// the q whose sums k_dir_apply formed), then r = (r - mr) - alpha (q - mq), its FP32 copy
+// (mixed) and the sums of r, r^2, r w/D. Reading d with its stencil instead of q saves the
+// store and the load of q.
+template <typename T, int KZ, bool UNI, bool FULL, bool SHFL>
+__device__ __forceinline__ void update_apply_march(const double* __restrict__ d, double* __restrict__ r,
+                                                   T* __restrict__ b_lo, const PLevel& L, const HotTile& t,
+                                                   double alpha, double mr, double mq, int write_lo, double (&v)[3]) {
+  const ZWin<UNI> zw(L);
+  const size_t plane = (size_t)L.nx * L.ny;
+  const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+  const Nb x = nb_x(L, t.i), y = nb_y(L, t.j);
+  const size_t oxm = (size_t)x.cm + (size_t)L.nx * t.j, oxp = (size_t)x.cp + (size_t)L.nx * t.j;
+  const size_t oym = (size_t)t.i + (size_t)L.nx * y.cm, oyp = (size_t)t.i + (size_t)L.nx * y.cp;
+  const int tx = threadIdx.x;
+  const size_t oxe = tx == 0 ? oxm : tx == 63 ? oxp : col;  // SHFL: the edge lanes' outside neighbour
+  const int cxy = bclass(t.i, L.nx) + 4 * bclass(t.j, L.ny);
+  double prev = d[col + plane * zw.kprev(t.k0)];
+  double cur = d[col + plane * t.k0];
+  double rc = r[col + plane * t.k0];
+  double lw = d[(SHFL ? oxe : oxm) + plane * t.k0], le = SHFL ? 0.0 : d[oxp + plane * t.k0],
+         ls = d[oym + plane * t.k0], ln = d[oyp + plane * t.k0];
+  double next = d[col + plane * zw.knext(t.k0)];
+#pragma unroll
+  for(int kk = 0; kk < KZ; ++kk) {
+    const int k = t.k0 + kk;
+    if(FULL || k < t.k1) {
+      // software pipeline: plane k + 1's lateral neighbours and residual, plane k + 2's centre
+      double nlw = 0.0, nle = 0.0, nls = 0.0, nln = 0.0, nnext = 0.0, nr = 0.0;
+      if(kk + 1 < KZ && (FULL || k + 1 < t.k1)) {
+        const size_t pk1 = plane * (size_t)(k + 1);
+        nlw = d[(SHFL ? oxe : oxm) + pk1];
+        if(!SHFL) nle = d[oxp + pk1];
+        nls = d[oym + pk1];
+        nln = d[oyp + pk1];
+        nnext = d[col + plane * zw.knext(k + 1)];
+        nr = r[col + pk1];
+      }
+      double wv, ev;
+      if(SHFL) {
+        const double up = __shfl_up(cur, 1, 64), dn = __shfl_down(cur, 1, 64);
+        wv = tx == 0 ? sgn(x.sm, lw) : up;
+        ev = tx == 63 ? sgn(x.sp, lw) : dn;
+      } else {
+        wv = sgn(x.sm, lw);
+        ev = sgn(x.sp, le);
+      }
+      const double qv = lap_neg<double>(cur, wv, ev, sgn(y.sm, ls), sgn(y.sp, ln), zw.below(k, prev, cur),
+                                        zw.above(k, next, cur), L.ihx2, L.ihy2, L.ihz2);
+      const size_t id = col + plane * k;
+      const double rn = (rc - mr) + -alpha * (qv - mq);
+      r[id] = rn;
+      if(write_lo) b_lo[id] = T(rn);
+      v[0] += rn;
+      v[1] += rn * rn;
+      v[2] += rn * (UNI ? L.winv[0] : L.winv[cxy + 16 * bclass(k, L.nz)]);
+      prev = cur;
+      cur = next;
+      next = nnext;
+      lw = nlw;
+      le = nle;
+      ls = nls;
+      ln = nln;
+      rc = nr;
+    }
+  }
+}
+
+template <typename T, int KZ, bool UNI>
+__global__ void __launch_bounds__(256) k_update_apply(const double* __restrict__ d, double* __restrict__ r,
+                                                      T* __restrict__ b_lo, PLevel L, double* partials,
+                                                      const double* ps, int write_lo) {
+  if(stopped(ps)) return;
+  const double alpha = ps[P_ALPHA], mr = ps[P_MR], mq = ps[P_MQ];
+  const HotTile t = hot_tile<KZ>(L);
+  double v[3] = {0.0, 0.0, 0.0};
+  if(t.in) {
+    const bool rowfull = t.i - (int)threadIdx.x + 64 <= L.nx;  // uniform over the wave
+    if(t.k1 - t.k0 == KZ) {
+      if(rowfull) update_apply_march<T, KZ, UNI, true, true>(d, r, b_lo, L, t, alpha, mr, mq, write_lo, v);
+      else update_apply_march<T, KZ, UNI, true, false>(d, r, b_lo, L, t, alpha, mr, mq, write_lo, v);
+    } else {
+      if(rowfull) update_apply_march<T, KZ, UNI, false, true>(d, r, b_lo, L, t, alpha, mr, mq, write_lo, v);
+      else update_apply_march<T, KZ, UNI, false, false>(d, r, b_lo, L, t, alpha, mr, mq, write_lo, v);
+    }
+  }
+  const int ops[3] = {kSum, kSum, kSum};
+  store_partials<3>(v, ops, partials);
+}
+
 template <typename T>
 __global__ void k_cast(const double* __restrict__ in, T* __restrict__ out, int n, const double* ps) {
   if(stopped(ps)) return;
   for(int id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x) out[id] = T(in[id]);
 }
 
-// Pending pressure update of the last iteration, then the sum of p.
-__global__ void k_pfinal(double* __restrict__ pc, const double* __restrict__ d0, const double* __restrict__ d1, int n,
-                         double* partials, const double* ps) {
-  const int pend = ps[P_PEND] != 0.0;
-  const double* d = ps[P_DSEL] != 0.0 ? d1 : d0;
-  const double alpha = ps[P_ALPHA], md = ps[P_MD];
+// ---------------------------------------------------------------- pressure accumulation
+// The search directions of a solve live in a ring of R buffers: iteration k writes slot k % R.
+// The pressure p = p0 + sum_k alpha_k (d_k - md_k) is formed per cell in iteration order,
+// p = p + alpha_k (d_k - md_k), the update the reference applies after every iteration, so the
+// result is bitwise that of a per-iteration update. Only iterations whose residual update ran
+// count (the first ps[P_ITER]; a breakdown stops before its update). The slot pointers travel
+// in the kernel arguments, alpha_k and md_k in coef (written by EpiUpdate).
+constexpr int kMaxRing = 32;
+struct DirRing {
+  const double* d[kMaxRing];
+  const double* coef;
+  int R;
+};
+
+__device__ inline double accumulate(double p, const DirRing& ring, int id, int k0, int k1) {
+#pragma unroll 4
+  for(int k = k0; k < k1; ++k) p = p + ring.coef[2 * k] * (ring.d[k % ring.R][id] - ring.coef[2 * k + 1]);
+  return p;
+}
+
+// Folds directions [k0, min(k1, iterations)) into the compact p (from zero when from_zero) before
+// their slots are reused.
+__global__ void k_pflush(double* __restrict__ pc, DirRing ring, int k0, int k1, int n, int from_zero,
+                         const double* ps) {
+  const int kend = min(k1, (int)ps[P_ITER]);
+  for(int id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x)
+    pc[id] = accumulate(from_zero ? 0.0 : pc[id], ring, id, k0, kend);
+}
+
+// The last directions [k0, iterations) added to the compact p (or to zero), written into the
+// active range of the ghosted result, with the sum of p for its mean.
+__global__ void k_pfinal(double* __restrict__ p_g, const double* __restrict__ pc, DirRing ring, int k0,
+                         int from_zero, PLevel L, Lay Lc, double* partials, const double* ps) {
+  const int kend = (int)ps[P_ITER];
   double v[1] = {0.0};
-  for(int id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x) {
-    double p = pc[id];
-    if(pend) {
-      p = p + alpha * (d[id] - md);
-      pc[id] = p;
-    }
+  for(int id = blockIdx.x * blockDim.x + threadIdx.x; id < L.n; id += gridDim.x * blockDim.x) {
+    const double p = accumulate(from_zero ? 0.0 : pc[id], ring, id, k0, kend);
+    const int row = id / L.nx;
+    p_g[Lc.at(id - row * L.nx + 1, row % L.ny + 1, row / L.ny + 1)] = p;
     v[0] += p;
   }
   const int ops[1] = {kSum};
   store_partials<1>(v, ops, partials);
 }
 
-__global__ void k_pout(const double* __restrict__ pc, PLevel L, Lay Lc, double* __restrict__ p_g, const double* ps,
-                       int kz) {
+// Removes the final mean from the active range of p (problems without a Dirichlet face).
+__global__ void k_psub(double* __restrict__ p_g, PLevel L, Lay Lc, const double* ps, int kz) {
   const double m = ps[P_PFINAL];
   const Tile t = tile(L, kz);
   if(t.i >= L.nx || t.j >= L.ny) return;
-  for(int k = t.k0; k < t.k1; ++k)
-    p_g[Lc.at(t.i + 1, t.j + 1, k + 1)] = pc[(size_t)t.i + (size_t)L.nx * (t.j + (size_t)L.ny * k)] - m;
+  for(int k = t.k0; k < t.k1; ++k) {
+    const size_t gi = Lc.at(t.i + 1, t.j + 1, k + 1);
+    p_g[gi] = p_g[gi] - m;
+  }
 }
 
 // Poisson operator on ghosted storage with the actual BCs (linsolve/poisson_solver.cpp:244-269).
@@ -2335,8 +2431,11 @@ class PoissonSolver {
   int precision;
   bool zm = true;
   std::vector<PLevel> lv;
-  // finest-level CG vectors (compact FP64); two direction buffers alternate
-  double *pc = nullptr, *r = nullptr, *d[2] = {nullptr, nullptr}, *q = nullptr;
+  // finest-level CG vectors (compact FP64): the pressure, the residual and the ring of search
+  // directions (q only for the path with separate direction and operator kernels)
+  double *pc = nullptr, *r = nullptr, *q = nullptr;
+  std::vector<double*> ring;
+  double* coef = nullptr;  // alpha_k and md_k per iteration (capacity hist_cap - 1 iterations)
   // level arrays: b (rhs) and two x buffers, element type T
   std::vector<void*> lb, lxa, lxb;
   double* ps = nullptr;       // scalar block
@@ -2413,6 +2512,23 @@ bool small_levels() {
 bool cg_fused_disabled() {
   const char* e = std::getenv("SOLVER_AMD_CG_UNFUSED");
   return e != nullptr && e[0] == '1';
+}
+
+// Slots of the search-direction ring: directions are kept until the pressure pass adds them
+// to p, so a solve of at most R iterations reads and writes p once; a longer one folds R
+// directions into p every R iterations. About 4 GiB of directions (32 slots up to 256^3 cells,
+// 4 at 512^3, at least 2); 4 slots in a no-allocation arena. SOLVER_AMD_CG_RING (2 to 32),
+// read when a solver is built, overrides the count.
+int ring_slots(size_t cells, bool arena) {
+  const char* e = std::getenv("SOLVER_AMD_CG_RING");
+  if(e != nullptr && e[0] != 0) {
+    const int v = std::atoi(e);
+    if(v < 2 || v > kMaxRing) throw InvalidArgument("SOLVER_AMD_CG_RING must be between 2 and 32");
+    return v;
+  }
+  if(arena) return 4;
+  const size_t budget = (size_t)4 << 30;
+  return (int)std::min<size_t>(std::max<size_t>(budget / (cells * sizeof(double)), 2), kMaxRing);
 }
 
 // Jacobi weights of the FP32 V-cycle of a MIXED solve on a grid of dimension `dims`.
@@ -2597,9 +2713,9 @@ PoissonSolver::~PoissonSolver() {
   };
   fr(pc);
   fr(r);
-  fr(d[0]);
-  fr(d[1]);
+  for(double* p : ring) fr(p);
   fr(q);
+  fr(coef);
   for(void* p : lb) fr(p);
   for(void* p : lxa) fr(p);
   for(void* p : lxb) fr(p);
@@ -2670,9 +2786,8 @@ void PoissonSolver::build() {
   const size_t n0 = (size_t)lv[0].n;
   dev_alloc(reinterpret_cast<void**>(&pc), n0 * sizeof(double));
   dev_alloc(reinterpret_cast<void**>(&r), n0 * sizeof(double));
-  dev_alloc(reinterpret_cast<void**>(&d[0]), n0 * sizeof(double));
-  dev_alloc(reinterpret_cast<void**>(&d[1]), n0 * sizeof(double));
-  dev_alloc(reinterpret_cast<void**>(&q), n0 * sizeof(double));
+  ring.assign(ring_slots(n0, arena != nullptr), nullptr);
+  for(double*& dk : ring) dev_alloc(reinterpret_cast<void**>(&dk), n0 * sizeof(double));
   const size_t es = precision == OF_PRECISION_MIXED ? sizeof(float) : sizeof(double);
   for(size_t l = 0; l < lv.size(); ++l) {
     void *b = nullptr, *xa = nullptr, *xb = nullptr;
@@ -2861,7 +2976,10 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
                                      bool zero_guess) {
   if(hist_cap < maxit + 1) {
     if(history && !arena) OFH_CHECK(hipFree(history));
+    if(coef && !arena) OFH_CHECK(hipFree(coef));
+    history = coef = nullptr;
     dev_alloc(reinterpret_cast<void**>(&history), (size_t)(maxit + 1) * sizeof(double));
+    dev_alloc(reinterpret_cast<void**>(&coef), (size_t)2 * maxit * sizeof(double));
     hist_cap = maxit + 1;
   }
   const PLevel& F = lv[0];
@@ -2872,6 +2990,12 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   const double n0 = (double)F.n;
   const HotLaunch hf = hot_launch(F);
   const bool cg_fused = !cg_fused_disabled();
+  if(!cg_fused && q == nullptr) dev_alloc(reinterpret_cast<void**>(&q), (size_t)F.n * sizeof(double));
+  const int R = (int)ring.size();
+  DirRing dr{};
+  for(int k = 0; k < R; ++k) dr.d[k] = ring[k];
+  dr.coef = coef;
+  dr.R = R;
   const unsigned pgrid = std::min<unsigned>(div_up((size_t)F.n, 256 * kPointIlp), 8192u);
   OFH_CHECK(hipMemsetAsync(ps, 0, P_COUNT * sizeof(double), c.stream));
   if(host_ps) {  // the previous solve has finished (its end synchronised the stream)
@@ -2879,7 +3003,7 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
     for(int q = 0; q < 2 * kHostMirror; ++q) hm[q] = 0.0;
   }
   if(zero_guess)
-    OFH_LAUNCH(k_setup1_zero, ll.grid, ll.block, 0, c.stream, rhs, Lc, F, r, pc, c.partials, ll.kz);
+    OFH_LAUNCH(k_setup1_zero, ll.grid, ll.block, 0, c.stream, rhs, Lc, F, r, c.partials, ll.kz);
   else
     OFH_LAUNCH(k_setup1, ll.grid, ll.block, 0, c.stream, rhs, p, Lc, F, r, pc, c.partials, ll.kz);
   launch_finalize(c.stream, c.partials, ll.grid, EpiSetup1{ps, n0, zmi});
@@ -2891,39 +3015,51 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   if(mixed)  // FP32 copy of the initial residual: the rhs of the finest FP32 level
     OFH_LAUNCH(k_cast<T>, 2048, 256, 0, c.stream, r, static_cast<T*>(lb[0]), F.n, ps);
   vcycle<T>();
+  const dim3 pg(2048);  // launch shape of the pressure passes (fixes the order of p's sum)
+  int folded = 0;       // directions [0, folded) have been added to pc
   int it = 0;
   for(; it < maxit; ++it) {
+    if(it > 0 && it % R == 0) {  // slot it % R is about to be reused: fold the last R directions
+      OFH_LAUNCH(k_pflush, pg, dim3(256), 0, c.stream, pc, dr, it - R, it, F.n, folded == 0 && zero_guess ? 1 : 0,
+                 ps);
+      folded = it;
+    }
+    double* dnew = ring[it % R];
+    const double* dold = ring[(it + R - 1) % R];  // unused (not read into any result) when it == 0
     if(cg_fused) {
       ProfScope prof_(c, "k_dir_apply");
-      OFH_HOT_LAUNCH(k_dir_apply<T, hf, c.stream, static_cast<const T*>(x_final(0)), d[(it + 1) % 2], d[it % 2], pc,
-                     q, F, c.partials, ps, it == 0 ? 1 : 0);
+      OFH_HOT_LAUNCH(k_dir_apply<T, hf, c.stream, static_cast<const T*>(x_final(0)), dold, dnew, F, c.partials, ps,
+                     it == 0 ? 1 : 0);
     } else {
       {
         ProfScope prof_(c, "k_direction");
-        OFH_LAUNCH(k_direction<T>, dim3(pgrid), dim3(256), 0, c.stream, static_cast<const T*>(x_final(0)),
-                           d[(it + 1) % 2], d[it % 2], pc, F.n, ps, it == 0 ? 1 : 0);
+        OFH_LAUNCH(k_direction<T>, dim3(pgrid), dim3(256), 0, c.stream, static_cast<const T*>(x_final(0)), dold,
+                   dnew, F.n, ps, it == 0 ? 1 : 0);
       }
       ProfScope prof_(c, "k_apply_dot");
       if(hf.kz == 8) {
         if(hf.uni)
-          OFH_LAUNCH((k_apply_dot<8, true>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+          OFH_LAUNCH((k_apply_dot<8, true>), hf.grid, hf.block, 0, c.stream, dnew, q, F, c.partials, ps);
         else
-          OFH_LAUNCH((k_apply_dot<8, false>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+          OFH_LAUNCH((k_apply_dot<8, false>), hf.grid, hf.block, 0, c.stream, dnew, q, F, c.partials, ps);
       } else {
         if(hf.uni)
-          OFH_LAUNCH((k_apply_dot<2, true>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+          OFH_LAUNCH((k_apply_dot<2, true>), hf.grid, hf.block, 0, c.stream, dnew, q, F, c.partials, ps);
         else
-          OFH_LAUNCH((k_apply_dot<2, false>), hf.grid, hf.block, 0, c.stream, d[it % 2], q, F, c.partials, ps);
+          OFH_LAUNCH((k_apply_dot<2, false>), hf.grid, hf.block, 0, c.stream, dnew, q, F, c.partials, ps);
       }
     }
-    launch_finalize(c.stream, c.partials, hf.grid, EpiDir{ps, n0, zmi, it % 2});
+    launch_finalize(c.stream, c.partials, hf.grid, EpiDir{ps, n0, zmi});
     {
       ProfScope prof_(c, "k_update");
-      OFH_HOT_LAUNCH(k_update<T, hf, c.stream, r, q, mixed ? static_cast<T*>(lb[0]) : nullptr, F, c.partials, ps,
-                     mixed ? 1 : 0);
+      T* blo = mixed ? static_cast<T*>(lb[0]) : nullptr;
+      if(cg_fused)
+        OFH_HOT_LAUNCH(k_update_apply<T, hf, c.stream, dnew, r, blo, F, c.partials, ps, mixed ? 1 : 0);
+      else
+        OFH_HOT_LAUNCH(k_update<T, hf, c.stream, r, q, blo, F, c.partials, ps, mixed ? 1 : 0);
     }
     launch_finalize(c.stream, c.partials, hf.grid,
-                    EpiUpdate{ps, history, host_ps_dev + (it % 2) * kHostMirror, n0, tol, it, zmi});
+                    EpiUpdate{ps, history, coef, host_ps_dev + (it % 2) * kHostMirror, n0, tol, it, zmi});
     if(arena) {
       double done = 0.0;
       OFH_CHECK(hipMemcpyAsync(&done, host_ps_dev + (it % 2) * kHostMirror + P_DONE, sizeof(double),
@@ -2938,12 +3074,13 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
     }
     vcycle<T>();
   }
-  {
-    const dim3 pg(2048);
-    OFH_LAUNCH(k_pfinal, pg, dim3(256), 0, c.stream, pc, d[0], d[1], F.n, c.partials, ps);
-    launch_finalize(c.stream, c.partials, pg, EpiPfinal{ps, n0, zmi});
-  }
-  OFH_LAUNCH(k_pout, ll.grid, ll.block, 0, c.stream, pc, F, Lc, p, ps, ll.kz);
+  // The host has seen the stop flag of iteration it - 1 (or it), so at most one queued
+  // iteration follows the last one that ran: every direction the pressure needs is in the ring,
+  // and the folds already queued covered [0, folded) clipped at the iteration count.
+  OFH_LAUNCH(k_pfinal, pg, dim3(256), 0, c.stream, p, pc, dr, folded, folded == 0 && zero_guess ? 1 : 0, F, Lc,
+             c.partials, ps);
+  launch_finalize(c.stream, c.partials, pg, EpiPfinal{ps, n0, zmi});
+  if(zm) OFH_LAUNCH(k_psub, ll.grid, ll.block, 0, c.stream, p, F, Lc, ps, ll.kz);
   OFH_CHECK(hipGetLastError());
   std::vector<double> hps(P_COUNT);
   OFH_CHECK(hipMemcpyAsync(hps.data(), ps, P_COUNT * sizeof(double), hipMemcpyDeviceToHost, c.stream));

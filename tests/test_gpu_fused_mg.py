@@ -152,3 +152,39 @@ def test_fused_solve_matches_oracle_and_fft(ctx, precision, shape):
     scale = np.max(np.abs(exact))
     assert np.max(np.abs(act - ref[1:-1, 1:-1, 1:-1])) <= 1e-10 * scale
     assert np.max(np.abs(act - exact)) <= 1e-9 * scale
+
+
+def solve_ring(ctx, g, pb, rhs, p0, precision, ring):
+    """A solve with `ring` search-direction slots ("" for the default)."""
+    os.environ["SOLVER_AMD_CG_RING"] = ring
+    try:
+        o = sa.ProjectionOptions(1.0, 1.0, 600, 1e-12)
+        sol = dev(p0)
+        d = sa.PoissonDiagnostics()
+        ctx.run("of_solve_pressure_poisson", C.byref(g), C.byref(pb), C.byref(o), precision, sa.ptr(dev(rhs)),
+                sa.ptr(sol), C.byref(d), None, 0)
+    finally:
+        os.environ.pop("SOLVER_AMD_CG_RING", None)
+    return host(sol), d.iterations
+
+
+# The pressure is accumulated from the kept search directions in one pass at the end; with a
+# ring shorter than the iteration count the oldest directions are folded into p every R
+# iterations. Each cell sees the same sequence of additions either way, so the results are
+# bitwise equal, from a zero and from a nonzero warm start.
+@pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
+@pytest.mark.parametrize("pbc,shape", [([sa.P_PERIODIC] * 6, (64, 48, 40)), ([1, 1, 0, 0, 2, 2], (72, 40, 32))])
+def test_direction_ring_folds_are_bitwise_neutral(ctx, precision, pbc, shape):
+    nx, ny, nz = shape
+    g = sa.grid(nx, ny, nz, 1.0 / nx, 1.1 / ny, 0.9 / nz)
+    pb = sa.pressure_bc_set(pbc)
+    rng = np.random.default_rng(11)
+    shp = sa.storage_shape(nx, ny, nz, sa.CELL)
+    rhs = rng.standard_normal(shp)
+    for p0 in (np.zeros(shp), rng.standard_normal(shp)):
+        ref, its = solve_ring(ctx, g, pb, rhs, p0, precision, "")
+        assert its > 6  # several folds with 2 and 3 slots
+        for ring in ("2", "3"):
+            sol, its_r = solve_ring(ctx, g, pb, rhs, p0, precision, ring)
+            assert its_r == its
+            assert np.array_equal(sol, ref)
