@@ -1079,8 +1079,13 @@ __global__ void __launch_bounds__(32 * ROWS) k_down_fused(const T* __restrict__ 
   store_partials<3>(v, ops, partials);
 }
 
-template <typename T, int ROWS>
-__global__ void __launch_bounds__(32 * ROWS) k_up_fused(const T* __restrict__ b, const T* __restrict__ x2,
+// NBUF = 3: loads two planes ahead through three register buffers. NBUF = 2: one plane ahead
+// through two, which frees registers for more resident blocks (the launch bound asks for 4
+// waves per SIMD in FP64, two 512-thread blocks per compute unit, and 6 in FP32); the other
+// blocks' work covers the shorter lead.
+template <typename T, int ROWS, int NBUF = 3>
+__global__ void __launch_bounds__(32 * ROWS, NBUF == 2 ? (sizeof(T) == 4 ? 6 : 4) : 1)
+    k_up_fused(const T* __restrict__ b, const T* __restrict__ x2,
                                                         const T* __restrict__ xc, PLevel L, PLevel Cl,
                                                         T* __restrict__ x5g, double* partials, double* ps,
                                                         int lslot, int cslot, int kz, const double* fold_src,
@@ -1163,6 +1168,7 @@ __global__ void __launch_bounds__(32 * ROWS) k_up_fused(const T* __restrict__ b,
   double v[2] = {0.0, 0.0};
   T q1[2] = {T(0), T(0)}, q2[2] = {T(0), T(0)};  // tile rhs of planes kp - 1, kp - 2
   const int last = (t.k1 - t.k0) + 1;
+  constexpr int D = NBUF - 1;  // planes between a fetch and its LDS store
   auto step = [&](int s, Pre& cur, Pre& nxt, auto always) {  // ALWAYS: see k_down_fused
     const int kp = t.k0 + s;
     {
@@ -1181,7 +1187,7 @@ __global__ void __launch_bounds__(32 * ROWS) k_up_fused(const T* __restrict__ b,
     }
     const T tb0 = own0 ? cur.tb[0] - mb : T(0), tb1 = own1 ? cur.tb[1] - mb : T(0);
     __syncthreads();
-    if(decltype(always)::value || s + 2 <= last) fetch(kp + 2, nxt);
+    if(decltype(always)::value || s + D <= last) fetch(kp + D, nxt);
     if(s >= 0) {  // x4 on plane kq = kp - 1, tile + halo 1
       const int kq = kp - 1;
       const int sm = ring3(kq - 1), sc = ring3(kq), sp = ring3(kq + 1);
@@ -1226,19 +1232,31 @@ __global__ void __launch_bounds__(32 * ROWS) k_up_fused(const T* __restrict__ b,
     q1[0] = tb0;
     q1[1] = tb1;
   };
-  Pre pa, pb, pc;  // rotate as in k_down_fused
-  fetch(t.k0 - 2, pa);
-  fetch(t.k0 - 1, pb);
-  int s = -2;
-  for(; s + 4 <= last; s += 3) {  // every step of the group fetches
-    step(s, pa, pc, std::true_type{});
-    step(s + 1, pb, pa, std::true_type{});
-    step(s + 2, pc, pb, std::true_type{});
+  if constexpr(NBUF == 3) {
+    Pre pa, pb, pc;  // rotate as in k_down_fused
+    fetch(t.k0 - 2, pa);
+    fetch(t.k0 - 1, pb);
+    int s = -2;
+    for(; s + 4 <= last; s += 3) {  // every step of the group fetches
+      step(s, pa, pc, std::true_type{});
+      step(s + 1, pb, pa, std::true_type{});
+      step(s + 2, pc, pb, std::true_type{});
+    }
+    if(s <= last) step(s, pa, pc, std::false_type{});
+    if(s + 1 <= last) step(s + 1, pb, pa, std::false_type{});
+    if(s + 2 <= last) step(s + 2, pc, pb, std::false_type{});
+    if(s + 3 <= last) step(s + 3, pa, pc, std::false_type{});
+  } else {
+    Pre pa, pb;
+    fetch(t.k0 - 2, pa);
+    int s = -2;
+    for(; s + 2 <= last; s += 2) {  // every step of the pair fetches
+      step(s, pa, pb, std::true_type{});
+      step(s + 1, pb, pa, std::true_type{});
+    }
+    if(s <= last) step(s, pa, pb, std::false_type{});
+    if(s + 1 <= last) step(s + 1, pb, pa, std::false_type{});
   }
-  if(s <= last) step(s, pa, pc, std::false_type{});
-  if(s + 1 <= last) step(s + 1, pb, pa, std::false_type{});
-  if(s + 2 <= last) step(s + 2, pc, pb, std::false_type{});
-  if(s + 3 <= last) step(s + 3, pa, pc, std::false_type{});
   const int ops[2] = {kSum, kSum};
   store_partials<2>(v, ops, partials);
 }
@@ -2447,6 +2465,7 @@ class PoissonSolver {
   std::vector<char> fused;    // per level: the fused all-periodic kernels run it
   int resident_down = 0, resident_up = 0;  // blocks of the fused kernels the device holds at once
   int rows_down = 8, rows_up = 8;          // tile rows of the fused kernels (8: 256 threads, 16: 512)
+  int up_bufs = 3;                         // prefetch buffers of the 16-row fused up kernel (2 or 3)
   double* partials2 = nullptr;  // second partial buffer: a folding consumer writes beside its input
   // coarse solve data
   int nc = 0;
@@ -2557,6 +2576,19 @@ void mixed_omega(int dims, double& w1, double& w2) {
 bool fp64_chebyshev() {
   const char* e = std::getenv("SOLVER_AMD_FP64_CHEBYSHEV");
   return e != nullptr && e[0] == '1';
+}
+
+// Prefetch buffers of the 16-row fused up kernel per precision, SOLVER_AMD_UP_BUFFERS =
+// "fp64,mixed" (each 2 or 3), read when a solver is built. Defaults: 2 in FP64 (128 VGPRs, two
+// blocks per compute unit instead of one: the 256^3 step ran 17.34 -> 17.10 ms), 3 in FP32
+// (its two-buffer variant needs scratch spills to reach three blocks and ran slower).
+int up_buffers(int precision) {
+  int r[2] = {2, 3};
+  if(const char* e = std::getenv("SOLVER_AMD_UP_BUFFERS")) {
+    if(std::sscanf(e, "%d,%d", &r[0], &r[1]) != 2 || r[0] < 2 || r[0] > 3 || r[1] < 2 || r[1] > 3)
+      throw InvalidArgument("SOLVER_AMD_UP_BUFFERS must hold two buffer counts (2 or 3)");
+  }
+  return precision == OF_PRECISION_MIXED ? r[1] : r[0];
 }
 
 // Tile rows of the fused level kernels per precision, overridden by SOLVER_AMD_FUSED_ROWS =
@@ -2772,15 +2804,18 @@ void PoissonSolver::build() {
     fused[l] = uni && P.rx == 2 && P.ry == 2 && P.rz == 2 && P.nx >= 4 && P.ny >= 4 && P.nz >= 4 && !fused_disabled();
   }
   fused_rows(precision, rows_down, rows_up);
+  up_bufs = up_buffers(precision);
   if(precision == OF_PRECISION_MIXED) {
     resident_down = rows_down == 16 ? resident_blocks(k_down_fused<float, 16>, 512, c.device)
                                     : resident_blocks(k_down_fused<float, 8>, 256, c.device);
-    resident_up = rows_up == 16 ? resident_blocks(k_up_fused<float, 16>, 512, c.device)
+    resident_up = rows_up == 16 ? (up_bufs == 2 ? resident_blocks(k_up_fused<float, 16, 2>, 512, c.device)
+                                                : resident_blocks(k_up_fused<float, 16>, 512, c.device))
                                 : resident_blocks(k_up_fused<float, 8>, 256, c.device);
   } else {
     resident_down = rows_down == 16 ? resident_blocks(k_down_fused<double, 16>, 512, c.device)
                                     : resident_blocks(k_down_fused<double, 8>, 256, c.device);
-    resident_up = rows_up == 16 ? resident_blocks(k_up_fused<double, 16>, 512, c.device)
+    resident_up = rows_up == 16 ? (up_bufs == 2 ? resident_blocks(k_up_fused<double, 16, 2>, 512, c.device)
+                                                : resident_blocks(k_up_fused<double, 16>, 512, c.device))
                                 : resident_blocks(k_up_fused<double, 8>, 256, c.device);
   }
   const size_t n0 = (size_t)lv[0].n;
@@ -2933,6 +2968,10 @@ void PoissonSolver::vcycle() {
         if(f.kz == 2 && small_levels())
           OFH_LAUNCH((k_up_small<T>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<const T*>(lxa[l]), xc, P,
                      C, static_cast<T*>(lxb[l]), out, ps, slot(l), slot(l + 1), up_src, up_nb, zmi);
+        else if(rows_up == 16 && up_bufs == 2)
+          OFH_LAUNCH((k_up_fused<T, 16, 2>), f.grid, dim3(512), 0, c.stream, bptr(l),
+                             static_cast<const T*>(lxa[l]), xc, P, C, static_cast<T*>(lxb[l]), out, ps, slot(l),
+                             slot(l + 1), f.kz, up_src, up_nb, zmi);
         else if(rows_up == 16)
           OFH_LAUNCH((k_up_fused<T, 16>), f.grid, dim3(512), 0, c.stream, bptr(l),
                              static_cast<const T*>(lxa[l]), xc, P, C, static_cast<T*>(lxb[l]), out, ps, slot(l),
