@@ -1642,11 +1642,12 @@ __global__ void __launch_bounds__(kTailThreads)
   auto XB = [&](int l) -> T* { return LDS ? sT + B.oxb[l] : B.xb[l]; };
   double* work = LDS ? sbuf + B.owork : gwork;
   // The start's global reads (level l0's b when the levels are LDS-resident, a small coarsest
-  // system's row swaps, multipliers and U) are issued before the fold, so their latencies overlap
-  // with it; the barrier after the fold orders the LDS stores before any use. Staging the coarsest
-  // system lets the serial elimination below read no global memory.
-  __shared__ double cfac[kCoarseStage * kCoarseStage], cU[kCoarseStage * kCoarseStage];
-  __shared__ int cperm[kCoarseStage];
+  // system's solution operator) are issued before the fold, so their latencies overlap with it;
+  // the barrier after the fold orders the LDS stores before any use. A coarsest system of at most
+  // kCoarseStage unknowns is solved by its solution operator C (in `factor`, built on the host
+  // from the recorded elimination): one matrix-vector product on the first wave instead of the
+  // serial elimination and back substitution.
+  __shared__ double cop[kCoarseStage * kCoarseStage];
   static_assert(kCoarseStage * kCoarseStage <= kTailThreads, "one staged entry per thread");
   const int tid = threadIdx.x;
   const int n0 = lvs[l0].n;
@@ -1660,15 +1661,7 @@ __global__ void __launch_bounds__(kTailThreads)
   }
   const int ncs = lvs[nl - 1].n;
   const bool cstage = ncs <= kCoarseStage;
-  double fpre = 0.0, upre = 0.0;
-  int ppre = 0;
-  if(cstage) {
-    if(tid < ncs * ncs) {
-      fpre = factor[tid];
-      upre = U[tid];
-    }
-    if(tid < ncs) ppre = perm[tid];
-  }
+  const double cpre = cstage && tid < ncs * ncs ? factor[tid] : 0.0;
   __shared__ double lm[kMaxLevels][4];  // per level: mb, m2, m4, m5
   __shared__ double lsbw[kMaxLevels];   // sum of raw b w/D
   __shared__ double rz_sh;
@@ -1684,13 +1677,7 @@ __global__ void __launch_bounds__(kTailThreads)
     }
     for(int id = tid + kTailPre * (int)blockDim.x; id < n0; id += blockDim.x) dst[id] = B.b[l0][id];
   }
-  if(cstage) {
-    if(tid < ncs * ncs) {
-      cfac[tid] = fpre;
-      cU[tid] = upre;
-    }
-    if(tid < ncs) cperm[tid] = ppre;
-  }
+  if(cstage && tid < ncs * ncs) cop[tid] = cpre;
   if(tid == 0) {
     lm[l0][0] = mb0;
     lsbw[l0] = sbw0;
@@ -1795,45 +1782,55 @@ __global__ void __launch_bounds__(kTailThreads)
     const T* b = Bp(lc);
     T* x = XA(lc);
     const T mb = T(lm[lc][0]);
-    const int* pv = cstage ? cperm : perm;
-    const double* fv = cstage ? cfac : factor;
-    const double* uv = cstage ? cU : U;
     for(int i = G.tid; i < n; i += G.nt) work[i] = (double)(b[i] - mb);
     G.sync();
-    if(zm && G.tid == 0) work[n - 1] = 0.0;
-    G.sync();
-    for(int p = 0; p < n; ++p) {
-      if(G.tid == 0) {
-        const int best = pv[p];
-        if(best != p) {
-          const double t = work[p];
-          work[p] = work[best];
-          work[best] = t;
+    if(cstage) {  // x = C (b - mb), row i on lane i
+      double xi = 0.0;
+      if(G.tid < n)
+        for(int j = 0; j < n; ++j) xi += cop[G.tid * n + j] * work[j];
+      G.sync();
+      if(G.tid < n) work[G.tid] = xi;
+      if(G.tid == 0) lm[lc][3] = 0.0;
+      G.sync();
+    } else {  // the recorded elimination (linsolve/poisson_solver.cpp:482-567)
+      const int* pv = perm;
+      const double* fv = factor;
+      const double* uv = U;
+      if(zm && G.tid == 0) work[n - 1] = 0.0;
+      G.sync();
+      for(int p = 0; p < n; ++p) {
+        if(G.tid == 0) {
+          const int best = pv[p];
+          if(best != p) {
+            const double t = work[p];
+            work[p] = work[best];
+            work[best] = t;
+          }
         }
+        G.sync();
+        const double wp = work[p];
+        for(int row = p + 1 + G.tid; row < n; row += G.nt) {
+          const double f = fv[(size_t)row * n + p];
+          if(f != 0.0) work[row] -= f * wp;
+        }
+        G.sync();
       }
-      G.sync();
-      const double wp = work[p];
-      for(int row = p + 1 + G.tid; row < n; row += G.nt) {
-        const double f = fv[(size_t)row * n + p];
-        if(f != 0.0) work[row] -= f * wp;
+      if(G.tid == 0) {
+        for(int row = n - 1; row >= 0; --row) {
+          double s = work[row];
+          for(int c = row + 1; c < n; ++c) s -= uv[(size_t)row * n + c] * work[c];
+          work[row] = s / uv[(size_t)row * n + row];
+        }
+        if(zm) {
+          double s = 0.0;
+          for(int i = 0; i < n; ++i) s += work[i];
+          const double m = s / (double)n;
+          for(int i = 0; i < n; ++i) work[i] -= m;
+        }
+        lm[lc][3] = 0.0;
       }
       G.sync();
     }
-    if(G.tid == 0) {
-      for(int row = n - 1; row >= 0; --row) {
-        double s = work[row];
-        for(int c = row + 1; c < n; ++c) s -= uv[(size_t)row * n + c] * work[c];
-        work[row] = s / uv[(size_t)row * n + row];
-      }
-      if(zm) {
-        double s = 0.0;
-        for(int i = 0; i < n; ++i) s += work[i];
-        const double m = s / (double)n;
-        for(int i = 0; i < n; ++i) work[i] -= m;
-      }
-      lm[lc][3] = 0.0;
-    }
-    G.sync();
     double rz[1] = {0.0};
     for(int i = G.tid; i < n; i += G.nt) {
       const T xv = T(work[i]);
@@ -2736,6 +2733,38 @@ void factor_coarse(const PLevel& L, bool zm, std::vector<int>& perm, std::vector
   U = m;
 }
 
+// Solution operator of a small coarsest system: column j is the recorded elimination applied
+// to the unit vector e_j exactly as the device applies it to a right-hand side (the last entry
+// zeroed when the last row was replaced by ones, the mean removed afterwards), evaluated in long
+// double and rounded once. x = C (b - mb) is then the elimination's result to rounding.
+std::vector<double> coarse_operator(int n, bool zm, const std::vector<int>& perm, const std::vector<double>& factor,
+                                    const std::vector<double>& U) {
+  std::vector<double> C((size_t)n * n, 0.0);
+  std::vector<long double> w(n);
+  for(int col = 0; col < n; ++col) {
+    std::fill(w.begin(), w.end(), 0.0L);
+    w[col] = 1.0L;
+    if(zm) w[n - 1] = 0.0L;
+    for(int p = 0; p < n; ++p) {
+      if(perm[p] != p) std::swap(w[p], w[perm[p]]);
+      for(int row = p + 1; row < n; ++row) w[row] -= (long double)factor[(size_t)row * n + p] * w[p];
+    }
+    for(int row = n - 1; row >= 0; --row) {
+      long double s = w[row];
+      for(int c = row + 1; c < n; ++c) s -= (long double)U[(size_t)row * n + c] * w[c];
+      w[row] = s / (long double)U[(size_t)row * n + row];
+    }
+    if(zm) {
+      long double m = 0.0L;
+      for(int i = 0; i < n; ++i) m += w[i];
+      m /= (long double)n;
+      for(int i = 0; i < n; ++i) w[i] -= m;
+    }
+    for(int row = 0; row < n; ++row) C[(size_t)row * n + col] = (double)w[row];
+  }
+  return C;
+}
+
 }  // namespace
 
 PoissonSolver::~PoissonSolver() {
@@ -2852,6 +2881,7 @@ void PoissonSolver::build() {
   std::vector<double> hf, hU;
   factor_coarse(lv.back(), zm, hperm, hf, hU);
   nc = lv.back().n;
+  if(nc <= kCoarseStage) hf = coarse_operator(nc, zm, hperm, hf, hU);  // the tail reads C from `factor`
   dev_alloc(reinterpret_cast<void**>(&perm), nc * sizeof(int));
   dev_alloc(reinterpret_cast<void**>(&factor), (size_t)nc * nc * sizeof(double));
   dev_alloc(reinterpret_cast<void**>(&U), (size_t)nc * nc * sizeof(double));
