@@ -33,7 +33,6 @@ HIPCC_FLAGS = [
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
-    "-munsafe-fp-atomics",
 ]
 
 

@@ -348,28 +348,80 @@ __global__ void k_advection_tl(VP V, VP D, int c, of_grid g, int scheme, double*
   }
 }
 
-// Adjoint advection: scatters a_bar_c into the three velocity adjoints (FP64 atomic adds; the
-// summation order, and so the last bits, vary from run to run).
-__global__ void k_advection_adj(VP V, const double* abar, int c, of_grid g, int scheme, double* ub, double* vb,
-                                double* wb) {
+// One face of component c's flux divergence, normal to axis a, whose right sample has index
+// m = co[a] along a: the upper face of point co - e_a and the lower face of point co (the two
+// points compute its flux from the same operands). `lin` is its linearisation at the forward
+// values and fbar the adjoint of its flux, (abar(co - e_a) - abar(co)) / h over whichever of the
+// two points is active; `exists` is false when neither is an active point of c.
+struct FaceAdj {
+  FaceLin lin;
+  double fbar;
+  bool exists;
+};
+__device__ inline FaceAdj face_adj(const VP& V, const double* abar, int c, int a, const int (&co)[3],
+                                   const of_grid& g, int scheme) {
+  FaceAdj o{};
   const Lay& L = V.L[c];
-  double* bars[3] = {ub, vb, wb};
-  for(size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < L.size(); id += (size_t)gridDim.x * blockDim.x) {
-    const Idx3 p = decompose(L, id);
-    if(!active(L, p.i, p.j, p.k)) continue;
-    const double a = abar[id];
-    if(a == 0.0) continue;
-    FaceIn F[6];
-    faces_of(V, c, p.i, p.j, p.k, g, F);
-    for(int f = 0; f < 6; ++f) {
-      const FaceLin fl = lin_at(V, c, F[f], scheme);
-      const double t = F[f].sign_h * a;
-      const double dvel = 0.5 * fl.R * t;
-      atomicAdd(bars[F[f].cv] + F[f].va, dvel);
-      atomicAdd(bars[F[f].cv] + F[f].vb, dvel);
-      for(int s = 0; s < 4; ++s)
-        if(F[f].qv[s] && fl.dR[s] != 0.0) atomicAdd(bars[c] + F[f].q[s], fl.vel * fl.dR[s] * t);
-    }
+  const int ext[3] = {L.ax, L.ay, L.az};
+  for(int b = 0; b < 3; ++b)
+    if(co[b] < 1 || co[b] > ext[b] + (b == a ? 1 : 0)) return o;
+  o.exists = true;
+  const int m = co[a];
+  const int da[3] = {a == 0, a == 1, a == 2};
+  const double h = a == 0 ? g.dx : a == 1 ? g.dy : g.dz;
+  const size_t at = L.at(co[0], co[1], co[2]);
+  const size_t below = L.at(co[0] - da[0], co[1] - da[1], co[2] - da[2]);
+  // the lower point's upper face (+1/h) and the upper point's lower face (-1/h)
+  const double t_lo = m >= 2 ? (1.0 / h) * abar[below] : 0.0;
+  const double t_up = m <= ext[a] ? (-1.0 / h) * abar[at] : 0.0;
+  o.fbar = t_lo + t_up;
+  // samples ll, l, r, rr of q_c along a, inside storage
+  const double* q = V.q[c];
+  const ptrdiff_t step = a == 0 ? 1 : a == 1 ? (ptrdiff_t)L.sx : (ptrdiff_t)L.sx * L.sy;
+  const int smax = L.s(a) - 1;
+  double s4[4];
+  for(int t = 0; t < 4; ++t) {
+    const int along = m - 2 + t;
+    s4[t] = along >= 0 && along <= smax ? q[(ptrdiff_t)at + (ptrdiff_t)(t - 2) * step] : 0.0;
+  }
+  // face velocity: component a at co and at co - e_a (c = a) or co - e_c (c != a)
+  const Lay& Lv = V.L[a];
+  const int dc[3] = {c == 0, c == 1, c == 2};
+  const int* dv = c == a ? da : dc;
+  const double* vq = V.q[a];
+  const double vel = 0.5 * (vq[Lv.at(co[0] - dv[0], co[1] - dv[1], co[2] - dv[2])] + vq[Lv.at(co[0], co[1], co[2])]);
+  o.lin = face_lin(s4[0], s4[1], s4[2], s4[3], m, smax, vel, scheme);
+  return o;
+}
+
+// Adjoint advection in gather form, deterministic: for every storage point of component a,
+// the sum of what the tangent's faces take from it, each face's flux adjoint formed once from
+// the output adjoints on its two sides. A sample of u_a is (1) sample t of the reconstruction
+// of the faces of component a normal to each axis b whose right sample lies at its index + 2 - t
+// (vel dR_t fbar), and (2) a velocity sample of the faces of every component c normal to a at
+// its own position and at the position one step along c (R fbar / 2). The contributions are
+// added in a fixed order and accumulated into out.
+__global__ void k_advection_adj(VP V, VP A, int a, of_grid g, int scheme, double* out) {
+  const Lay& La = V.L[a];
+  for(size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < La.size(); id += (size_t)gridDim.x * blockDim.x) {
+    const Idx3 p = decompose(La, id);
+    const int q[3] = {p.i, p.j, p.k};
+    double acc = 0.0;
+    for(int b = 0; b < 3; ++b)
+      for(int t = 0; t < 4; ++t) {
+        int co[3] = {q[0], q[1], q[2]};
+        co[b] = q[b] + 2 - t;
+        const FaceAdj f = face_adj(V, A.q[a], a, b, co, g, scheme);
+        if(f.exists && f.lin.dR[t] != 0.0) acc += f.lin.vel * f.lin.dR[t] * f.fbar;
+      }
+    for(int c = 0; c < 3; ++c)
+      for(int s = 0; s < 2; ++s) {
+        int co[3] = {q[0], q[1], q[2]};
+        co[c] += s;
+        const FaceAdj f = face_adj(V, A.q[c], c, a, co, g, scheme);
+        if(f.exists) acc += 0.5 * f.lin.R * f.fbar;
+      }
+    out[id] += acc;
   }
 }
 
@@ -479,9 +531,9 @@ void launch_advection_tangent(Context& c, const of_grid& g, int scheme, const do
 void launch_advection_adjoint(Context& c, const of_grid& g, int scheme, const double* const* vel,
                               const double* const* abar, double* const* bar) {
   const VP V = vp_of(g, vel[0], vel[1], vel[2]);
+  const VP A = vp_of(g, abar[0], abar[1], abar[2]);
   for(int comp = 0; comp < 3; ++comp)
-    OFH_LAUNCH(k_advection_adj, stride_blocks(V.L[comp].size()), 256, 0, c.stream, V, abar[comp], comp, g,
-                       scheme, bar[0], bar[1], bar[2]);
+    OFH_LAUNCH(k_advection_adj, stride_blocks(V.L[comp].size()), 256, 0, c.stream, V, A, comp, g, scheme, bar[comp]);
 }
 
 // ------------------------------------------------------------------ checkpointed driver
