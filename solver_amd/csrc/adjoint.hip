@@ -22,7 +22,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <memory>
+#include <thread>
 #include <vector>
 
 namespace ofh {
@@ -562,6 +566,16 @@ struct AdjointRun {
   // tangent-linear state (allocated on first use)
   double *du[3]{}, *da[3]{}, *dp = nullptr, *dA[3]{}, *dr[3]{}, *dlap[3]{}, *dpred[3]{}, *dfc[3]{};
   double *dphi = nullptr, *dprhs = nullptr, *dg[3]{};
+  // Overlapped reverse sweep (allocated on the first backward that has room for it): a second
+  // stepper on a context of its own (stream, reduction buffers) recomputes window w - 1 on a
+  // worker thread while this thread runs the adjoint steps of window w, the two windows in
+  // alternating buffers. Both halves are latency-bound on a few compute units at the sizes
+  // where the copy fits, so they overlap almost entirely.
+  std::unique_ptr<Context> rctx;
+  std::unique_ptr<Stepper> rec;
+  std::vector<double*> window2;
+  std::vector<int> last_hp;  // has_prev of the last window's steps, kept by the forward pass
+  bool last_kept = false;    // `window` holds the last window's c^n (until a recompute overwrites it)
   std::vector<double*> owned;
 
   AdjointRun(Context& ctx, const of_stepper_config& cfg, int every_) : c(ctx), fwd(ctx, cfg), every(every_) {
@@ -589,6 +603,9 @@ struct AdjointRun {
   }
   ~AdjointRun() {
     OFH_RELEASE(hipStreamSynchronize(c.stream));
+    if(rctx) OFH_RELEASE(hipStreamSynchronize(rctx->stream));
+    rec.reset();
+    rctx.reset();
     for(double* p : owned) OFH_RELEASE(hipFree(p));
   }
   double* alloc(size_t n) {
@@ -600,15 +617,16 @@ struct AdjointRun {
   }
   Lay face(int a) const { return make_lay(fwd.g, OF_FACE_X + a); }
   Lay cell() const { return make_lay(fwd.g, OF_CELL); }
-  void copy(double* dst, const double* src, size_t n) {
-    OFH_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  void copy(double* dst, const double* src, size_t n, hipStream_t stream = nullptr) {
+    OFH_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, stream ? stream : c.stream));
   }
   void zero(double* p, size_t n) { OFH_CHECK(hipMemsetAsync(p, 0, n * sizeof(double), c.stream)); }
   void zero_inactive(double* p, const Lay& L) {
     OFH_LAUNCH(k_zero_inactive, stride_blocks(L.size()), 256, 0, c.stream, p, L);
   }
   void axpy(const Lay& L, double a, const double* x, double* y) { launch_axpy_active(c, L, a, x, y); }
-  double* state(int i) { return i < 3 ? fwd.vel[i] : i == 3 ? fwd.p_total : fwd.adv[i - 4]; }
+  static double* state(Stepper& s, int i) { return i < 3 ? s.vel[i] : i == 3 ? s.p_total : s.adv[i - 4]; }
+  double* state(int i) { return state(fwd, i); }
   size_t state_size(int i) const { return i < 3 ? lay[1 + i] : i == 3 ? lay[0] : lay[1 + (i - 4)]; }
 
   // out = Solve(rhs), computed as 2^e Solve(2^-e rhs) with 2^e the power of two nearest above
@@ -646,10 +664,47 @@ struct AdjointRun {
     k.has_prev = fwd.has_prev;
     k.m = fwd.metrics;
   }
-  void restore(const Ckpt& k) {
-    for(int i = 0; i < 7; ++i) copy(state(i), k.f[i], state_size(i));
-    fwd.has_prev = k.has_prev;
-    fwd.metrics = k.m;
+  void restore(const Ckpt& k, Stepper& s) {
+    for(int i = 0; i < 7; ++i) copy(state(s, i), k.f[i], state_size(i), s.c.stream);
+    s.has_prev = k.has_prev;
+    s.metrics = k.m;
+  }
+  void restore(const Ckpt& k) { restore(k, fwd); }
+
+  // Recomputes window w on stepper s (on its stream), keeping c^n = BCv(u^n) of every step in
+  // buf and has_prev in hp; the window's copies may still be in flight on return.
+  void recompute(Stepper& s, int w, std::vector<double*>& buf, std::vector<int>& hp) {
+    const int n0 = w * every, n1 = std::min(steps, n0 + every);
+    restore(ckpts[w], s);
+    for(int n = n0; n < n1; ++n) {
+      launch_velocity_bc(s.c, s.g, s.bcs, s.vel[0], s.vel[1], s.vel[2]);
+      for(int a = 0; a < 3; ++a) copy(buf[3 * (n - n0) + a], s.vel[a], lay[1 + a], s.c.stream);
+      hp[n - n0] = s.has_prev;
+      if(n + 1 < n1) s.step();
+    }
+  }
+
+  // Sets up the overlapped sweep when SOLVER_AMD_ADJOINT_OVERLAP is not "0" and the device has
+  // room for the second stepper and window with a wide margin (at 512^3 it does not, and the
+  // sweep stays serial). Returns whether the overlapped sweep is available.
+  bool overlap_ready() {
+    const char* e = std::getenv("SOLVER_AMD_ADJOINT_OVERLAP");
+    if(e != nullptr && e[0] == '0') return false;
+    if(rec) {
+      while(window2.size() < window.size()) window2.push_back(alloc(lay[1 + window2.size() % 3]));
+      return true;
+    }
+    size_t free_b = 0, total_b = 0;
+    OFH_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const size_t face = lay[1] * sizeof(double), cell = lay[0] * sizeof(double);
+    const size_t need = 18 * face + 8 * cell + ((size_t)4 << 30) + window.size() * face;
+    if((double)free_b < 1.25 * (double)need + (double)((size_t)8 << 30)) return false;
+    rctx = std::make_unique<Context>();
+    context_init(*rctx, c.device, nullptr);
+    rec = std::make_unique<Stepper>(*rctx, fwd.cfg);
+    while(window2.size() < window.size()) window2.push_back(alloc(lay[1 + window2.size() % 3]));
+    OFH_CHECK(hipStreamSynchronize(c.stream));
+    return true;
   }
 
   double forward(const double* const* u0, const double* p0, int nsteps, of_step_metrics* m) {
@@ -665,6 +720,8 @@ struct AdjointRun {
     const int wsteps = std::min(every, nsteps);
     while((int)window.size() < 3 * wsteps)
       for(int a = 0; a < 3; ++a) window.push_back(alloc(lay[1 + a]));
+    last_hp.assign(every, 0);
+    last_kept = false;
     steps = 0;
     // initial state: only active entries of the inputs are read
     for(int a = 0; a < 3; ++a) {
@@ -683,11 +740,21 @@ struct AdjointRun {
     fwd.has_prev = 0;
     std::memset(&fwd.metrics, 0, sizeof fwd.metrics);
     fwd.metrics.dt = fwd.cfg.dt;
+    // the last window's c^n = BCv(u^n) are kept as the forward pass makes them (the BC fill is
+    // idempotent, and the step applies it again), so the reverse sweep starts without
+    // recomputing that window
+    const int last0 = (nck - 1) * every;
     for(int n = 0; n < nsteps; ++n) {
       if(n % every == 0) save(ckpts[n / every]);
+      if(n >= last0) {
+        launch_velocity_bc(c, fwd.g, fwd.bcs, fwd.vel[0], fwd.vel[1], fwd.vel[2]);
+        for(int a = 0; a < 3; ++a) copy(window[3 * (n - last0) + a], fwd.vel[a], lay[1 + a]);
+        last_hp[n - last0] = fwd.has_prev;
+      }
       fwd.step();
     }
     for(int a = 0; a < 3; ++a) copy(ufinal[a], fwd.vel[a], lay[1 + a]);
+    last_kept = true;
     final_metrics = fwd.metrics;
     if(m) *m = fwd.metrics;
     steps = nsteps;
@@ -769,19 +836,62 @@ struct AdjointRun {
     }
     zero(pb, lay[0]);
     const int nwin = (steps + every - 1) / every;
-    std::vector<int> hp(every);
-    for(int w = nwin - 1; w >= 0; --w) {
+    auto adjoint_window = [&](int w, const std::vector<double*>& buf, const std::vector<int>& hp) {
       const int n0 = w * every, n1 = std::min(steps, n0 + every);
-      restore(ckpts[w]);  // recompute the window, keeping c^n of every step
-      for(int n = n0; n < n1; ++n) {
-        launch_velocity_bc(c, g, fwd.bcs, fwd.vel[0], fwd.vel[1], fwd.vel[2]);
-        for(int a = 0; a < 3; ++a) copy(window[3 * (n - n0) + a], fwd.vel[a], lay[1 + a]);
-        hp[n - n0] = fwd.has_prev;
-        if(n + 1 < n1) fwd.step();
-      }
       for(int n = n1 - 1; n >= n0; --n) {
-        const double* cv[3] = {window[3 * (n - n0)], window[3 * (n - n0) + 1], window[3 * (n - n0) + 2]};
+        const double* cv[3] = {buf[3 * (n - n0)], buf[3 * (n - n0) + 1], buf[3 * (n - n0) + 2]};
         adjoint_step(cv, hp[n - n0] != 0);
+      }
+    };
+    // the last window is in `window` as the forward pass left it, unless an earlier sweep's
+    // recomputes have written over it
+    if(!last_kept) {
+      recompute(fwd, nwin - 1, window, last_hp);
+      last_kept = true;
+    }
+    if(!overlap_ready()) {
+      std::vector<int> hp(every);
+      for(int w = nwin - 1; w >= 0; --w) {
+        if(w == nwin - 1) {
+          adjoint_window(w, window, last_hp);
+          continue;
+        }
+        last_kept = false;
+        recompute(fwd, w, window, hp);
+        adjoint_window(w, window, hp);
+      }
+    } else {
+      for(int a = 0; a < 3; ++a) rec->force[a] = fwd.force[a];  // read-only, shared
+      // window w in buffer (nwin - 1 - w) mod 2: the kept last window is in `window`
+      std::vector<double*>* buf[2] = {&window, &window2};
+      std::vector<int> hp[2] = {last_hp, std::vector<int>(every)};
+      auto slot = [&](int w) { return (nwin - 1 - w) % 2; };
+      for(int w = nwin - 1; w >= 0; --w) {
+        // window w - 1 is recomputed on the worker thread while the adjoint steps of window w
+        // run here; before the next round both streams are drained, so the buffer of window
+        // w is free for window w - 2
+        std::exception_ptr err;
+        std::thread worker;
+        if(w > 0 && slot(w - 1) == 0) last_kept = false;
+        if(w > 0)
+          worker = std::thread([&, w] {
+            try {
+              OFH_CHECK(hipSetDevice(c.device));
+              recompute(*rec, w - 1, *buf[slot(w - 1)], hp[slot(w - 1)]);  // slot 0 is `window`
+              OFH_CHECK(hipStreamSynchronize(rctx->stream));
+            } catch(...) {
+              err = std::current_exception();
+            }
+          });
+        try {
+          adjoint_window(w, *buf[slot(w)], hp[slot(w)]);
+          OFH_CHECK(hipStreamSynchronize(c.stream));
+        } catch(...) {
+          if(worker.joinable()) worker.join();
+          throw;
+        }
+        if(worker.joinable()) worker.join();
+        if(err) std::rethrow_exception(err);
       }
     }
     // u^0 = BCv(active input), p^0 = BCp(active input)

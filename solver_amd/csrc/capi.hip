@@ -197,6 +197,21 @@ AdjointRun& A(of_adjoint* a) {
 
 }  // namespace
 
+void ofh::context_init(Context& x, int device, hipStream_t stream) {
+  x.device = device;
+  OFH_CHECK(hipSetDevice(device));
+  if(stream) {
+    x.stream = stream;
+  } else {
+    OFH_CHECK(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    x.own_stream = true;
+  }
+  OFH_CHECK(hipMalloc(&x.partials, sizeof(double) * kMaxRed * kMaxPartials));
+  OFH_CHECK(hipMalloc(&x.scal, sizeof(double) * kScalCount));
+  OFH_CHECK(hipMemsetAsync(x.scal, 0, sizeof(double) * kScalCount, x.stream));
+  OFH_CHECK(hipHostMalloc(&x.host_scal, sizeof(double) * kScalCount, hipHostMallocDefault));
+}
+
 int ofh::guarded_call(const std::function<void()>& body) { return guarded(body); }
 
 extern "C" {
@@ -212,19 +227,7 @@ int of_context_create(int device, void* stream, of_context** out) {
     need(out, "out");
     auto* c = new of_context();
     try {
-      Context& x = c->ctx;
-      x.device = device;
-      OFH_CHECK(hipSetDevice(device));
-      if(stream) {
-        x.stream = static_cast<hipStream_t>(stream);
-      } else {
-        OFH_CHECK(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
-        x.own_stream = true;
-      }
-      OFH_CHECK(hipMalloc(&x.partials, sizeof(double) * kMaxRed * kMaxPartials));
-      OFH_CHECK(hipMalloc(&x.scal, sizeof(double) * kScalCount));
-      OFH_CHECK(hipMemsetAsync(x.scal, 0, sizeof(double) * kScalCount, x.stream));
-      OFH_CHECK(hipHostMalloc(&x.host_scal, sizeof(double) * kScalCount, hipHostMallocDefault));
+      context_init(c->ctx, device, static_cast<hipStream_t>(stream));
     } catch(...) {
       delete c;
       throw;

@@ -55,6 +55,11 @@ struct Context {
   ~Context();
 };
 
+// Sets up a context on `device`: its stream (`stream`, or a new non-blocking one of default
+// priority that the context owns), reduction partials and scalars. Throws HipError on failure;
+// the context's destructor frees whatever was set up.
+void context_init(Context& x, int device, hipStream_t stream);
+
 // Brackets one launch with timing events when `name` is the profiled kernel.
 struct ProfScope {
   Context& c;
