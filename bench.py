@@ -194,6 +194,7 @@ def kernel_bytes(name, n_cells, mixed):
         "k_predictor_rhs": 7 * f64,                     # q, 2 other components, p, A_prev; A, rhs
         "k_line_solve_x": 3 * 2 * f64,                  # 3 components: line read + written once
         "k_line_solve_s": 3 * 2 * f64,
+        "k_finish_periodic": 13 * f64,                  # pred x3, phi, prhs, old u x3, p read; u x3, p written
         "k_helm_conv_x": 2 * f64,                       # one component: line read + written once
         "k_helm_conv_s": 2 * f64,
     }
@@ -217,11 +218,12 @@ def step_kernel_bytes(n, iters, mixed):
     """HBM bytes the launches of one step read and write by their own arrays (each kernel's
     distinct arrays read + written once, the fused kernels counted as fused): predictor RHS
     3 x 7, periodic line sweeps 9 x 2, pressure RHS 4, Poisson set-up 4 (+1.5 FP32 cast), the
-    pressure pass's write 1 and mean removal 2, correction 9, pressure update 4, diagnostics 6;
+    pressure pass's write 1 and mean removal 2, correction with the pressure update and the
+    diagnostics in one pass 13;
     per MGPCG iteration the fused direction/operator 3 (2.5 mixed), the residual update with the
     operator recomputed 3 (3.5 mixed), the pressure pass's read of the iteration's direction 1,
     and the fused V-cycle levels 2.125 (down) + 3.125 (up) x 8/7 in the multigrid precision."""
-    fixed = 21 + 18 + 4 + 4 + (1.5 if mixed else 0.0) + 1 + 2 + 9 + 4 + 6
+    fixed = 21 + 18 + 4 + 4 + (1.5 if mixed else 0.0) + 1 + 2 + 13
     per_it = 7.0 + 5.25 * 8.0 / 7.0 * (0.5 if mixed else 1.0)
     return (fixed + iters * per_it) * n ** 3 * 8
 
