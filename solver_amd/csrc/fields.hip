@@ -450,10 +450,13 @@ __device__ inline double reconstruct4_1div(double ll, double left, double right,
 __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, const double* __restrict__ p, Lay Lp,
                                                                    const double* prev, double* adv,
                                                                    double* __restrict__ rhs, RhsParams P) {
-  __shared__ double sq[5][kRHY][kRHX];
-  __shared__ double ssz[kRY + 2][kRX + 2];
-  __shared__ double ssy[kRY][kRX + 2];
-  __shared__ double sm12[kRY][kRX + 2];
+  // q ring of six planes (k - 2 .. k + 2 in use while plane k + 3 is committed) and the second
+  // differences double-buffered by plane parity: with these, a plane needs two barriers (after
+  // its commit, after its second differences) and none at its end
+  __shared__ double sq[6][kRHY][kRHX];
+  __shared__ double ssz_[2][kRY + 2][kRX + 2];
+  __shared__ double ssy_[2][kRY][kRX + 2];
+  __shared__ double sm12_[2][kRY][kRX + 2];
   const int c = P.c;
   const Lay& L = c == 0 ? V.Lu : c == 1 ? V.Lv : V.Lw;
   const double* q = c == 0 ? V.u : c == 1 ? V.v : V.w;
@@ -470,7 +473,7 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
   const int ic = min(i, L.ax), jc = min(j, L.ay);  // load coordinates
   const double inv0 = P.fc.inv[0], inv1 = P.fc.inv[1], inv2 = P.fc.inv[2];
   const bool three_d = P.fc.three_d != 0;
-  auto slot = [](int kk) { return (kk + 10) % 5; };
+  auto slot = [](int kk) { return (kk + 12) % 6; };
   // staged plane elements of this thread: e = tid + m * threads
   constexpr int kPlaneElems = kRHY * kRHX;
   constexpr int kPE = (kPlaneElems + kRX * kRY - 1) / (kRX * kRY);
@@ -569,24 +572,31 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
     if(k + 1 < ke) fetch(k + 3);
     __syncthreads();
     const int s0 = slot(k), sp = slot(k + 1), sm = slot(k - 1);
-    // delta_z q over x/y halo 1, delta_y q over x halo 1 (sd_axis)
-    auto dz_at = [&](int hy, int hx) {
-      ssz[hy][hx] = (sq[sp][hy + 1][hx + 1] - 2.0 * sq[s0][hy + 1][hx + 1] + sq[sm][hy + 1][hx + 1]) * inv2;
+    auto (&ssz)[kRY + 2][kRX + 2] = ssz_[k & 1];
+    auto (&ssy)[kRY][kRX + 2] = ssy_[k & 1];
+    auto (&sm12)[kRY][kRX + 2] = sm12_[k & 1];
+    // delta_z q over x/y halo 1, delta_y q over x halo 1 (sd_axis), and delta_y delta_z q over
+    // x halo 1 (mixed_sd with outer = y, inner = z) from delta_z q values formed here by the
+    // same expression as ssz's, so no barrier separates it from them
+    auto dz = [&](int hy, int hx) {
+      return (sq[sp][hy + 1][hx + 1] - 2.0 * sq[s0][hy + 1][hx + 1] + sq[sm][hy + 1][hx + 1]) * inv2;
     };
     auto dy_at = [&](int hy, int hx) {
       ssy[hy][hx] = (sq[s0][hy + 3][hx + 1] - 2.0 * sq[s0][hy + 2][hx + 1] + sq[s0][hy + 1][hx + 1]) * inv1;
     };
-    dz_at(y1a, x1a);
-    if(has1b) dz_at(y1b, x1b);
-    if(has2a) dy_at(y2a, x2a);
-    if(has2b) dy_at(y2b, x2b);
-    __syncthreads();
-    // delta_y delta_z q over x halo 1 (mixed_sd with outer = y, inner = z)
     auto dyz_at = [&](int hy, int hx) {
-      sm12[hy][hx] = (ssz[hy + 2][hx] - 2.0 * ssz[hy + 1][hx] + ssz[hy][hx]) * inv1;
+      sm12[hy][hx] = (dz(hy + 2, hx) - 2.0 * dz(hy + 1, hx) + dz(hy, hx)) * inv1;
     };
-    if(has2a) dyz_at(y2a, x2a);
-    if(has2b) dyz_at(y2b, x2b);
+    ssz[y1a][x1a] = dz(y1a, x1a);
+    if(has1b) ssz[y1b][x1b] = dz(y1b, x1b);
+    if(has2a) {
+      dy_at(y2a, x2a);
+      dyz_at(y2a, x2a);
+    }
+    if(has2b) {
+      dy_at(y2b, x2b);
+      dyz_at(y2b, x2b);
+    }
     __syncthreads();
     if(in) {
       const int cy = ty + 2, cx = tx + 2;
@@ -657,7 +667,6 @@ __global__ void __launch_bounds__(kRX * kRY) k_predictor_rhs_tiled(VelPtrs V, co
       adv[at] = a;
       rhs[at] = r;
     }
-    __syncthreads();
   }
 }
 
