@@ -576,6 +576,13 @@ struct AdjointRun {
   std::vector<double*> window2;
   std::vector<int> last_hp;  // has_prev of the last window's steps, kept by the forward pass
   bool last_kept = false;    // `window` holds the last window's c^n (until a recompute overwrites it)
+  // Solver of the adjoint and tangent-linear steps. Those solves are linear solves of derivative
+  // fields, not part of the forward trajectory, so they need the tolerance, not the forward's
+  // arithmetic: behind an FP64 forward run they use the FP32 V-cycle preconditioner (the mixed
+  // mode's: FP64 outer CG, same relative tolerance), which needs about half the iterations.
+  // SOLVER_AMD_ADJOINT_SOLVE=fp64, read when the run is created, keeps the forward's solver.
+  PoissonSolver* dsolver = nullptr;
+  bool own_dsolver = false;
   std::vector<double*> owned;
 
   AdjointRun(Context& ctx, const of_stepper_config& cfg, int every_) : c(ctx), fwd(ctx, cfg), every(every_) {
@@ -598,6 +605,14 @@ struct AdjointRun {
     phib = alloc(lay[0]);
     prhsb = alloc(lay[0]);
     tmpc = alloc(lay[0]);
+    const char* e = std::getenv("SOLVER_AMD_ADJOINT_SOLVE");
+    const bool fp64_solves = e != nullptr && std::strcmp(e, "fp64") == 0;
+    if(cfg.precision == OF_PRECISION_FP64 && !fp64_solves) {
+      dsolver = poisson_create(c, g, fwd.pbcs, OF_PRECISION_MIXED);
+      own_dsolver = true;
+    } else {
+      dsolver = fwd.poisson;
+    }
     red = alloc(8);
     OFH_CHECK(hipStreamSynchronize(c.stream));
   }
@@ -606,6 +621,7 @@ struct AdjointRun {
     if(rctx) OFH_RELEASE(hipStreamSynchronize(rctx->stream));
     rec.reset();
     rctx.reset();
+    if(own_dsolver) poisson_destroy(dsolver);
     for(double* p : owned) OFH_RELEASE(hipFree(p));
   }
   double* alloc(size_t n) {
@@ -651,7 +667,7 @@ struct AdjointRun {
     (void)std::frexp(h[1], &e);  // h[1] = m 2^e with m in [0.5, 1)
     const double down = std::ldexp(1.0, -e), up = std::ldexp(1.0, e);
     launch_scale_active(c, Lc, down, rhs, rhs);
-    pr = poisson_solve(fwd.poisson, rhs, out, fwd.cfg.poisson_max_iterations, fwd.cfg.poisson_tolerance, false,
+    pr = poisson_solve(dsolver, rhs, out, fwd.cfg.poisson_max_iterations, fwd.cfg.poisson_tolerance, false,
                        true);  // out was zeroed above
     if(!pr.diag.converged) solve_failed(what, pr);
     launch_scale_active(c, Lc, up, rhs, rhs);
