@@ -2259,6 +2259,156 @@ __global__ void __launch_bounds__(256) k_update_apply(const double* __restrict__
   store_partials<3>(v, ops, partials);
 }
 
+// ---- the two CG stencil passes with the lateral neighbours shared through LDS
+// On a grid periodic on every axis whose 64 x 4 tiles and 8-plane ranges are all full, each
+// block stages one plane of the stencil operand at a time: its 4 rows plus the one-cell ring
+// around them (rows j0-1 and j0+4 loaded by the first and last wave, the two halo columns by
+// the second), formed once per cell and read by the neighbours from LDS instead of through the
+// cache. The plane buffers alternate by parity, so a plane needs one barrier: a buffer is
+// rewritten two planes later, after every thread has passed the barrier of the plane between.
+// Tile mapping, per-thread order, values and partial sums are those of k_dir_apply and
+// k_update_apply (the periodic full-row march), bitwise.
+constexpr int kLdsKz = 8;
+struct RingHalo {
+  int hr, hc;  // LDS row/column of this thread's ring cell (hr < 0: none)
+  size_t off;  // its column offset within a plane
+  __device__ RingHalo(const PLevel& L, int i0, int j0, int tx, int ty) : hr(-1), hc(-1), off(0) {
+    const int nx = L.nx, ny = L.ny;
+    if(ty == 0) {
+      hr = 0;
+      hc = tx + 1;
+      off = (size_t)(i0 + tx) + (size_t)nx * (size_t)(j0 == 0 ? ny - 1 : j0 - 1);
+    } else if(ty == 3) {
+      hr = 5;
+      hc = tx + 1;
+      off = (size_t)(i0 + tx) + (size_t)nx * (size_t)(j0 + 4 == ny ? 0 : j0 + 4);
+    } else if(ty == 1 && tx < 12) {
+      const int side = tx / 6;
+      hr = tx - 6 * side;
+      hc = side ? 65 : 0;
+      const int gi = side ? (i0 + 64 == nx ? 0 : i0 + 64) : (i0 == 0 ? nx - 1 : i0 - 1);
+      const int gj = j0 - 1 + hr;
+      off = (size_t)gi + (size_t)nx * (size_t)(gj < 0 ? gj + ny : (gj >= ny ? gj - ny : gj));
+    }
+  }
+};
+__device__ inline int wrap_plane(int k, int nz) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); }
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dir_apply_lds(const T* __restrict__ z, const double* __restrict__ d_old,
+                                                       double* __restrict__ d_new, PLevel L, double* partials,
+                                                       const double* ps, int first) {
+  if(stopped(ps)) return;
+  constexpr int KZ = kLdsKz;
+  __shared__ double sd[2][6][68];
+  DirCoef cf;
+  cf.mz = ps[P_LEVEL + L_MX5];
+  cf.md = ps[P_MD];
+  cf.beta = first ? 0.0 : ps[P_BETA];
+  cf.first = first;
+  const HotTile t = hot_tile<KZ>(L);
+  const int nz = L.nz, tx = threadIdx.x, ty = threadIdx.y;
+  const size_t plane = (size_t)L.nx * L.ny;
+  const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+  const RingHalo h(L, t.i - tx, t.j - ty, tx, ty);
+  const bool has_h = h.hr >= 0;
+  // centre values of planes k0-1 .. k0+KZ (three ahead of use), ring values one plane ahead
+  T zc[KZ + 2];
+  double dc[KZ + 2];
+  T zh[KZ];
+  double dh[KZ];
+  auto load_c = [&](int m) {
+    const size_t kp = plane * (size_t)wrap_plane(t.k0 - 1 + m, nz);
+    zc[m] = z[col + kp];
+    dc[m] = d_old[col + kp];
+  };
+  auto load_h = [&](int m) {
+    zh[m] = T(0);
+    dh[m] = 0.0;
+    if(has_h) {
+      const size_t kp = plane * (size_t)(t.k0 + m);
+      zh[m] = z[h.off + kp];
+      dh[m] = d_old[h.off + kp];
+    }
+  };
+  load_c(0);
+  load_c(1);
+  load_c(2);
+  load_h(0);
+  double prev = cf.d(zc[0], dc[0]);
+  double cur = cf.d(zc[1], dc[1]);
+  double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for(int kk = 0; kk < KZ; ++kk) {
+    if(kk + 3 < KZ + 2) load_c(kk + 3);
+    if(kk + 1 < KZ) load_h(kk + 1);
+    const double next = cf.d(zc[kk + 2], dc[kk + 2]);
+    double(&buf)[6][68] = sd[kk & 1];
+    buf[ty + 1][tx + 1] = cur;
+    if(has_h) buf[h.hr][h.hc] = cf.d(zh[kk], dh[kk]);
+    __syncthreads();
+    const double qv = lap_neg<double>(cur, buf[ty + 1][tx], buf[ty + 1][tx + 2], buf[ty][tx + 1], buf[ty + 2][tx + 1],
+                                      prev, next, L.ihx2, L.ihy2, L.ihz2);
+    d_new[col + plane * (size_t)(t.k0 + kk)] = cur;
+    v[0] += cur;
+    v[1] += qv;
+    v[2] += cur * qv;
+    prev = cur;
+    cur = next;
+  }
+  const int ops[3] = {kSum, kSum, kSum};
+  store_partials<3>(v, ops, partials);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_update_apply_lds(const double* __restrict__ d, double* __restrict__ r,
+                                                          T* __restrict__ b_lo, PLevel L, double* partials,
+                                                          const double* ps, int write_lo) {
+  if(stopped(ps)) return;
+  constexpr int KZ = kLdsKz;
+  __shared__ double sd[2][6][68];
+  const double alpha = ps[P_ALPHA], mr = ps[P_MR], mq = ps[P_MQ];
+  const HotTile t = hot_tile<KZ>(L);
+  const int nz = L.nz, tx = threadIdx.x, ty = threadIdx.y;
+  const size_t plane = (size_t)L.nx * L.ny;
+  const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
+  const RingHalo h(L, t.i - tx, t.j - ty, tx, ty);
+  const bool has_h = h.hr >= 0;
+  double dc[KZ + 2], rc[KZ], dh[KZ];
+  auto load_c = [&](int m) { dc[m] = d[col + plane * (size_t)wrap_plane(t.k0 - 1 + m, nz)]; };
+  auto load_p = [&](int m) {
+    const size_t kp = plane * (size_t)(t.k0 + m);
+    rc[m] = r[col + kp];
+    dh[m] = has_h ? d[h.off + kp] : 0.0;
+  };
+  load_c(0);
+  load_c(1);
+  load_c(2);
+  load_p(0);
+  double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for(int kk = 0; kk < KZ; ++kk) {
+    if(kk + 3 < KZ + 2) load_c(kk + 3);
+    if(kk + 1 < KZ) load_p(kk + 1);
+    const double cur = dc[kk + 1];
+    double(&buf)[6][68] = sd[kk & 1];
+    buf[ty + 1][tx + 1] = cur;
+    if(has_h) buf[h.hr][h.hc] = dh[kk];
+    __syncthreads();
+    const double qv = lap_neg<double>(cur, buf[ty + 1][tx], buf[ty + 1][tx + 2], buf[ty][tx + 1], buf[ty + 2][tx + 1],
+                                      dc[kk], dc[kk + 2], L.ihx2, L.ihy2, L.ihz2);
+    const size_t id = col + plane * (size_t)(t.k0 + kk);
+    const double rn = (rc[kk] - mr) + -alpha * (qv - mq);
+    r[id] = rn;
+    if(write_lo) b_lo[id] = T(rn);
+    v[0] += rn;
+    v[1] += rn * rn;
+    v[2] += rn * L.winv[0];
+  }
+  const int ops[3] = {kSum, kSum, kSum};
+  store_partials<3>(v, ops, partials);
+}
+
 template <typename T>
 __global__ void k_cast(const double* __restrict__ in, T* __restrict__ out, int n, const double* ps) {
   if(stopped(ps)) return;
@@ -2371,10 +2521,15 @@ LevelLaunch level_launch(const PLevel& L) {
 
 // Launch shape of the hot level kernels: 1D grid of 64x4 tiles, KZ = 8 planes per thread when
 // that still gives >= 1024 tiles, else 2.
+bool lds_cg_disabled() {
+  const char* e = std::getenv("SOLVER_AMD_CG_LDS");
+  return e != nullptr && e[0] == '0';
+}
 struct HotLaunch {
   dim3 grid, block;
   int kz;
   bool uni;
+  bool lds;  // the LDS-staged CG passes apply: periodic, all tiles and plane ranges full
 };
 HotLaunch hot_launch(const PLevel& L) {
   HotLaunch h;
@@ -2383,6 +2538,7 @@ HotLaunch hot_launch(const PLevel& L) {
   h.kz = plane * (size_t)((L.nz + 7) / 8) >= 1024 ? 8 : 2;
   h.grid = dim3((unsigned)(plane * (size_t)((L.nz + h.kz - 1) / h.kz)));
   h.uni = L.per[0] && L.per[1] && L.per[2];
+  h.lds = h.uni && h.kz == kLdsKz && L.nx % 64 == 0 && L.ny % 4 == 0 && L.nz % kLdsKz == 0 && !lds_cg_disabled();
   if(h.grid.x > (unsigned)kMaxPartials) throw InvalidArgument("grid too large for the reduction buffer");
   return h;
 }
@@ -3095,7 +3251,11 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
     }
     double* dnew = ring[it % R];
     const double* dold = ring[(it + R - 1) % R];  // unused (not read into any result) when it == 0
-    if(cg_fused) {
+    if(cg_fused && hf.lds) {
+      ProfScope prof_(c, "k_dir_apply");
+      OFH_LAUNCH(k_dir_apply_lds<T>, hf.grid, hf.block, 0, c.stream, static_cast<const T*>(x_final(0)), dold, dnew, F,
+                 c.partials, ps, it == 0 ? 1 : 0);
+    } else if(cg_fused) {
       ProfScope prof_(c, "k_dir_apply");
       OFH_HOT_LAUNCH(k_dir_apply<T, hf, c.stream, static_cast<const T*>(x_final(0)), dold, dnew, F, c.partials, ps,
                      it == 0 ? 1 : 0);
@@ -3122,7 +3282,10 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
     {
       ProfScope prof_(c, "k_update");
       T* blo = mixed ? static_cast<T*>(lb[0]) : nullptr;
-      if(cg_fused)
+      if(cg_fused && hf.lds)
+        OFH_LAUNCH(k_update_apply_lds<T>, hf.grid, hf.block, 0, c.stream, dnew, r, blo, F, c.partials, ps,
+                   mixed ? 1 : 0);
+      else if(cg_fused)
         OFH_HOT_LAUNCH(k_update_apply<T, hf, c.stream, dnew, r, blo, F, c.partials, ps, mixed ? 1 : 0);
       else
         OFH_HOT_LAUNCH(k_update<T, hf, c.stream, r, q, blo, F, c.partials, ps, mixed ? 1 : 0);

@@ -122,7 +122,7 @@ def test_fused_cg_matches_two_kernel_path(ctx, precision, pbc, shape):
 
 
 @pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
-@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 32)])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 32), (128, 128, 128)])
 def test_fused_solve_matches_oracle_and_fft(ctx, precision, shape):
     """The fused levels (64-wide tiles, several levels above the tail) against the oracle's
     restatement of the reference MGPCG and against the exact periodic solution (FFT): FP64
@@ -188,3 +188,41 @@ def test_direction_ring_folds_are_bitwise_neutral(ctx, precision, pbc, shape):
             sol, its_r = solve_ring(ctx, g, pb, rhs, p0, precision, ring)
             assert its_r == its
             assert np.array_equal(sol, ref)
+
+
+def solve_lds(ctx, g, rhs, p0, precision, lds):
+    """A periodic solve with the LDS-staged CG passes on (lds=True) or the marching ones."""
+    os.environ["SOLVER_AMD_CG_LDS"] = "1" if lds else "0"
+    try:
+        o = sa.ProjectionOptions(1.0, 1.0, 400, 1e-12 if precision == sa.FP64 else 1e-10)
+        pb = sa.pressure_bc_set([sa.P_PERIODIC] * 6)
+        sol = dev(p0)
+        d = sa.PoissonDiagnostics()
+        ctx.run("of_solve_pressure_poisson", C.byref(g), C.byref(pb), C.byref(o), precision, sa.ptr(dev(rhs)),
+                sa.ptr(sol), C.byref(d), None, 0)
+    finally:
+        os.environ.pop("SOLVER_AMD_CG_LDS", None)
+    return host(sol), d
+
+
+# On periodic grids whose 64 x 4 tiles and 8-plane ranges are all full (and at least 1024 of
+# them), the search-direction and residual passes stage the lateral neighbours in LDS. They form
+# every value by the marching kernels' expressions with the same tiles and per-thread order, so
+# the solves are bitwise those of the marching path (SOLVER_AMD_CG_LDS=0), iteration for
+# iteration, from a zero and from a nonzero warm start.
+@pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
+@pytest.mark.parametrize("shape,h", [((128, 128, 128), (1.0, 1.0, 1.0)), ((64, 256, 128), (1.0, 0.7, 1.3))])
+def test_lds_cg_passes_are_bitwise_the_marching_ones(ctx, precision, shape, h):
+    nx, ny, nz = shape
+    assert (nx // 64) * (ny // 4) * (nz // 8) >= 1024
+    g = sa.grid(nx, ny, nz, h[0] / nx, h[1] / ny, h[2] / nz)
+    rng = np.random.default_rng(nx + 3 * ny + 7 * nz)
+    shp = sa.storage_shape(nx, ny, nz, sa.CELL)
+    rhs = rng.standard_normal(shp)
+    for p0 in (np.zeros(shp), rng.standard_normal(shp)):
+        a, da = solve_lds(ctx, g, rhs, p0, precision, True)
+        b, db = solve_lds(ctx, g, rhs, p0, precision, False)
+        assert da.converged == db.converged == 1
+        assert da.iterations == db.iterations > 5
+        assert da.relative_residual == db.relative_residual
+        assert np.array_equal(a, b)
