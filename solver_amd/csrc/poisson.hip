@@ -927,14 +927,6 @@ __global__ void __launch_bounds__(32 * ROWS) k_down_fused(const T* __restrict__ 
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
   const size_t cplane = (size_t)Cl.nx * Cl.ny;
-  double mbd, sbw;
-  down_fold(fold, ps, lslot, (double)L.n, zm, mbd, sbw);
-  const T mb = T(mbd);
-  const T m1 = zm ? T((sbw - mbd * L.winv_sum) / (double)L.n) : T(0);
-  const T w = T(L.winv[0]);
-  const T w2 = w * T(L.wr);  // second sweep
-  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
-  const T inv_children = T(1.0 / 8.0);
   const FusedTile t = fused_tile<ROWS>(L, kz);
   const int tx = tid & 63, ty = tid >> 6;
   const int gi = t.i0 + tx, gj0 = t.j0 + 2 * ty;
@@ -979,6 +971,20 @@ __global__ void __launch_bounds__(32 * ROWS) k_down_fused(const T* __restrict__ 
     tb[0] = b[own_off0 + kz];
     tb[1] = b[own_off1 + kz];
   };
+  // three register buffers rotate: a step stores plane kp and fetches plane kp + 2. The first
+  // two planes are requested before the finer level's partials are folded, so the two round
+  // trips overlap.
+  T pa[NB], pb[NB], pc[NB], ta[2], tb[2], tc[2];
+  fetch(t.k0 - 2, pa, ta);
+  fetch(t.k0 - 1, pb, tb);
+  double mbd, sbw;
+  down_fold(fold, ps, lslot, (double)L.n, zm, mbd, sbw);
+  const T mb = T(mbd);
+  const T m1 = zm ? T((sbw - mbd * L.winv_sum) / (double)L.n) : T(0);
+  const T w = T(L.winv[0]);
+  const T w2 = w * T(L.wr);  // second sweep
+  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+  const T inv_children = T(1.0 / 8.0);
   double v[3] = {0.0, 0.0, 0.0};
   T acc = T(0);
   T q1[2] = {T(0), T(0)}, q2[2] = {T(0), T(0)};  // tile rhs of planes kp - 1, kp - 2
@@ -1060,10 +1066,6 @@ __global__ void __launch_bounds__(32 * ROWS) k_down_fused(const T* __restrict__ 
     q1[0] = tb0;
     q1[1] = tb1;
   };
-  // three register buffers rotate: a step stores plane kp and fetches plane kp + 2
-  T pa[NB], pb[NB], pc[NB], ta[2], tb[2], tc[2];
-  fetch(t.k0 - 2, pa, ta);
-  fetch(t.k0 - 1, pb, tb);
   int s = -2;
   for(; s + 4 <= last; s += 3) {  // every step of the group fetches
     step(s, pa, ta, pc, tc, std::true_type{});
@@ -1102,23 +1104,6 @@ __global__ void __launch_bounds__(32 * ROWS, NBUF == 2 ? (sizeof(T) == 4 ? 6 : 4
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
   const size_t cplane = (size_t)Cl.nx * Cl.ny;
-  const T mb = T(ps[lslot + L_MB]);
-  const T m2 = T(ps[lslot + L_MX2]);
-  // mean of the coarse correction: EpiSmoothLast of the coarser level, folded here when its
-  // partials are passed (fold_src), else stored in ps by its finalize or by the tail
-  double mcd;
-  if(fold_src != nullptr) {
-    double r[2];
-    fold_all<2>(fold_src, fold_nb, r);
-    mcd = zm ? r[0] / (double)Cl.n : 0.0;
-    if(blockIdx.x == 0 && tid == 0) ps[cslot + L_MX5] = mcd;
-  } else {
-    mcd = ps[cslot + L_MX5];
-  }
-  const T mc = T(mcd);
-  const T w = T(L.winv[0]);
-  const T w2 = w * T(L.wr);  // first post-smoothing sweep
-  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
   const FusedTile t = fused_tile<ROWS>(L, kz);
   const int tx = tid & 63, ty = tid >> 6;
   const int gi = t.i0 + tx, gj0 = t.j0 + 2 * ty;
@@ -1165,6 +1150,28 @@ __global__ void __launch_bounds__(32 * ROWS, NBUF == 2 ? (sizeof(T) == 4 ? 6 : 4
     p.tb[0] = b[own_off0 + kz];
     p.tb[1] = b[own_off1 + kz];
   };
+  // the first planes are requested before the means are read and the coarser level's partials
+  // are folded, so those round trips overlap
+  Pre pa, pb, pc;
+  fetch(t.k0 - 2, pa);
+  if constexpr(NBUF == 3) fetch(t.k0 - 1, pb);
+  const T mb = T(ps[lslot + L_MB]);
+  const T m2 = T(ps[lslot + L_MX2]);
+  // mean of the coarse correction: EpiSmoothLast of the coarser level, folded here when its
+  // partials are passed (fold_src), else stored in ps by its finalize or by the tail
+  double mcd;
+  if(fold_src != nullptr) {
+    double r[2];
+    fold_all<2>(fold_src, fold_nb, r);
+    mcd = zm ? r[0] / (double)Cl.n : 0.0;
+    if(blockIdx.x == 0 && tid == 0) ps[cslot + L_MX5] = mcd;
+  } else {
+    mcd = ps[cslot + L_MX5];
+  }
+  const T mc = T(mcd);
+  const T w = T(L.winv[0]);
+  const T w2 = w * T(L.wr);  // first post-smoothing sweep
+  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
   double v[2] = {0.0, 0.0};
   T q1[2] = {T(0), T(0)}, q2[2] = {T(0), T(0)};  // tile rhs of planes kp - 1, kp - 2
   const int last = (t.k1 - t.k0) + 1;
@@ -1232,10 +1239,7 @@ __global__ void __launch_bounds__(32 * ROWS, NBUF == 2 ? (sizeof(T) == 4 ? 6 : 4
     q1[0] = tb0;
     q1[1] = tb1;
   };
-  if constexpr(NBUF == 3) {
-    Pre pa, pb, pc;  // rotate as in k_down_fused
-    fetch(t.k0 - 2, pa);
-    fetch(t.k0 - 1, pb);
+  if constexpr(NBUF == 3) {  // pa, pb, pc rotate as in k_down_fused
     int s = -2;
     for(; s + 4 <= last; s += 3) {  // every step of the group fetches
       step(s, pa, pc, std::true_type{});
@@ -1247,8 +1251,6 @@ __global__ void __launch_bounds__(32 * ROWS, NBUF == 2 ? (sizeof(T) == 4 ? 6 : 4
     if(s + 2 <= last) step(s + 2, pc, pb, std::false_type{});
     if(s + 3 <= last) step(s + 3, pa, pc, std::false_type{});
   } else {
-    Pre pa, pb;
-    fetch(t.k0 - 2, pa);
     int s = -2;
     for(; s + 2 <= last; s += 2) {  // every step of the pair fetches
       step(s, pa, pb, std::true_type{});
@@ -1287,14 +1289,6 @@ __global__ void __launch_bounds__(512) k_down_small(const T* __restrict__ b, T* 
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
   const size_t cplane = (size_t)Cl.nx * Cl.ny;
-  double mbd, sbw;
-  down_fold(fold, ps, lslot, (double)L.n, zm, mbd, sbw);
-  const T mb = T(mbd);
-  const T m1 = zm ? T((sbw - mbd * L.winv_sum) / (double)L.n) : T(0);
-  const T w = T(L.winv[0]);
-  const T w2 = w * T(L.wr);
-  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
-  const T inv_children = T(1.0 / 8.0);
   const FusedTile t = fused_tile<ROWS>(L, 2);
   const int tx = tid & 63, ty = tid >> 6;
   const int gi = t.i0 + tx, gj0 = t.j0 + 2 * ty;
@@ -1324,7 +1318,8 @@ __global__ void __launch_bounds__(512) k_down_small(const T* __restrict__ b, T* 
                 : -1;
   }
   const size_t own_off0 = own0 ? (size_t)gi + (size_t)nx * gj0 : 0, own_off1 = own1 ? own_off0 + nx : 0;
-  // phase A: every load of the six planes, then x1 = (b - mb) w - m1 and b - mb into LDS
+  // phase A: every load of the six planes, issued before the finer level's partials are folded
+  // so the two round trips overlap; then x1 = (b - mb) w - m1 and b - mb into LDS
   T raw[P1][NB], tb[2][2];
 #pragma unroll
   for(int p = 0; p < P1; ++p) {
@@ -1338,6 +1333,14 @@ __global__ void __launch_bounds__(512) k_down_small(const T* __restrict__ b, T* 
     tb[r][0] = b[own_off0 + kz];
     tb[r][1] = b[own_off1 + kz];
   }
+  double mbd, sbw;
+  down_fold(fold, ps, lslot, (double)L.n, zm, mbd, sbw);
+  const T mb = T(mbd);
+  const T m1 = zm ? T((sbw - mbd * L.winv_sum) / (double)L.n) : T(0);
+  const T w = T(L.winv[0]);
+  const T w2 = w * T(L.wr);
+  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+  const T inv_children = T(1.0 / 8.0);
 #pragma unroll
   for(int p = 0; p < P1; ++p) {
 #pragma unroll
@@ -1430,21 +1433,6 @@ __global__ void __launch_bounds__(512) k_up_small(const T* __restrict__ b, const
   const int nx = L.nx, ny = L.ny, nz = L.nz;
   const size_t plane = (size_t)nx * ny;
   const size_t cplane = (size_t)Cl.nx * Cl.ny;
-  const T mb = T(ps[lslot + L_MB]);
-  const T m2 = T(ps[lslot + L_MX2]);
-  double mcd;
-  if(fold_src != nullptr) {
-    double r[2];
-    fold_all<2>(fold_src, fold_nb, r);
-    mcd = zm ? r[0] / (double)Cl.n : 0.0;
-    if(blockIdx.x == 0 && tid == 0) ps[cslot + L_MX5] = mcd;
-  } else {
-    mcd = ps[cslot + L_MX5];
-  }
-  const T mc = T(mcd);
-  const T w = T(L.winv[0]);
-  const T w2 = w * T(L.wr);
-  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
   const FusedTile t = fused_tile<ROWS>(L, 2);
   const int tx = tid & 63, ty = tid >> 6;
   const int gi = t.i0 + tx, gj0 = t.j0 + 2 * ty;
@@ -1470,7 +1458,9 @@ __global__ void __launch_bounds__(512) k_up_small(const T* __restrict__ b, const
     ha[m] = (e % XW + 1) + AW * (e / XW + 1);
   }
   const size_t own_off0 = own0 ? (size_t)gi + (size_t)nx * gj0 : 0, own_off1 = own1 ? own_off0 + nx : 0;
-  // phase A: every load, then x3 = (x2 - m2) + (xc - mc)[parent] and b - mb into LDS
+  // phase A: every load, issued before the means are read and the coarser level's partials are
+  // folded so the round trips overlap, then x3 = (x2 - m2) + (xc - mc)[parent] and b - mb into
+  // LDS
   T ra[P1][NA], rc[P1][NA], rx[P2][NX], tb[2][2];
 #pragma unroll
   for(int p = 0; p < P1; ++p) {
@@ -1494,6 +1484,21 @@ __global__ void __launch_bounds__(512) k_up_small(const T* __restrict__ b, const
     tb[r][0] = b[own_off0 + kz];
     tb[r][1] = b[own_off1 + kz];
   }
+  const T mb = T(ps[lslot + L_MB]);
+  const T m2 = T(ps[lslot + L_MX2]);
+  double mcd;
+  if(fold_src != nullptr) {
+    double r[2];
+    fold_all<2>(fold_src, fold_nb, r);
+    mcd = zm ? r[0] / (double)Cl.n : 0.0;
+    if(blockIdx.x == 0 && tid == 0) ps[cslot + L_MX5] = mcd;
+  } else {
+    mcd = ps[cslot + L_MX5];
+  }
+  const T mc = T(mcd);
+  const T w = T(L.winv[0]);
+  const T w2 = w * T(L.wr);
+  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
 #pragma unroll
   for(int p = 0; p < P1; ++p) {
 #pragma unroll
@@ -2312,40 +2317,25 @@ __global__ void __launch_bounds__(256) k_dir_apply_lds(const T* __restrict__ z, 
   const size_t col = (size_t)t.i + (size_t)L.nx * t.j;
   const RingHalo h(L, t.i - tx, t.j - ty, tx, ty);
   const bool has_h = h.hr >= 0;
-  // centre values of planes k0-1 .. k0+KZ (three ahead of use), ring values one plane ahead
-  T zc[KZ + 2];
-  double dc[KZ + 2];
-  T zh[KZ];
-  double dh[KZ];
-  auto load_c = [&](int m) {
-    const size_t kp = plane * (size_t)wrap_plane(t.k0 - 1 + m, nz);
-    zc[m] = z[col + kp];
-    dc[m] = d_old[col + kp];
-  };
-  auto load_h = [&](int m) {
-    zh[m] = T(0);
-    dh[m] = 0.0;
-    if(has_h) {
-      const size_t kp = plane * (size_t)(t.k0 + m);
-      zh[m] = z[h.off + kp];
-      dh[m] = d_old[h.off + kp];
-    }
-  };
-  load_c(0);
-  load_c(1);
-  load_c(2);
-  load_h(0);
-  double prev = cf.d(zc[0], dc[0]);
-  double cur = cf.d(zc[1], dc[1]);
+  // A rolled march with two planes of look-ahead in named registers: the centre values of
+  // plane k + 3 and the ring values of plane k + 2 are requested while plane k is computed.
+  // Every load is unconditional (h.off is 0 without a ring cell), so no branch join waits for
+  // one; a fully unrolled march let the compiler hoist every plane's loads to the start (94
+  // VGPRs, five waves per SIMD instead of eight).
+  auto at_c = [&](int k) { return col + plane * (size_t)wrap_plane(k, nz); };
+  auto at_h = [&](int k) { return h.off + plane * (size_t)k; };
+  const T z0 = z[at_c(t.k0 - 1)], z1 = z[at_c(t.k0)];
+  const double d0 = d_old[at_c(t.k0 - 1)], d1 = d_old[at_c(t.k0)];
+  T zn1 = z[at_c(t.k0 + 1)], zn2 = z[at_c(t.k0 + 2)], zh1 = z[at_h(t.k0)], zh2 = z[at_h(t.k0 + 1)];
+  double dn1 = d_old[at_c(t.k0 + 1)], dn2 = d_old[at_c(t.k0 + 2)], dh1 = d_old[at_h(t.k0)],
+         dh2 = d_old[at_h(t.k0 + 1)];
+  double prev = cf.d(z0, d0);
+  double cur = cf.d(z1, d1);
   double v[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for(int kk = 0; kk < KZ; ++kk) {
-    if(kk + 3 < KZ + 2) load_c(kk + 3);
-    if(kk + 1 < KZ) load_h(kk + 1);
-    const double next = cf.d(zc[kk + 2], dc[kk + 2]);
+  auto plane_step = [&](int kk, double next, T zhv, double dhv) {
     double(&buf)[6][68] = sd[kk & 1];
     buf[ty + 1][tx + 1] = cur;
-    if(has_h) buf[h.hr][h.hc] = cf.d(zh[kk], dh[kk]);
+    if(has_h) buf[h.hr][h.hc] = cf.d(zhv, dhv);
     __syncthreads();
     const double qv = lap_neg<double>(cur, buf[ty + 1][tx], buf[ty + 1][tx + 2], buf[ty][tx + 1], buf[ty + 2][tx + 1],
                                       prev, next, L.ihx2, L.ihy2, L.ihz2);
@@ -2355,7 +2345,24 @@ __global__ void __launch_bounds__(256) k_dir_apply_lds(const T* __restrict__ z, 
     v[2] += cur * qv;
     prev = cur;
     cur = next;
+  };
+#pragma unroll 1
+  for(int kk = 0; kk < KZ - 2; ++kk) {
+    const int k = t.k0 + kk;
+    const T zl = z[at_c(k + 3)], zhl = z[at_h(k + 2)];
+    const double dl = d_old[at_c(k + 3)], dhl = d_old[at_h(k + 2)];
+    plane_step(kk, cf.d(zn1, dn1), zh1, dh1);
+    zn1 = zn2;
+    dn1 = dn2;
+    zn2 = zl;
+    dn2 = dl;
+    zh1 = zh2;
+    dh1 = dh2;
+    zh2 = zhl;
+    dh2 = dhl;
   }
+  plane_step(KZ - 2, cf.d(zn1, dn1), zh1, dh1);
+  plane_step(KZ - 1, cf.d(zn2, dn2), zh2, dh2);
   const int ops[3] = {kSum, kSum, kSum};
   store_partials<3>(v, ops, partials);
 }
