@@ -162,17 +162,35 @@ __device__ inline void store_partials(double (&v)[NV], const int (&ops)[NV], dou
 }
 
 // Combines nb partials per value in a fixed order (one block); result valid in thread 0.
-template <int NV>
+// Thread t folds partials t, t + blockDim.x, ... in that order. They are read BATCH at a time
+// by unconditional loads at clamped indices and the out-of-range ones skipped when folded, so a
+// batch costs one memory round trip (a load under the loop's bound test was waited for before
+// the next iteration's loads were issued: eight round trips for 8192 partials). NT > 0 is the
+// block's thread count (blockDim.x otherwise, which is read from the dispatch packet: one more
+// round trip before the first partial can be requested).
+template <int NV, int BATCH = (NV <= 4 ? 8 : 4), int NT = 0>
 __device__ inline void fold_partials(const double* __restrict__ partials, unsigned nb, const int (&ops)[NV],
                                      double (&out)[NV]) {
 #pragma unroll
   for(int k = 0; k < NV; ++k) out[k] = ops[k] == kSum ? 0.0 : -INFINITY;
-#pragma unroll 4
-  for(unsigned b = threadIdx.x; b < nb; b += blockDim.x) {
+  const unsigned nt = NT > 0 ? (unsigned)NT : blockDim.x;
+  const unsigned last = nb > 0 ? nb - 1 : 0;
+  for(unsigned base = threadIdx.x; base < nb; base += BATCH * nt) {
+    double x[BATCH][NV];
 #pragma unroll
-    for(int k = 0; k < NV; ++k) {
-      const double x = partials[(size_t)k * kMaxPartials + b];
-      out[k] = ops[k] == kSum ? out[k] + x : fmax(out[k], x);
+    for(int u = 0; u < BATCH; ++u) {
+      const unsigned b = min(base + u * nt, last);
+#pragma unroll
+      for(int k = 0; k < NV; ++k) x[u][k] = partials[(size_t)k * kMaxPartials + b];
+    }
+#pragma unroll
+    for(int u = 0; u < BATCH; ++u) {
+      const bool in = base + u * nt < nb;
+#pragma unroll
+      for(int k = 0; k < NV; ++k) {
+        const double f = ops[k] == kSum ? out[k] + x[u][k] : fmax(out[k], x[u][k]);
+        out[k] = in ? f : out[k];
+      }
     }
   }
   block_reduce<NV>(out, ops);
@@ -187,7 +205,7 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_finalize(const double* __r
 #pragma unroll
   for(int k = 0; k < Epi::NV; ++k) ops[k] = epi.op(k);
   double res[Epi::NV];
-  fold_partials<Epi::NV>(partials, nb, ops, res);
+  fold_partials<Epi::NV, (Epi::NV <= 4 ? 8 : 4), kFinalizeThreads>(partials, nb, ops, res);
   if(threadIdx.x == 0) epi(res);
 }
 

@@ -823,13 +823,15 @@ __global__ void __launch_bounds__(256) k_smooth_last(const T* __restrict__ b, co
 // combines the partials in the same fixed order, so every block sees the same values, and
 // block 0 stores the epilogue's scalars for the later kernels. The producer and the consumer
 // use different partial buffers.
-template <int N>
+// NT: the calling block's thread count. Producers folded here have at most a few blocks per
+// thread, so two partials per value are requested at a time.
+template <int N, int NT>
 __device__ inline void fold_all(const double* __restrict__ src, unsigned nb, double (&out)[N]) {
   __shared__ double fsh[N];
   int ops[N];
 #pragma unroll
   for(int k = 0; k < N; ++k) ops[k] = kSum;
-  fold_partials<N>(src, nb, ops, out);
+  fold_partials<N, 2, NT>(src, nb, ops, out);
   if(threadIdx.x == 0 && threadIdx.y == 0) {
 #pragma unroll
     for(int k = 0; k < N; ++k) fsh[k] = out[k];
@@ -847,6 +849,7 @@ struct DownFold {
   int pslot;  // slot of the finer level
   double pn;  // cells of the finer level
 };
+template <int NT>
 __device__ inline void down_fold(const DownFold& f, double* ps, int lslot, double n, int zm, double& mb,
                                  double& sbw) {
   if(f.src == nullptr) {
@@ -855,7 +858,7 @@ __device__ inline void down_fold(const DownFold& f, double* ps, int lslot, doubl
     return;
   }
   double r[3];
-  fold_all<3>(f.src, f.nb, r);
+  fold_all<3, NT>(f.src, f.nb, r);
   mb = zm ? r[1] / n : 0.0;
   sbw = r[2];
   if(blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
@@ -978,7 +981,7 @@ __global__ void __launch_bounds__(32 * ROWS) k_down_fused(const T* __restrict__ 
   fetch(t.k0 - 2, pa, ta);
   fetch(t.k0 - 1, pb, tb);
   double mbd, sbw;
-  down_fold(fold, ps, lslot, (double)L.n, zm, mbd, sbw);
+  down_fold<kThreads>(fold, ps, lslot, (double)L.n, zm, mbd, sbw);
   const T mb = T(mbd);
   const T m1 = zm ? T((sbw - mbd * L.winv_sum) / (double)L.n) : T(0);
   const T w = T(L.winv[0]);
@@ -1162,7 +1165,7 @@ __global__ void __launch_bounds__(32 * ROWS, NBUF == 2 ? (sizeof(T) == 4 ? 6 : 4
   double mcd;
   if(fold_src != nullptr) {
     double r[2];
-    fold_all<2>(fold_src, fold_nb, r);
+    fold_all<2, kThreads>(fold_src, fold_nb, r);
     mcd = zm ? r[0] / (double)Cl.n : 0.0;
     if(blockIdx.x == 0 && tid == 0) ps[cslot + L_MX5] = mcd;
   } else {
@@ -1334,7 +1337,7 @@ __global__ void __launch_bounds__(512) k_down_small(const T* __restrict__ b, T* 
     tb[r][1] = b[own_off1 + kz];
   }
   double mbd, sbw;
-  down_fold(fold, ps, lslot, (double)L.n, zm, mbd, sbw);
+  down_fold<kThreads>(fold, ps, lslot, (double)L.n, zm, mbd, sbw);
   const T mb = T(mbd);
   const T m1 = zm ? T((sbw - mbd * L.winv_sum) / (double)L.n) : T(0);
   const T w = T(L.winv[0]);
@@ -1489,7 +1492,7 @@ __global__ void __launch_bounds__(512) k_up_small(const T* __restrict__ b, const
   double mcd;
   if(fold_src != nullptr) {
     double r[2];
-    fold_all<2>(fold_src, fold_nb, r);
+    fold_all<2, kThreads>(fold_src, fold_nb, r);
     mcd = zm ? r[0] / (double)Cl.n : 0.0;
     if(blockIdx.x == 0 && tid == 0) ps[cslot + L_MX5] = mcd;
   } else {
@@ -1672,7 +1675,7 @@ __global__ void __launch_bounds__(kTailThreads)
   __shared__ double rz_sh;
   const int s0 = P_LEVEL + 8 * l0;
   double mb0, sbw0;
-  down_fold(fold, ps, s0, (double)n0, zm, mb0, sbw0);
+  down_fold<kTailThreads>(fold, ps, s0, (double)n0, zm, mb0, sbw0);
   if(LDS) {
     T* dst = Bp(l0);
 #pragma unroll

@@ -63,8 +63,8 @@ constexpr unsigned kStepDiagMaxMask = 0x7eu;  // values 1..6 are maxima
 // The end of a step on a grid periodic on every axis, in one pass over the cells: the corrected
 // velocity on every active face (each cell's lower faces, and the upper face at the end of a
 // line), the total-pressure update and the step diagnostics, each value by the expression of
-// k_correct, k_pressure_update and k_step_diag, with k_step_diag's launch shape and per-thread
-// order (so its partial sums are the same). On such a grid the two copies of a duplicated face
+// k_correct, k_pressure_update and k_step_diag (the maxima are those of k_step_diag; the sums
+// differ by rounding, columns being longer here). On such a grid the two copies of a duplicated face
 // are bitwise equal before the ghost fill (pred carries its filled ghosts, phi its periodic
 // copies), so the fill that follows leaves every value read here unchanged. 13 array passes
 // against 19 for the three kernels.
@@ -425,7 +425,9 @@ void Stepper::step() {
   }
   launch_pressure_bc(c, g, pbcs, phi);
   if(all_periodic) {  // correction, pressure update and diagnostics in one pass, then the fills
-    const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz);
+    // columns of at least 16 planes: a quarter of the z re-reads of phi and w at block edges and
+    // of the partials the finalize folds (the diagnostics' sums differ by rounding only)
+    const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz, 16);
     {
       ProfScope prof_(c, "k_finish_periodic", true);
       OFH_LAUNCH(k_finish_periodic, cl.grid, cl.block, 0, c.stream, pred[0], pred[1], pred[2], phi, prhs, vel[0],
