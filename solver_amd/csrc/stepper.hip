@@ -80,26 +80,45 @@ __global__ void k_finish_periodic(const double* __restrict__ pu, const double* _
   double vals[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if(i <= g.nx && j <= g.ny) {
     const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + g.nz);
+    // per-thread column offsets plus uniform plane strides (no 64-bit index products per point)
+    const size_t sc = (size_t)Lc.sx * Lc.sy, su = (size_t)Lu.sx * Lu.sy, sv = (size_t)Lv.sx * Lv.sy,
+                 sw = (size_t)Lw.sx * Lw.sy;
+    const size_t oc = Lc.at(i, j, 0), ou = Lu.at(i, j, 0), ov = Lv.at(i, j, 0), ow = Lw.at(i, j, 0);
+    const size_t cy = (size_t)Lc.sx, vy = (size_t)Lv.sx;
+    const bool xe = i == g.nx, ye = j == g.ny;
+    // phi of plane k and the lower w face of plane k carried along the column: wb of plane k is
+    // wf of plane k - 1 (the same operands, so the same value)
+    const double phm = phi[oc + sc * (size_t)(kb - 1)];
+    double ph0 = phi[oc + sc * (size_t)kb];
+    double wb = pw[ow + sw * (size_t)kb] + scale * ((ph0 - phm) / g.dz);
+    double wob = wo[ow + sw * (size_t)kb];
     for(int k = kb; k < ke; ++k) {
-      const size_t at = Lc.at(i, j, k);
-      const double pc = phi[at];
+      const size_t c0 = oc + sc * (size_t)k, u0 = ou + su * (size_t)k, v0 = ov + sv * (size_t)k;
+      const size_t w1 = ow + sw * (size_t)(k + 1);
+      // every load is unconditional (the upper-face values of the line ends are read by every
+      // thread and used where they apply), so no branch join waits for a load
+      const double php = phi[c0 + sc], pxm = phi[c0 - 1], pxp = phi[c0 + 1], pym = phi[c0 - cy],
+                   pyp = phi[c0 + cy];
+      const double pu0 = pu[u0], pu1 = pu[u0 + 1], pv0 = pv[v0], pv1 = pv[v0 + vy], pw1 = pw[w1];
+      const double uo0 = uo[u0], uo1 = uo[u0 + 1], vo0 = vo[v0], vo1 = vo[v0 + vy], wo1 = wo[w1];
+      const double ptc = pt[c0], rc = prhs[c0];
+      const double pc = ph0;
       // k_correct: out = pred + scale * (phi(i) - phi(i - 1)) / h
-      const double uw = pu[Lu.at(i, j, k)] + scale * ((pc - phi[Lc.at(i - 1, j, k)]) / g.dx);
-      const double ue = pu[Lu.at(i + 1, j, k)] + scale * ((phi[Lc.at(i + 1, j, k)] - pc) / g.dx);
-      const double vs = pv[Lv.at(i, j, k)] + scale * ((pc - phi[Lc.at(i, j - 1, k)]) / g.dy);
-      const double vn = pv[Lv.at(i, j + 1, k)] + scale * ((phi[Lc.at(i, j + 1, k)] - pc) / g.dy);
-      const double wb = pw[Lw.at(i, j, k)] + scale * ((pc - phi[Lc.at(i, j, k - 1)]) / g.dz);
-      const double wf = pw[Lw.at(i, j, k + 1)] + scale * ((phi[Lc.at(i, j, k + 1)] - pc) / g.dz);
-      u[Lu.at(i, j, k)] = uw;
-      v[Lv.at(i, j, k)] = vs;
-      w[Lw.at(i, j, k)] = wb;
-      if(i == g.nx) u[Lu.at(i + 1, j, k)] = ue;
-      if(j == g.ny) v[Lv.at(i, j + 1, k)] = vn;
-      if(k == g.nz) w[Lw.at(i, j, k + 1)] = wf;
+      const double uw = pu0 + scale * ((pc - pxm) / g.dx);
+      const double ue = pu1 + scale * ((pxp - pc) / g.dx);
+      const double vs = pv0 + scale * ((pc - pym) / g.dy);
+      const double vn = pv1 + scale * ((pyp - pc) / g.dy);
+      const double wf = pw1 + scale * ((php - pc) / g.dz);
+      u[u0] = uw;
+      v[v0] = vs;
+      w[w1 - sw] = wb;
+      if(xe) u[u0 + 1] = ue;
+      if(ye) v[v0 + vy] = vn;
+      if(k == g.nz) w[w1] = wf;
       // k_pressure_update
-      double p = pt[at] + 1.0 * pc;
-      p = p + sp * prhs[at];
-      pt[at] = p;
+      double p = ptc + 1.0 * pc;
+      p = p + sp * rc;
+      pt[c0] = p;
       // k_step_diag
       const double a = (ue - uw) / g.dx;
       const double b = (vn - vs) / g.dy;
@@ -112,14 +131,17 @@ __global__ void k_finish_periodic(const double* __restrict__ pu, const double* _
       vals[4] = fmax(vals[4], fabs(vc));
       vals[5] = fmax(vals[5], fabs(wc));
       vals[2] = fmax(vals[2], dt * (fabs(uc) / g.dx + fabs(vc) / g.dy + fabs(wc) / g.dz));
-      double ch = fabs(uw - uo[Lu.at(i, j, k)]);
-      ch = fmax(ch, fabs(vs - vo[Lv.at(i, j, k)]));
-      ch = fmax(ch, fabs(wb - wo[Lw.at(i, j, k)]));
-      if(i == g.nx) ch = fmax(ch, fabs(ue - uo[Lu.at(i + 1, j, k)]));
-      if(j == g.ny) ch = fmax(ch, fabs(vn - vo[Lv.at(i, j + 1, k)]));
-      if(k == g.nz) ch = fmax(ch, fabs(wf - wo[Lw.at(i, j, k + 1)]));
+      double ch = fabs(uw - uo0);
+      ch = fmax(ch, fabs(vs - vo0));
+      ch = fmax(ch, fabs(wb - wob));
+      if(xe) ch = fmax(ch, fabs(ue - uo1));
+      if(ye) ch = fmax(ch, fabs(vn - vo1));
+      if(k == g.nz) ch = fmax(ch, fabs(wf - wo1));
       vals[6] = fmax(vals[6], ch);
       if(!isfinite(dv) || !isfinite(ch) || !isfinite(uc) || !isfinite(vc) || !isfinite(wc)) vals[7] += 1.0;
+      ph0 = php;
+      wb = wf;
+      wob = wo1;
     }
   }
   const int ops[8] = {kSum, kMax, kMax, kMax, kMax, kMax, kMax, kSum};
