@@ -2576,6 +2576,18 @@ FusedLaunch fused_launch(const PLevel& L, int resident, int rows) {
   return f;
 }
 
+// The one-shot small-level kernels (k_down_small, k_up_small) own 16-row tiles of two planes
+// whatever row count the marching kernels use: their grid is every such tile of the level.
+FusedLaunch small_launch(const PLevel& L) {
+  FusedLaunch f;
+  f.kz = 2;
+  const size_t tiles = (size_t)((L.nx + kFX - 1) / kFX) * (size_t)((L.ny + 15) / 16);
+  const size_t n = tiles * (size_t)((L.nz + 1) / 2);
+  if(n > (size_t)kMaxPartials) throw InvalidArgument("grid too large for the reduction buffer");
+  f.grid = dim3((unsigned)n);
+  return f;
+}
+
 // Blocks of `kernel` (launched with `threads` threads) that the device holds at once.
 template <typename K>
 int resident_blocks(K kernel, int threads, int device) {
@@ -3079,11 +3091,13 @@ void PoissonSolver::vcycle() {
     const PLevel& P = lv[l];
     const PLevel& C = lv[l + 1];
     if(fused[l]) {
-      const FusedLaunch f = fused_launch(P, resident_down, rows_down);
+      FusedLaunch f = fused_launch(P, resident_down, rows_down);
+      const bool small = f.kz == 2 && small_levels();
+      if(small) f = small_launch(P);  // its own tiles, whatever rows_down is
       double* out = pbuf[which];
       {
         ProfScope prof_(c, "k_down_fused", l == 0);
-        if(f.kz == 2 && small_levels())
+        if(small)
           OFH_LAUNCH((k_down_small<T>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<T*>(lxa[l]), P, C,
                      static_cast<T*>(lb[l + 1]), out, ps, slot(l), zmi, pend);
         else if(rows_down == 16)
@@ -3157,11 +3171,13 @@ void PoissonSolver::vcycle() {
     const PLevel& C = lv[l + 1];
     const T* xc = static_cast<const T*>(x_final(l + 1));
     if(fused[l]) {
-      const FusedLaunch f = fused_launch(P, resident_up, rows_up);
+      FusedLaunch f = fused_launch(P, resident_up, rows_up);
+      const bool small = f.kz == 2 && small_levels();
+      if(small) f = small_launch(P);
       double* out = pbuf[which];
       {
         ProfScope prof_(c, "k_up_fused", l == 0);
-        if(f.kz == 2 && small_levels())
+        if(small)
           OFH_LAUNCH((k_up_small<T>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<const T*>(lxa[l]), xc, P,
                      C, static_cast<T*>(lxb[l]), out, ps, slot(l), slot(l + 1), up_src, up_nb, zmi);
         else if(rows_up == 16 && up_bufs == 2)

@@ -55,6 +55,46 @@ def test_fused_solve_matches_unfused(ctx, precision, shape):
     assert rel_l2_active(a, b, mean_removed=True) <= 10 * tol
 
 
+def solve_env(ctx, g, rhs, precision, tol, env):
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        o = sa.ProjectionOptions(1.0, 1.0, 400, tol)
+        pb = sa.pressure_bc_set([sa.P_PERIODIC] * 6)
+        sol = zeros(g, sa.CELL)
+        d = sa.PoissonDiagnostics()
+        ctx.run("of_solve_pressure_poisson", C.byref(g), C.byref(pb), C.byref(o), precision, sa.ptr(dev(rhs)),
+                sa.ptr(sol), C.byref(d), None, 0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return host(sol), d
+
+
+# The marching level kernels come in 8- and 16-row tiles and the up kernel with two or three
+# prefetch buffers; the one-shot small-level kernels keep their own 16-row tiles whatever the
+# marching kernels use (their grid once followed the marching kernels' row count). Every variant
+# takes the same iterations as the default and agrees with it to rounding.
+@pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 128, 128)])
+@pytest.mark.parametrize("env", [{"SOLVER_AMD_FUSED_ROWS": "8,8,8,8"}, {"SOLVER_AMD_FUSED_ROWS": "16,8,16,8"},
+                                 {"SOLVER_AMD_UP_BUFFERS": "3,2"}])
+def test_fused_launch_variants_match_default(ctx, precision, shape, env):
+    nx, ny, nz = shape
+    g = sa.grid(nx, ny, nz, 2 * math.pi / nx, 2 * math.pi / ny, 2 * math.pi / nz)
+    rng = np.random.default_rng(nx * 7 + ny + nz)
+    rhs = rng.standard_normal(sa.storage_shape(nx, ny, nz, sa.CELL))
+    tol = 1e-12 if precision == sa.FP64 else 1e-10
+    a, da = solve_env(ctx, g, rhs, precision, tol, {})
+    b, db = solve_env(ctx, g, rhs, precision, tol, env)
+    assert da.converged == db.converged == 1
+    assert abs(da.iterations - db.iterations) <= (0 if precision == sa.FP64 else 1), (da.iterations, db.iterations)
+    assert rel_l2_active(a, b, mean_removed=True) <= 10 * tol
+
+
 @pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
 def test_fused_stepper_matches_unfused(ctx, precision):
     n, steps = 48, 4
