@@ -1266,6 +1266,190 @@ __global__ void __launch_bounds__(32 * ROWS, NBUF == 2 ? (sizeof(T) == 4 ? 6 : 4
   store_partials<2>(v, ops, partials);
 }
 
+// ---------------------------------------------------------------- up leg on per-point z windows
+// The up leg of a fused level (prolongation and the two post-smoothing sweeps) with all three
+// stages on one thread-to-point map: thread t owns the points e = t, t + 512, t + 1024 of the
+// tile's halo-2 plane (68 x 20) and keeps in registers, per point, x3 of planes k-2..k, x4 of
+// planes k-3..k-1 and b - mb of planes k-2 and k-1. A stage then takes its z neighbours and its
+// right-hand side from its own registers and reads only the four in-plane neighbours from LDS,
+// from the plane of x3 or x4 stored one step earlier (two planes each, by parity), so a plane
+// needs one barrier and 4 LDS reads per stencil (k_up_fused: two barriers, three-plane rings and
+// 8 reads). Each value is the same per-point expression as in k_up_fused; only the order in which
+// a block accumulates its partial sums differs.
+template <typename T, int NBUF>
+__global__ void __launch_bounds__(512, 4) k_up_zw(const T* __restrict__ b, const T* __restrict__ x2,
+                                               const T* __restrict__ xc, PLevel L, PLevel Cl, T* __restrict__ x5g,
+                                               double* partials, double* ps, int lslot, int cslot, int kz,
+                                               const double* fold_src, unsigned fold_nb, int zm) {
+  if(stopped(ps)) return;
+  constexpr int ROWS = 16, kThreads = 32 * ROWS;
+  constexpr int AW = kFX + 4, AH = ROWS + 4, AP = AW * AH;  // halo-2 plane
+  constexpr int NA = (AP + kThreads - 1) / kThreads;
+  __shared__ T s3[2][AP];
+  __shared__ T s4[2][AP];
+  const int tid = threadIdx.x;
+  const int nx = L.nx, ny = L.ny, nz = L.nz;
+  const size_t plane = (size_t)nx * ny;
+  const size_t cplane = (size_t)Cl.nx * Cl.ny;
+  const FusedTile t = fused_tile<ROWS>(L, kz);
+  // per point: in-plane offset (and of its coarse parent), whether x3 is taken from the level
+  // (oka; else 0), whether x4 is formed (h1: halo 1) and b enters (okx), whether x5 is stored
+  unsigned offa[NA], offc[NA];
+  bool oka[NA], okx[NA], h1[NA], own[NA];
+#pragma unroll
+  for(int m = 0; m < NA; ++m) {
+    const int e = tid + m * kThreads;
+    const int ex = e % AW, ey = e / AW;
+    const int gx = t.i0 + ex - 2, gy = t.j0 + ey - 2;
+    const bool in = e < AP;
+    oka[m] = in && gx < nx + 2 && gy < ny + 2;
+    h1[m] = in && ex >= 1 && ex <= AW - 2 && ey >= 1 && ey <= AH - 2;
+    okx[m] = h1[m] && gx < nx + 1 && gy < ny + 1;
+    own[m] = in && ex >= 2 && ex < AW - 2 && ey >= 2 && ey < AH - 2 && gx < nx && gy < ny;
+    const int ii = oka[m] ? wrap_index(gx, nx) : 0, jj = oka[m] ? wrap_index(gy, ny) : 0;
+    offa[m] = (unsigned)ii + (unsigned)nx * (unsigned)jj;
+    offc[m] = (unsigned)(ii >> 1) + (unsigned)Cl.nx * (unsigned)(jj >> 1);
+  }
+  struct Pre {
+    T a[NA];  // x2 of plane kp
+    T c[NA];  // the coarse correction at the parent, plane kp
+    T x[NA];  // b of plane kp - 1 (the plane of the step's x4)
+  };
+  auto fetch = [&](int kp, Pre& p) {  // unconditional loads at in-bounds offsets
+    const int kw = wrap_index(kp, nz);
+    const size_t kzo = plane * (size_t)kw, cz = cplane * (size_t)(kw >> 1);
+    const size_t kbo = plane * (size_t)wrap_index(kp - 1, nz);
+#pragma unroll
+    for(int m = 0; m < NA; ++m) {
+      p.a[m] = x2[offa[m] + kzo];
+      p.c[m] = xc[offc[m] + cz];
+      p.x[m] = b[offa[m] + kbo];
+    }
+  };
+  // the first planes are requested before the means are read and the coarser level's partials
+  // are folded, so those round trips overlap
+  Pre pa, pb, pc;
+  fetch(t.k0 - 2, pa);
+  if constexpr(NBUF == 3) fetch(t.k0 - 1, pb);
+  const T mb = T(ps[lslot + L_MB]);
+  const T m2 = T(ps[lslot + L_MX2]);
+  double mcd;
+  if(fold_src != nullptr) {
+    double r[2];
+    fold_all<2, kThreads>(fold_src, fold_nb, r);
+    mcd = zm ? r[0] / (double)Cl.n : 0.0;
+    if(blockIdx.x == 0 && tid == 0) ps[cslot + L_MX5] = mcd;
+  } else {
+    mcd = ps[cslot + L_MX5];
+  }
+  const T mc = T(mcd);
+  const T w = T(L.winv[0]);
+  const T w2 = w * T(L.wr);  // first post-smoothing sweep
+  const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+  T w3[NA][3], w4[NA][3], bold[NA];  // x3 and x4 of three planes (oldest first), b - mb of kp - 2
+#pragma unroll
+  for(int m = 0; m < NA; ++m) {
+    bold[m] = T(0);
+#pragma unroll
+    for(int q = 0; q < 3; ++q) w3[m][q] = w4[m][q] = T(0);
+  }
+  double v[2] = {0.0, 0.0};
+  const int last = (t.k1 - t.k0) + 1;
+  constexpr int D = NBUF - 1;  // planes between a fetch and its use
+  // ALWAYS: the step fetches (not under a branch); STEADY: s >= 2, so all three stages run with
+  // no test, in one basic block the scheduler can interleave
+  auto step = [&](int s, Pre& cur, Pre& nxt, auto always, auto steady) {
+    const int kp = t.k0 + s;
+    // x3 on plane kp at every point of the halo-2 plane; b - mb of plane kp - 1
+    T* d3 = s3[kp & 1];
+    T bnew[NA];
+#pragma unroll
+    for(int m = 0; m < NA; ++m) {
+      const T x3 = oka[m] ? (cur.a[m] - m2) + (cur.c[m] - mc) : T(0);
+      if(tid + m * kThreads < AP) d3[tid + m * kThreads] = x3;
+      w3[m][0] = w3[m][1];
+      w3[m][1] = w3[m][2];
+      w3[m][2] = x3;
+      bnew[m] = okx[m] ? cur.x[m] - mb : T(0);
+    }
+    if(decltype(always)::value || s + D <= last) fetch(kp + D, nxt);
+    if(decltype(steady)::value || s >= 0) {  // x4 on plane kq = kp - 1 (halo 1): x3 stored last step
+      const int kq = kp - 1;
+      const T* a3 = s3[kq & 1];
+      T* d4 = s4[kq & 1];
+#pragma unroll
+      for(int m = 0; m < NA; ++m) {
+        T x4 = T(0);
+        if(h1[m]) {
+          const int e = tid + m * kThreads;
+          const T c = w3[m][1];
+          const T ax = lap_neg<T>(c, a3[e - 1], a3[e + 1], a3[e - AW], a3[e + AW], w3[m][0], w3[m][2], ihx2, ihy2,
+                                  ihz2);
+          x4 = c + w2 * (bnew[m] + T(-1) * ax);
+          d4[e] = x4;
+        }
+        w4[m][0] = w4[m][1];
+        w4[m][1] = w4[m][2];
+        w4[m][2] = x4;
+      }
+    }
+    if(decltype(steady)::value || s >= 2) {  // x5 on plane kr = kp - 2 (tile): x4 stored last step
+      const int kr = kp - 2;
+      const T* a4 = s4[kr & 1];
+#pragma unroll
+      for(int m = 0; m < NA; ++m) {
+        if(own[m]) {
+          const int e = tid + m * kThreads;
+          const T c = w4[m][1];
+          const T ax = lap_neg<T>(c, a4[e - 1], a4[e + 1], a4[e - AW], a4[e + AW], w4[m][0], w4[m][2], ihx2, ihy2,
+                                  ihz2);
+          const T bt = bold[m];
+          const T out = c + w * (bt + T(-1) * ax);
+          x5g[offa[m] + plane * (size_t)kr] = out;
+          v[0] += (double)out;
+          v[1] += (double)bt * (double)out;
+        }
+      }
+    }
+#pragma unroll
+    for(int m = 0; m < NA; ++m) bold[m] = bnew[m];
+    __syncthreads();
+  };
+  // the four ramp steps (s = -2 .. 1) fetch (last >= 3), then the steady steps
+  const std::true_type yes{};
+  const std::false_type no{};
+  if constexpr(NBUF == 3) {  // pa, pb, pc rotate: a step commits one and refills another
+    step(-2, pa, pc, yes, no);
+    step(-1, pb, pa, yes, no);
+    step(0, pc, pb, yes, no);
+    step(1, pa, pc, yes, no);
+    int s = 2;
+    for(; s + 4 <= last; s += 3) {
+      step(s, pb, pa, yes, yes);
+      step(s + 1, pc, pb, yes, yes);
+      step(s + 2, pa, pc, yes, yes);
+    }
+    if(s <= last) step(s, pb, pa, no, yes);
+    if(s + 1 <= last) step(s + 1, pc, pb, no, yes);
+    if(s + 2 <= last) step(s + 2, pa, pc, no, yes);
+    if(s + 3 <= last) step(s + 3, pb, pa, no, yes);
+  } else {
+    step(-2, pa, pb, yes, no);
+    step(-1, pb, pa, yes, no);
+    step(0, pa, pb, yes, no);
+    step(1, pb, pa, yes, no);
+    int s = 2;
+    for(; s + 2 <= last; s += 2) {
+      step(s, pa, pb, yes, yes);
+      step(s + 1, pb, pa, yes, yes);
+    }
+    if(s <= last) step(s, pa, pb, no, yes);
+    if(s + 1 <= last) step(s + 1, pb, pa, no, yes);
+  }
+  const int ops[2] = {kSum, kSum};
+  store_partials<2>(v, ops, partials);
+}
+
 // ---------------------------------------------------------------- small fused levels in one shot
 // On the small all-periodic levels (64^3 and 32^3 at 256^3) each block of the fused kernels
 // owns two planes and its march is six dependent plane steps (four of them halo planes) with
@@ -2641,6 +2825,7 @@ class PoissonSolver {
   int resident_down = 0, resident_up = 0;  // blocks of the fused kernels the device holds at once
   int rows_down = 8, rows_up = 8;          // tile rows of the fused kernels (8: 256 threads, 16: 512)
   int up_bufs = 3;                         // prefetch buffers of the 16-row fused up kernel (2 or 3)
+  bool up_zw = true;                       // 16-row up leg on per-point z windows (k_up_zw)
   double* partials2 = nullptr;  // second partial buffer: a folding consumer writes beside its input
   // coarse solve data
   int nc = 0;
@@ -2764,6 +2949,13 @@ int up_buffers(int precision) {
       throw InvalidArgument("SOLVER_AMD_UP_BUFFERS must hold two buffer counts (2 or 3)");
   }
   return precision == OF_PRECISION_MIXED ? r[1] : r[0];
+}
+
+// SOLVER_AMD_UP_ZW=0 keeps the two-barrier 16-row up kernel (k_up_fused) instead of the one on
+// per-point z windows (k_up_zw); read when a solver is built.
+bool up_windows() {
+  const char* e = std::getenv("SOLVER_AMD_UP_ZW");
+  return e == nullptr || e[0] != '0';
 }
 
 // Tile rows of the fused level kernels per precision, overridden by SOLVER_AMD_FUSED_ROWS =
@@ -3012,6 +3204,7 @@ void PoissonSolver::build() {
   }
   fused_rows(precision, rows_down, rows_up);
   up_bufs = up_buffers(precision);
+  up_zw = up_windows() && rows_up == 16;
   if(precision == OF_PRECISION_MIXED) {
     resident_down = rows_down == 16 ? resident_blocks(k_down_fused<float, 16>, 512, c.device)
                                     : resident_blocks(k_down_fused<float, 8>, 256, c.device);
@@ -3024,6 +3217,14 @@ void PoissonSolver::build() {
     resident_up = rows_up == 16 ? (up_bufs == 2 ? resident_blocks(k_up_fused<double, 16, 2>, 512, c.device)
                                                 : resident_blocks(k_up_fused<double, 16>, 512, c.device))
                                 : resident_blocks(k_up_fused<double, 8>, 256, c.device);
+  }
+  if(up_zw) {
+    if(precision == OF_PRECISION_MIXED)
+      resident_up = up_bufs == 2 ? resident_blocks(k_up_zw<float, 2>, 512, c.device)
+                                 : resident_blocks(k_up_zw<float, 3>, 512, c.device);
+    else
+      resident_up = up_bufs == 2 ? resident_blocks(k_up_zw<double, 2>, 512, c.device)
+                                 : resident_blocks(k_up_zw<double, 3>, 512, c.device);
   }
   const size_t n0 = (size_t)lv[0].n;
   dev_alloc(reinterpret_cast<void**>(&pc), n0 * sizeof(double));
@@ -3180,6 +3381,12 @@ void PoissonSolver::vcycle() {
         if(small)
           OFH_LAUNCH((k_up_small<T>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<const T*>(lxa[l]), xc, P,
                      C, static_cast<T*>(lxb[l]), out, ps, slot(l), slot(l + 1), up_src, up_nb, zmi);
+        else if(up_zw && up_bufs == 2)
+          OFH_LAUNCH((k_up_zw<T, 2>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<const T*>(lxa[l]), xc, P,
+                     C, static_cast<T*>(lxb[l]), out, ps, slot(l), slot(l + 1), f.kz, up_src, up_nb, zmi);
+        else if(up_zw)
+          OFH_LAUNCH((k_up_zw<T, 3>), f.grid, dim3(512), 0, c.stream, bptr(l), static_cast<const T*>(lxa[l]), xc, P,
+                     C, static_cast<T*>(lxb[l]), out, ps, slot(l), slot(l + 1), f.kz, up_src, up_nb, zmi);
         else if(rows_up == 16 && up_bufs == 2)
           OFH_LAUNCH((k_up_fused<T, 16, 2>), f.grid, dim3(512), 0, c.stream, bptr(l),
                              static_cast<const T*>(lxa[l]), xc, P, C, static_cast<T*>(lxb[l]), out, ps, slot(l),
