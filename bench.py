@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import math
 import os
 import platform
@@ -230,8 +231,9 @@ def step_kernel_bytes(n, iters, mixed):
 
 def measured_traffic(kernel, mixed):
     """HBM bytes per finest-grid launch of `kernel` from the committed rocprofv3 counter summary
-    (rows are named with template arguments, e.g. k_dir_apply<float, 8, true>; the first row of
-    a kernel is its finest-grid launch)."""
+    (rows are named with template arguments, e.g. k_dir_apply_lds<float>; the variants of one
+    operation (_lds, _zw) answer for its name; the first row of a kernel is its finest-grid
+    launch)."""
     try:
         text = open(TRAFFIC_TABLE).read()
     except OSError:
@@ -241,7 +243,7 @@ def measured_traffic(kernel, mixed):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cells) < 6:
             continue
-        name = cells[0]
+        name = re.sub(r"_(lds|zw)(?=<|$| )", "", cells[0])
         if name == kernel or name.startswith(f"{kernel}<{elem}") or (name.startswith(f"{kernel}<") and "," not in name
                                                                        and elem in name):
             try:
