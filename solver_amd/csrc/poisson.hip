@@ -1785,25 +1785,73 @@ struct TailBufs {
   int oxa[kMaxLevels];
   int oxb[kMaxLevels];
   int owork;  // LDS offset of the coarse work vector (doubles)
+  int n0;     // cells of the first tail level
+  int ncs;    // unknowns of the coarsest level
+  const T* b0;  // b of the first tail level (global)
 };
 constexpr int kTailLdsBytes = 152 * 1024;
 constexpr int kCoarseStage = 16;  // coarsest systems of up to this many unknowns are read from LDS
 constexpr int kTailPre = 4;       // level-l0 values per thread read ahead of the fold (4096 cells)
 
-// Thread groups of the tail. Levels of more than kWaveCells cells run on the whole workgroup
-// (workgroup barriers, block-wide sums); the smaller ones and the coarsest solve run on its
+// Sum of a double over a wave, the same value in every lane: butterfly exchanges within each row
+// of 16 lanes by DPP moves (lane ^ 1, lane ^ 2, half-row mirror, row mirror; a row then holds its
+// sum in every lane, the same in each since a + b == b + a), then the four row sums, read by lane
+// and added in a fixed order. No LDS round trip: about a tenth of a shuffle tree's latency.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ inline double lane_value(double v, int lane) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline double wave_sum_all(double v) {
+  v = v + dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+  v = v + dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+  v = v + dpp_move<0x141>(v);  // row_half_mirror: lane i <-> 7 - i in each half-row
+  v = v + dpp_move<0x140>(v);  // row_mirror: lane i <-> 15 - i in each row
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
+// Thread groups of the tail. Levels of more than kWaveCells cells (and the column-mapped ones of
+// more than 64) run on the whole workgroup; the smaller ones and the coarsest solve run on its
 // first wave alone, which needs no workgroup barrier: a wave-level fence orders its LDS and
-// global accesses, and sums are wave reductions. Results of sums are valid in thread 0.
+// global accesses. sums() leaves the group's sums in every thread and orders the group's earlier
+// writes before its later reads.
 constexpr int kWaveCells = 512;
+constexpr int kTailWaves = kTailThreads / 64;
 struct TailBlock {
   int tid, nt;
+  double (*red)[kTailWaves][4];  // two LDS buffers of wave sums, used in turn
+  mutable int flip;
   __device__ void sync() const { __syncthreads(); }
+  // One barrier: each wave's sum is stored, then every thread adds the wave sums in wave order.
+  // A buffer is rewritten two calls later, after every thread has passed the barrier of the call
+  // between, so its reads are done.
   template <int N>
   __device__ void sums(double (&v)[N]) const {
-    int ops[N];
+    static_assert(N <= 4, "at most four sums at once");
 #pragma unroll
-    for(int k = 0; k < N; ++k) ops[k] = kSum;
-    block_reduce<N>(v, ops);
+    for(int k = 0; k < N; ++k) v[k] = wave_sum_all(v[k]);
+    double (*r)[4] = red[flip];
+    flip ^= 1;
+    if((tid & 63) == 0) {
+#pragma unroll
+      for(int k = 0; k < N; ++k) r[tid >> 6][k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for(int k = 0; k < N; ++k) {
+      double a = r[0][k];
+#pragma unroll
+      for(int w = 1; w < kTailWaves; ++w) a = a + r[w][k];
+      v[k] = a;
+    }
   }
 };
 struct TailWave {
@@ -1815,9 +1863,63 @@ struct TailWave {
   template <int N>
   __device__ void sums(double (&v)[N]) const {
 #pragma unroll
-    for(int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+    for(int k = 0; k < N; ++k) v[k] = wave_sum_all(v[k]);
+    sync();
   }
 };
+
+// Column map of a tail level that is periodic on every axis, has power-of-two sizes and halves
+// every axis: thread t of a group of nt threads owns the cells (i, j, kq km + m), m < km, with
+// i = t mod nx, j = (t / nx) mod ny, kq = t / (nx ny) and km = cells / nt. Its neighbour and
+// parent offsets come from shifts and masks once per thread (the generic loops divide the cell
+// id and test every boundary per cell: about 130 instructions a cell on the tail's one CU,
+// against about 50 here). The eight children of a coarse cell sit in lanes t and t ^ 1 (x),
+// t ^ nx (y) and in cells m, m + 1 of a thread (z; lanes t ^ (nx ny) when km = 1), so the
+// restriction sums them by lane exchanges in the generic loops' order: x pairs, then y pairs,
+// then z pairs. Every per-cell value is the generic formula's with all signs +1 and the uniform
+// weight.
+__device__ inline bool pow2i(int v) { return v > 0 && (v & (v - 1)) == 0; }
+__device__ inline int log2i(int v) { return 31 - __clz(v); }
+
+// Cells per thread: cells / nt on a level of at least nt cells, else 2 (on cells / 2 threads).
+__device__ inline int col_km(const PLevel& L, int nt) { return L.n >= nt ? L.n / nt : 2; }
+__device__ inline bool col_ok(const PLevel& L, const PLevel& Cl, int nt) {
+  if(!(pow2i(L.nx) && pow2i(L.ny) && pow2i(L.nz))) return false;
+  if(L.nx < 2 || L.ny < 2 || L.nz < 2 || L.nx > 32) return false;
+  if(L.rx != 2 || L.ry != 2 || L.rz != 2) return false;
+  if(2 * Cl.nx != L.nx || 2 * Cl.ny != L.ny || 2 * Cl.nz != L.nz) return false;
+  const int km = col_km(L, nt), na = L.n / km, plane = L.nx * L.ny;
+  if(na % 64 != 0 || na > nt || km > L.nz) return false;  // whole waves own cells
+  return km >= 2 ? km % 2 == 0 : 2 * plane <= 64;
+}
+
+struct ColMap {
+  int km, k0, o0;          // cells per thread, first k, offset of the first cell
+  int plane, nz;
+  int dxm, dxp, dym, dyp;  // in-plane neighbour offsets (periodic wrap)
+  int i, j;
+  bool act;  // the thread owns cells (whole waves do)
+  __device__ int dzm(int k) const { return k == 0 ? (nz - 1) * plane : -plane; }
+  __device__ int dzp(int k) const { return k == nz - 1 ? (1 - nz) * plane : plane; }
+};
+__device__ inline ColMap col_map(const PLevel& L, int t, int nt) {
+  ColMap c;
+  const int lx = log2i(L.nx), ly = log2i(L.ny);
+  c.i = t & (L.nx - 1);
+  c.j = (t >> lx) & (L.ny - 1);
+  const int kq = t >> (lx + ly);
+  c.km = col_km(L, nt);
+  c.act = t < L.n / c.km;
+  c.plane = L.nx * L.ny;
+  c.nz = L.nz;
+  c.k0 = kq * c.km;
+  c.o0 = c.i + (c.j << lx) + c.k0 * c.plane;
+  c.dxm = c.i == 0 ? L.nx - 1 : -1;
+  c.dxp = c.i == L.nx - 1 ? 1 - L.nx : 1;
+  c.dym = c.j == 0 ? (L.ny - 1) * L.nx : -L.nx;
+  c.dyp = c.j == L.ny - 1 ? (1 - L.ny) * L.nx : L.nx;
+  return c;
+}
 
 // LDS: every tail level lives in one LDS buffer (b of level l0 is copied in, x of level l0
 // copied out); otherwise the levels are read and written in global memory.
@@ -1826,34 +1928,49 @@ __global__ void __launch_bounds__(kTailThreads)
     k_vtail(const PLevel* __restrict__ lvs, int l0, int nl, TailBufs<T> B, const int* __restrict__ perm,
             const double* __restrict__ factor, const double* __restrict__ U, double* __restrict__ gwork, double* ps,
             int zm, DownFold fold) {
-  if(stopped(ps)) return;
   __shared__ double sbuf[kTailLdsBytes / sizeof(double)];
   T* sT = reinterpret_cast<T*>(sbuf);
   auto Bp = [&](int l) -> T* { return LDS ? sT + B.ob[l] : B.b[l]; };
   auto XA = [&](int l) -> T* { return LDS ? sT + B.oxa[l] : B.xa[l]; };
   auto XB = [&](int l) -> T* { return LDS ? sT + B.oxb[l] : B.xb[l]; };
   double* work = LDS ? sbuf + B.owork : gwork;
-  // The start's global reads (level l0's b when the levels are LDS-resident, a small coarsest
-  // system's solution operator) are issued before the fold, so their latencies overlap with it;
-  // the barrier after the fold orders the LDS stores before any use. A coarsest system of at most
-  // kCoarseStage unknowns is solved by its solution operator C (in `factor`, built on the host
-  // from the recorded elimination): one matrix-vector product on the first wave instead of the
-  // serial elimination and back substitution.
+  // The start's global reads are issued together, ahead of the stop test: the stop flag, level
+  // l0's b when the levels are LDS-resident, a small coarsest system's solution operator, the
+  // partials the fold combines and the tail levels' parameter structs (the last two only to bring
+  // them into the caches: the fold's own loads and each level's scalar loads of its fields then
+  // hit there). The empty asm statements take the loaded values, so the loads complete before
+  // the test and are not moved past it: one memory round trip instead of three (flag, level
+  // sizes, data). A coarsest system of at most kCoarseStage unknowns is solved by its solution
+  // operator C (in `factor`, built on the host from the recorded elimination): one matrix-vector
+  // product on the first wave instead of the serial elimination and back substitution.
   __shared__ double cop[kCoarseStage * kCoarseStage];
   static_assert(kCoarseStage * kCoarseStage <= kTailThreads, "one staged entry per thread");
   const int tid = threadIdx.x;
-  const int n0 = lvs[l0].n;
+  const double done = ps[P_DONE];
+  const int n0 = B.n0;
   T bpre[kTailPre];
   if(LDS) {
 #pragma unroll
-    for(int u = 0; u < kTailPre; ++u) {
-      const int id = tid + u * (int)blockDim.x;
-      bpre[u] = id < n0 ? B.b[l0][id] : T(0);
-    }
+    for(int u = 0; u < kTailPre; ++u) bpre[u] = B.b0[min(tid + u * kTailThreads, n0 - 1)];
   }
-  const int ncs = lvs[nl - 1].n;
+  const int ncs = B.ncs;
   const bool cstage = ncs <= kCoarseStage;
-  const double cpre = cstage && tid < ncs * ncs ? factor[tid] : 0.0;
+  const double cload = factor[min(tid, ncs * ncs - 1)];  // in the operator or the multipliers
+  const bool fsrc = fold.src != nullptr && fold.nb > 0;
+  const unsigned fb = fsrc ? min((unsigned)tid, fold.nb - 1) : 0u;
+  const double* f0 = fsrc ? fold.src + fb : ps;  // ps[0]: an in-bounds stand-in
+  const double* f1 = fsrc ? fold.src + kMaxPartials + fb : ps;
+  const double* f2 = fsrc ? fold.src + 2 * (size_t)kMaxPartials + fb : ps;
+  const double fwarm = *f0 + *f1 + *f2;
+  const int nw = (int)((nl - l0) * sizeof(PLevel) / sizeof(float));
+  const float lwarm = reinterpret_cast<const float*>(lvs + l0)[min(tid, nw - 1)];
+  asm volatile("" ::"v"(cload), "v"(fwarm), "v"(lwarm));
+  if(LDS) {
+#pragma unroll
+    for(int u = 0; u < kTailPre; ++u) asm volatile("" ::"v"(bpre[u]));
+  }
+  if(done != 0.0) return;
+  const double cpre = cstage && tid < ncs * ncs ? cload : 0.0;
   __shared__ double lm[kMaxLevels][4];  // per level: mb, m2, m4, m5
   __shared__ double lsbw[kMaxLevels];   // sum of raw b w/D
   __shared__ double rz_sh;
@@ -1864,10 +1981,10 @@ __global__ void __launch_bounds__(kTailThreads)
     T* dst = Bp(l0);
 #pragma unroll
     for(int u = 0; u < kTailPre; ++u) {
-      const int id = tid + u * (int)blockDim.x;
+      const int id = tid + u * kTailThreads;
       if(id < n0) dst[id] = bpre[u];
     }
-    for(int id = tid + kTailPre * (int)blockDim.x; id < n0; id += blockDim.x) dst[id] = B.b[l0][id];
+    for(int id = tid + kTailPre * kTailThreads; id < n0; id += kTailThreads) dst[id] = B.b[l0][id];
   }
   if(cstage && tid < ncs * ncs) cop[tid] = cpre;
   if(tid == 0) {
@@ -1875,105 +1992,210 @@ __global__ void __launch_bounds__(kTailThreads)
     lsbw[l0] = sbw0;
   }
   __syncthreads();
-  // down leg of level l: two sweeps (x2 and its mean), residual restricted to level l + 1
-  auto down = [&](int l, const auto& G) {
+  // down leg of level l: two sweeps (x2 and its mean), residual restricted to level l + 1. On
+  // entry mbl and sbwl hold level l's rhs mean and sum of raw b w/D in every thread of G; on
+  // return, level l + 1's (also kept in lm and lsbw for the up leg).
+  auto down = [&](int l, const auto& G, double& mbl, double& sbwl) {
     const PLevel& L = lvs[l];
     const PLevel& Cl = lvs[l + 1];
     const T* b = Bp(l);
     T* x2 = XA(l);
-    const T mb = T(lm[l][0]);
-    const T m1 = zm ? T((lsbw[l] - lm[l][0] * L.winv_sum) / (double)L.n) : T(0);
-    const int pxy = L.nx * L.ny;
-    double acc[1] = {0.0};
-    for(int id = G.tid; id < L.n; id += G.nt) {
-      const int k = small_div(id, pxy), rem = id - k * pxy, j = small_div(rem, L.nx), i = rem - j * L.nx;
-      const T xv = smooth12_cell<T, UNI, int>(L, b, mb, m1, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
-      x2[id] = xv;
-      acc[0] += (double)xv;
-    }
-    G.sums(acc);
-    if(G.tid == 0) lm[l][1] = zm ? acc[0] / (double)L.n : 0.0;
-    G.sync();
-    const T mx = T(lm[l][1]);
-    const T inv_children = T(1.0 / (double)(L.rx * L.ry * L.rz));
-    T* bc = Bp(l + 1);
-    const int cxy = Cl.nx * Cl.ny;
-    double sc[2] = {0.0, 0.0};
-    for(int id = G.tid; id < Cl.n; id += G.nt) {
-      const int kc = small_div(id, cxy), rem = id - kc * cxy, jc = small_div(rem, Cl.nx), ic = rem - jc * Cl.nx;
-      const int i0 = ic * L.rx, j0 = jc * L.ry, k0 = kc * L.rz;
-      auto R = [&](int i, int j, int k) {
-        return residual_cell<T, UNI, int>(L, b, x2, mb, mx, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
+    const T mb = T(mbl);
+    const T m1 = zm ? T((sbwl - mbl * L.winv_sum) / (double)L.n) : T(0);
+    double sc[2] = {0.0, 0.0};  // coarse rhs: sum, sum of raw b w/D
+    if(UNI && col_ok(L, Cl, G.nt)) {  // smooth12_cell and residual_cell on the column map
+      const ColMap cm = col_map(L, G.tid, G.nt);
+      const T wc = T(L.winv[0]), wcr = wc * T(L.wr);
+      const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+      auto X1 = [&](int o) { return (b[o] - mb) * wc - m1; };
+      double acc[1] = {0.0};
+      for(int m = 0; cm.act && m < cm.km; ++m) {
+        const int k = cm.k0 + m, o = cm.o0 + m * cm.plane;
+        const T bt = b[o] - mb;
+        const T c = bt * wc - m1;
+        const T ax = lap_neg<T>(c, X1(o + cm.dxm), X1(o + cm.dxp), X1(o + cm.dym), X1(o + cm.dyp), X1(o + cm.dzm(k)),
+                                X1(o + cm.dzp(k)), ihx2, ihy2, ihz2);
+        const T xv = c + wcr * (bt + T(-1) * ax);
+        x2[o] = xv;
+        acc[0] += (double)xv;
+      }
+      G.sums(acc);
+      const double mxd = zm ? acc[0] / (double)L.n : 0.0;
+      if(G.tid == 0) lm[l][1] = mxd;
+      const T mx = T(mxd);
+      auto X = [&](int o) { return x2[o] - mx; };
+      auto res = [&](int m) {  // fine residual, summed over the x pair and then the y pair
+        const int k = cm.k0 + m, o = cm.o0 + m * cm.plane;
+        const T c = X(o);
+        const T ax = lap_neg<T>(c, X(o + cm.dxm), X(o + cm.dxp), X(o + cm.dym), X(o + cm.dyp), X(o + cm.dzm(k)),
+                                X(o + cm.dzp(k)), ihx2, ihy2, ihz2);
+        T r = (b[o] - mb) + T(-1) * ax;
+        r = r + __shfl_xor(r, 1, 64);
+        return r + __shfl_xor(r, L.nx, 64);
       };
-      // same combination order as the tiled kernel: x pairs, then y pairs, then z pairs
-      auto row = [&](int j, int k) {
-        T r = R(i0, j, k);
-        if(L.rx == 2) r = r + R(i0 + 1, j, k);
-        return r;
+      const T inv_children = T(1.0 / 8.0);
+      const bool owner = ((cm.i | cm.j) & 1) == 0;
+      T* bc = Bp(l + 1);
+      const double cw = Cl.winv[0];
+      const int cbase = (cm.i >> 1) + Cl.nx * (cm.j >> 1);
+      const int cplane = Cl.nx * Cl.ny;
+      auto put = [&](int k, T a) {  // k: the even fine plane of the pair
+        const T val = inv_children * a;
+        bc[cbase + cplane * (k >> 1)] = val;
+        sc[0] += (double)val;
+        sc[1] += (double)val * cw;
       };
-      auto pl = [&](int k) {
-        T r = row(j0, k);
-        if(L.ry == 2) r = r + row(j0 + 1, k);
-        return r;
-      };
-      T a = pl(k0);
-      if(L.rz == 2) a = a + pl(k0 + 1);
-      const T val = inv_children * a;
-      bc[id] = val;
-      sc[0] += (double)val;
-      sc[1] += (double)val * Cl.winv[bclass(ic, Cl.nx) + 4 * (bclass(jc, Cl.ny) + 4 * bclass(kc, Cl.nz))];
+      if(!cm.act) {
+      } else if(cm.km == 1) {
+        const T p = res(0);
+        const T a = p + __shfl_xor(p, cm.plane, 64);
+        if(owner && (cm.k0 & 1) == 0) put(cm.k0, a);
+      } else {
+        for(int m = 0; m < cm.km; m += 2) {
+          const T p0 = res(m), p1 = res(m + 1);
+          if(owner) put(cm.k0 + m, p0 + p1);
+        }
+      }
+    } else {
+      const int pxy = L.nx * L.ny;
+      double acc[1] = {0.0};
+      for(int id = G.tid; id < L.n; id += G.nt) {
+        const int k = small_div(id, pxy), rem = id - k * pxy, j = small_div(rem, L.nx), i = rem - j * L.nx;
+        const T xv = smooth12_cell<T, UNI, int>(L, b, mb, m1, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
+        x2[id] = xv;
+        acc[0] += (double)xv;
+      }
+      G.sums(acc);
+      const double mxd = zm ? acc[0] / (double)L.n : 0.0;
+      if(G.tid == 0) lm[l][1] = mxd;
+      const T mx = T(mxd);
+      const T inv_children = T(1.0 / (double)(L.rx * L.ry * L.rz));
+      T* bc = Bp(l + 1);
+      const int cxy = Cl.nx * Cl.ny;
+      for(int id = G.tid; id < Cl.n; id += G.nt) {
+        const int kc = small_div(id, cxy), rem = id - kc * cxy, jc = small_div(rem, Cl.nx), ic = rem - jc * Cl.nx;
+        const int i0 = ic * L.rx, j0 = jc * L.ry, k0 = kc * L.rz;
+        auto R = [&](int i, int j, int k) {
+          return residual_cell<T, UNI, int>(L, b, x2, mb, mx, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k));
+        };
+        // same combination order as the tiled kernel: x pairs, then y pairs, then z pairs
+        auto row = [&](int j, int k) {
+          T r = R(i0, j, k);
+          if(L.rx == 2) r = r + R(i0 + 1, j, k);
+          return r;
+        };
+        auto pl = [&](int k) {
+          T r = row(j0, k);
+          if(L.ry == 2) r = r + row(j0 + 1, k);
+          return r;
+        };
+        T a = pl(k0);
+        if(L.rz == 2) a = a + pl(k0 + 1);
+        const T val = inv_children * a;
+        bc[id] = val;
+        sc[0] += (double)val;
+        sc[1] += (double)val * Cl.winv[bclass(ic, Cl.nx) + 4 * (bclass(jc, Cl.ny) + 4 * bclass(kc, Cl.nz))];
+      }
     }
     G.sums(sc);
+    mbl = zm ? sc[0] / (double)Cl.n : 0.0;
+    sbwl = sc[1];
     if(G.tid == 0) {
-      lm[l + 1][0] = zm ? sc[0] / (double)Cl.n : 0.0;
-      lsbw[l + 1] = sc[1];
+      lm[l + 1][0] = mbl;
+      lsbw[l + 1] = sbwl;
     }
-    G.sync();
   };
-  // up leg of level l: prolongation + sweep (x4 and its mean), last sweep (x5, its mean, r.z)
-  auto up = [&](int l, const auto& G) {
+  // up leg of level l: prolongation + sweep (x4 and its mean), last sweep (x5, its mean, r.z). On
+  // entry mcl holds the mean of level l + 1's x in every thread of G; on return, level l's.
+  auto up = [&](int l, const auto& G, double& mcl) {
     const PLevel& L = lvs[l];
     const PLevel& Cl = lvs[l + 1];
     const T* b = Bp(l);
-    const T mb = T(lm[l][0]), m2 = T(lm[l][1]), mc = T(lm[l + 1][3]);
+    const T mb = T(lm[l][0]), m2 = T(lm[l][1]), mc = T(mcl);
+    const bool col = UNI && col_ok(L, Cl, G.nt);
     const int pxy = L.nx * L.ny;
     double acc[1] = {0.0};
-    for(int id = G.tid; id < L.n; id += G.nt) {
-      const int k = small_div(id, pxy), rem = id - k * pxy, j = small_div(rem, L.nx), i = rem - j * L.nx;
-      const T xv =
-          prolong_smooth_cell<T, UNI, int>(L, Cl, b, XA(l), XA(l + 1), mb, m2, mc, i, j, k, nb_x(L, i), nb_y(L, j),
-                                           nb_z(L, k));
-      XB(l)[id] = xv;
-      acc[0] += (double)xv;
+    if(col) {  // prolong_smooth_cell on the column map
+      const ColMap cm = col_map(L, G.tid, G.nt);
+      const T wcr = T(L.winv[0]) * T(L.wr);
+      const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+      const T* x2 = XA(l);
+      const T* xc = XA(l + 1);
+      T* x4 = XB(l);
+      const int cnx = Cl.nx, cplane = Cl.nx * Cl.ny;
+      const int px = cm.i >> 1, pxm = ((cm.i - 1) & (L.nx - 1)) >> 1, pxp = ((cm.i + 1) & (L.nx - 1)) >> 1;
+      const int py = (cm.j >> 1) * cnx, pym = (((cm.j - 1) & (L.ny - 1)) >> 1) * cnx,
+                pyp = (((cm.j + 1) & (L.ny - 1)) >> 1) * cnx;
+      for(int m = 0; cm.act && m < cm.km; ++m) {
+        const int k = cm.k0 + m, o = cm.o0 + m * cm.plane;
+        const int pz = (k >> 1) * cplane, pzm = (((k - 1) & (L.nz - 1)) >> 1) * cplane,
+                  pzp = (((k + 1) & (L.nz - 1)) >> 1) * cplane;
+        auto X3 = [&](int of, int oc) { return (x2[of] - m2) + (xc[oc] - mc); };
+        const T c = X3(o, px + py + pz);
+        const T ax = lap_neg<T>(c, X3(o + cm.dxm, pxm + py + pz), X3(o + cm.dxp, pxp + py + pz),
+                                X3(o + cm.dym, px + pym + pz), X3(o + cm.dyp, px + pyp + pz),
+                                X3(o + cm.dzm(k), px + py + pzm), X3(o + cm.dzp(k), px + py + pzp), ihx2, ihy2, ihz2);
+        const T xv = c + wcr * ((b[o] - mb) + T(-1) * ax);
+        x4[o] = xv;
+        acc[0] += (double)xv;
+      }
+    } else {
+      for(int id = G.tid; id < L.n; id += G.nt) {
+        const int k = small_div(id, pxy), rem = id - k * pxy, j = small_div(rem, L.nx), i = rem - j * L.nx;
+        const T xv = prolong_smooth_cell<T, UNI, int>(L, Cl, b, XA(l), XA(l + 1), mb, m2, mc, i, j, k, nb_x(L, i),
+                                                      nb_y(L, j), nb_z(L, k));
+        XB(l)[id] = xv;
+        acc[0] += (double)xv;
+      }
     }
     G.sums(acc);
-    if(G.tid == 0) lm[l][2] = zm ? acc[0] / (double)L.n : 0.0;
-    G.sync();
-    const T m4 = T(lm[l][2]);
+    const double m4d = zm ? acc[0] / (double)L.n : 0.0;
+    if(G.tid == 0) lm[l][2] = m4d;
+    const T m4 = T(m4d);
     double s5[2] = {0.0, 0.0};  // sum of x5, r.z
-    for(int id = G.tid; id < L.n; id += G.nt) {
-      const int k = small_div(id, pxy), rem = id - k * pxy, j = small_div(rem, L.nx), i = rem - j * L.nx;
-      T bt;
-      const T xv =
-          smooth_last_cell<T, UNI, int>(L, b, XB(l), mb, m4, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k), bt);
-      XA(l)[id] = xv;
-      s5[0] += (double)xv;
-      s5[1] += (double)bt * (double)xv;
+    if(col) {  // smooth_last_cell on the column map
+      const ColMap cm = col_map(L, G.tid, G.nt);
+      const T wc = T(L.winv[0]);
+      const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+      const T* x4 = XB(l);
+      T* x5 = XA(l);
+      auto X = [&](int o) { return x4[o] - m4; };
+      for(int m = 0; cm.act && m < cm.km; ++m) {
+        const int k = cm.k0 + m, o = cm.o0 + m * cm.plane;
+        const T c = X(o);
+        const T ax = lap_neg<T>(c, X(o + cm.dxm), X(o + cm.dxp), X(o + cm.dym), X(o + cm.dyp), X(o + cm.dzm(k)),
+                                X(o + cm.dzp(k)), ihx2, ihy2, ihz2);
+        const T bt = b[o] - mb;
+        const T xv = c + wc * (bt + T(-1) * ax);
+        x5[o] = xv;
+        s5[0] += (double)xv;
+        s5[1] += (double)bt * (double)xv;
+      }
+    } else {
+      for(int id = G.tid; id < L.n; id += G.nt) {
+        const int k = small_div(id, pxy), rem = id - k * pxy, j = small_div(rem, L.nx), i = rem - j * L.nx;
+        T bt;
+        const T xv =
+            smooth_last_cell<T, UNI, int>(L, b, XB(l), mb, m4, i, j, k, nb_x(L, i), nb_y(L, j), nb_z(L, k), bt);
+        XA(l)[id] = xv;
+        s5[0] += (double)xv;
+        s5[1] += (double)bt * (double)xv;
+      }
     }
     G.sums(s5);
+    mcl = zm ? s5[0] / (double)L.n : 0.0;
     if(G.tid == 0) {
-      lm[l][3] = zm ? s5[0] / (double)L.n : 0.0;
+      lm[l][3] = mcl;
       if(l == 0) rz_sh = s5[1];
     }
-    G.sync();
   };
-  // coarsest level: elimination with the recorded row swaps and multipliers
-  auto coarsest = [&](const auto& G) {
+  // coarsest level, rhs mean mbc: elimination with the recorded row swaps and multipliers
+  auto coarsest = [&](const auto& G, double mbc) {
     const int lc = nl - 1;
     const int n = lvs[lc].n;
     const T* b = Bp(lc);
     T* x = XA(lc);
-    const T mb = T(lm[lc][0]);
+    const T mb = T(mbc);
     for(int i = G.tid; i < n; i += G.nt) work[i] = (double)(b[i] - mb);
     G.sync();
     if(cstage) {  // x = C (b - mb), row i on lane i
@@ -2035,27 +2257,32 @@ __global__ void __launch_bounds__(kTailThreads)
     }
     G.sync();
   };
-  const TailBlock GB{tid, (int)blockDim.x};
+  __shared__ double tred[2][kTailWaves][4];
+  const TailBlock GB{tid, (int)blockDim.x, tred, 0};
   // levels l0 .. lw - 1 on the workgroup, lw .. nl - 1 on the first wave
+  auto on_block = [&](int l) {
+    return lvs[l].n > kWaveCells || (UNI && lvs[l].n > 64 && col_ok(lvs[l], lvs[l + 1], kTailThreads));
+  };
   int lw = l0;
-  while(lw < nl - 1 && lvs[lw].n > kWaveCells) ++lw;
+  while(lw < nl - 1 && on_block(lw)) ++lw;
   const bool wave_coarsest = lvs[nl - 1].n <= kWaveCells;
-  for(int l = l0; l < lw; ++l) down(l, GB);
+  double mbl = lm[l0][0], sbwl = lsbw[l0];
+  for(int l = l0; l < lw; ++l) down(l, GB, mbl, sbwl);
   if(tid < 64) {
     const TailWave GW{tid, 64};
-    for(int l = lw; l < nl - 1; ++l) down(l, GW);
-    if(wave_coarsest) coarsest(GW);
+    for(int l = lw; l < nl - 1; ++l) down(l, GW, mbl, sbwl);
+    if(wave_coarsest) coarsest(GW, mbl);
   }
   __syncthreads();
-  if(!wave_coarsest) {
-    coarsest(GB);
-  }
+  if(!wave_coarsest) coarsest(GB, mbl);  // every level above it ran on the workgroup
+  double mcl = 0.0;  // mean of the coarsest level's x
   if(tid < 64) {
     const TailWave GW{tid, 64};
-    for(int l = nl - 2; l >= lw; --l) up(l, GW);
+    for(int l = nl - 2; l >= lw; --l) up(l, GW, mcl);
   }
   __syncthreads();
-  for(int l = min(lw, nl - 1) - 1; l >= l0; --l) up(l, GB);
+  mcl = lm[min(lw, nl - 1)][3];
+  for(int l = min(lw, nl - 1) - 1; l >= l0; --l) up(l, GB, mcl);
   if(LDS) {
     const int n0 = lvs[l0].n;
     const T* src = XA(l0);
@@ -3345,6 +3572,9 @@ void PoissonSolver::vcycle() {
     }
     const size_t work_off = (tot * sizeof(T) + sizeof(double) - 1) / sizeof(double);
     tb.owork = (int)work_off;
+    tb.n0 = lv[l0].n;
+    tb.ncs = lv[nl - 1].n;
+    tb.b0 = bptr(l0);
     const bool in_lds = (work_off + (size_t)nc) * sizeof(double) <= (size_t)kTailLdsBytes;
     ProfScope prof_(c, "k_vtail");
     // all tail levels share the boundary types of the finest level
