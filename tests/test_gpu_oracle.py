@@ -116,6 +116,40 @@ def test_poisson_periodic_64_fft_and_oracle(ctx, precision):
     assert rel_max(gp, ref[1:-1, 1:-1, 1:-1]) <= 1e-10
 
 
+# Periodic power-of-two grids whose small levels take the one-workgroup tail's column map in each
+# of its shapes: the finest level inside the tail (16^3; 32x8x16 with rows of 32), levels of 512
+# cells on four waves, one cell per lane with the restriction's z pairs across lanes (planes of 8,
+# 16 and 32 cells), anisotropic sizes, and the generic loops once an axis stops halving. The FFT solution is an independent closed form. The spacings are unequal but close:
+# with strongly anisotropic ones the point-Jacobi V-cycle needs 80-odd iterations and the residual
+# stalls near the 1e-12 tolerance, where rounding alone moves the iteration that first dips below
+# it (16x32x64 with spacings 1/16, 1.7/32, 0.8/64: 85 here, 88 in the oracle, the same before the
+# column map existed).
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 32, 32), (64, 32, 16), (16, 32, 64), (32, 8, 16)])
+@pytest.mark.parametrize("precision", [sa.FP64, sa.MIXED])
+def test_poisson_periodic_pow2_shapes_fft_and_oracle(ctx, shape, precision):
+    nx, ny, nz = shape
+    h = (0.9 / 32, 1.0 / 32, 1.1 / 32)
+    g = sa.grid(nx, ny, nz, *h)
+    og = ofref.make_grid(nx, ny, nz, *h)
+    rng = np.random.default_rng(nx + 3 * ny + 5 * nz)
+    rhs = rng.standard_normal(sa.storage_shape(nx, ny, nz, sa.CELL))
+    tol = 1e-12
+    o = sa.ProjectionOptions(1.0, 1.0, 400, tol)
+    pb = sa.pressure_bc_set([sa.P_PERIODIC] * 6)
+    sol = zeros(g, sa.CELL)
+    d = sa.PoissonDiagnostics()
+    rhs_d = dev(rhs)
+    ctx.run("of_solve_pressure_poisson", C.byref(g), C.byref(pb), C.byref(o), precision, sa.ptr(rhs_d), sa.ptr(sol),
+            C.byref(d), None, 0)
+    assert d.converged == 1 and d.relative_residual <= tol
+    gp = host(sol)[1:-1, 1:-1, 1:-1]
+    assert rel_max(gp, periodic_fft_solution(rhs[1:-1, 1:-1, 1:-1], h)) <= 1e-9
+    ref, rd, _ = ofref.poisson(og, ofref.pbcset([ofref.P_PERIODIC] * 6), ofref.opts(1.0, 1.0, 400, tol), rhs.copy())
+    if precision == sa.FP64:
+        assert d.iterations == rd.iterations
+    assert rel_max(gp, ref[1:-1, 1:-1, 1:-1]) <= 1e-10
+
+
 @pytest.mark.parametrize("pbc,shape", [
     ([1, 1, 1, 1, 0, 0], (48, 40, 1)),
     ([0, 0, 0, 0, 0, 0], (40, 48, 1)),
