@@ -114,6 +114,37 @@ constexpr int kFinalizeThreads = 1024;
 
 enum RedOp : int { kSum = 0, kMax = 1 };
 
+// DPP lane moves of a double (its two 32-bit halves moved alike).
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// The value of a double in one lane (uniform).
+__device__ inline double lane_value(double v, int lane) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// Sum (op kSum) or maximum (kMax) of a double over a full wave, the same value in every lane:
+// butterfly exchanges within each row of 16 lanes by DPP moves (lane ^ 1, lane ^ 2, half-row
+// mirror, row mirror: every lane of a row then holds the row's result, the same in each since the
+// operations commute exactly), then the four row results, read by lane and combined in a fixed
+// order. No LDS round trip: a shuffle tree through the LDS permute unit takes about ten times as
+// long. Every lane of the wave must be active.
+__device__ inline double wave_reduce_all(double v, int op) {
+  auto f = [op](double a, double b) { return op == kSum ? a + b : fmax(a, b); };
+  v = f(v, dpp_move<0xB1>(v));   // quad_perm [1,0,3,2]: lane ^ 1
+  v = f(v, dpp_move<0x4E>(v));   // quad_perm [2,3,0,1]: lane ^ 2
+  v = f(v, dpp_move<0x141>(v));  // row_half_mirror: lane i <-> 7 - i in each half-row
+  v = f(v, dpp_move<0x140>(v));  // row_mirror: lane i <-> 15 - i in each row
+  return f(f(lane_value(v, 0), lane_value(v, 16)), f(lane_value(v, 32), lane_value(v, 48)));
+}
+__device__ inline double wave_sum_all(double v) { return wave_reduce_all(v, kSum); }
+// Shuffle-tree reductions for blocks whose last wave is partial (result valid in lane 0).
 __device__ inline double wave_sum(double v) {
   for(int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   return v;
@@ -124,7 +155,9 @@ __device__ inline double wave_max(double v) {
 }
 
 // Block-wide reduction of NV values, result valid in thread 0. ops[k] selects sum or max.
-// Blocks of up to 1024 threads.
+// Blocks of up to 1024 threads. Each wave reduces its values, then wave 0 reduces the wave
+// results (one per lane, the identity in the lanes past the last wave): DPP moves when the block
+// is whole waves, shuffle trees otherwise.
 template <int NV>
 __device__ inline void block_reduce(double (&v)[NV], const int (&ops)[NV]) {
   __shared__ double red[16][NV];
@@ -132,19 +165,22 @@ __device__ inline void block_reduce(double (&v)[NV], const int (&ops)[NV]) {
   const int nthreads = blockDim.x * blockDim.y * blockDim.z;
   const int lane = tid & 63, wave = tid >> 6;
   const int nwaves = (nthreads + 63) >> 6;
+  const bool full = (nthreads & 63) == 0;
+  auto reduce = [full](double x, int op) {
+    return full ? wave_reduce_all(x, op) : (op == kSum ? wave_sum(x) : wave_max(x));
+  };
 #pragma unroll
-  for(int k = 0; k < NV; ++k) v[k] = ops[k] == kSum ? wave_sum(v[k]) : wave_max(v[k]);
+  for(int k = 0; k < NV; ++k) v[k] = reduce(v[k], ops[k]);
   if(lane == 0) {
 #pragma unroll
     for(int k = 0; k < NV; ++k) red[wave][k] = v[k];
   }
   __syncthreads();
-  if(tid == 0) {
+  if(wave == 0) {
 #pragma unroll
     for(int k = 0; k < NV; ++k) {
-      double acc = red[0][k];
-      for(int w = 1; w < nwaves; ++w) acc = ops[k] == kSum ? acc + red[w][k] : fmax(acc, red[w][k]);
-      v[k] = acc;
+      const double x = red[min(lane, nwaves - 1)][k];
+      v[k] = reduce(lane < nwaves ? x : (ops[k] == kSum ? 0.0 : -INFINITY), ops[k]);
     }
   }
   __syncthreads();

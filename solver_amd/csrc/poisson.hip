@@ -1793,31 +1793,6 @@ constexpr int kTailLdsBytes = 152 * 1024;
 constexpr int kCoarseStage = 16;  // coarsest systems of up to this many unknowns are read from LDS
 constexpr int kTailPre = 4;       // level-l0 values per thread read ahead of the fold (4096 cells)
 
-// Sum of a double over a wave, the same value in every lane: butterfly exchanges within each row
-// of 16 lanes by DPP moves (lane ^ 1, lane ^ 2, half-row mirror, row mirror; a row then holds its
-// sum in every lane, the same in each since a + b == b + a), then the four row sums, read by lane
-// and added in a fixed order. No LDS round trip: about a tenth of a shuffle tree's latency.
-template <int CTRL>
-__device__ inline double dpp_move(double v) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-__device__ inline double lane_value(double v, int lane) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-__device__ inline double wave_sum_all(double v) {
-  v = v + dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
-  v = v + dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
-  v = v + dpp_move<0x141>(v);  // row_half_mirror: lane i <-> 7 - i in each half-row
-  v = v + dpp_move<0x140>(v);  // row_mirror: lane i <-> 15 - i in each row
-  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
-}
-
 // Thread groups of the tail. Levels of more than kWaveCells cells (and the column-mapped ones of
 // more than 64) run on the whole workgroup; the smaller ones and the coarsest solve run on its
 // first wave alone, which needs no workgroup barrier: a wave-level fence orders its LDS and
