@@ -144,6 +144,12 @@ __device__ inline double wave_reduce_all(double v, int op) {
   return f(f(lane_value(v, 0), lane_value(v, 16)), f(lane_value(v, 32), lane_value(v, 48)));
 }
 __device__ inline double wave_sum_all(double v) { return wave_reduce_all(v, kSum); }
+// The value held by lane ^ 1, by a DPP quad permute ([1,0,3,2]): an exact move, so the value of
+// __shfl_xor(v, 1, 64) without a trip through the LDS permute unit. Every lane must be active.
+__device__ inline double lane_xor1(double v) { return dpp_move<0xB1>(v); }
+__device__ inline float lane_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
 // Shuffle-tree reductions for blocks whose last wave is partial (result valid in lane 0).
 __device__ inline double wave_sum(double v) {
   for(int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

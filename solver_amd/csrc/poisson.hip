@@ -1050,7 +1050,7 @@ __global__ void __launch_bounds__(32 * ROWS) k_down_fused(const T* __restrict__ 
         const T ax = lap_neg<T>(c, sx[sc][xe - 1], sx[sc][xe + 1], sx[sc][xe - XW], sx[sc][xe + XW], sx[sm][xe],
                                 sx[sp][xe], ihx2, ihy2, ihz2);
         T r = q2[rr] + T(-1) * ax;
-        r += __shfl_xor(r, 1, 64);  // x pair
+        r += lane_xor1(r);  // x pair
         xs[rr] = r;
       }
       const T ysum = xs[0] + xs[1];  // y pair
@@ -1584,7 +1584,7 @@ __global__ void __launch_bounds__(512) k_down_small(const T* __restrict__ b, T* 
       const T ax = lap_neg<T>(c, sc[xe - 1], sc[xe + 1], sc[xe - XW], sc[xe + XW], sm[xe], sp[xe], ihx2, ihy2, ihz2);
       const T bq = (rr == 0 ? own0 : own1) ? tb[r][rr] - mb : T(0);
       T res = bq + T(-1) * ax;
-      res += __shfl_xor(res, 1, 64);  // x pair
+      res += lane_xor1(res);  // x pair
       xs[rr] = res;
     }
     const T ysum = xs[0] + xs[1];  // y pair
@@ -2005,7 +2005,7 @@ __global__ void __launch_bounds__(kTailThreads)
         const T ax = lap_neg<T>(c, X(o + cm.dxm), X(o + cm.dxp), X(o + cm.dym), X(o + cm.dyp), X(o + cm.dzm(k)),
                                 X(o + cm.dzp(k)), ihx2, ihy2, ihz2);
         T r = (b[o] - mb) + T(-1) * ax;
-        r = r + __shfl_xor(r, 1, 64);
+        r = r + lane_xor1(r);
         return r + __shfl_xor(r, L.nx, 64);
       };
       const T inv_children = T(1.0 / 8.0);
