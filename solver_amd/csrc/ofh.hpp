@@ -154,9 +154,12 @@ PoissonSolver* poisson_create_in(Context& c, const of_grid& g, const of_pressure
 void poisson_destroy(PoissonSolver* s);
 // Solves -lap(p) = rhs (reference sign conventions); rhs and p in ghosted cell storage.
 // p holds the warm start on entry (zero_guess: the warm start is zero and p is not read). Only
-// the active range of p is written.
+// the active range of p is written. With deferred_mean, a problem with a null space leaves the
+// final mean of p in it instead of subtracting it: *deferred_mean points at that mean in device
+// memory (null when there is none), valid until the solver's next solve, and the caller
+// subtracts it on read (p - mean is then bitwise the value the subtraction would have stored).
 PoissonResult poisson_solve(PoissonSolver* s, const double* rhs, double* p, int max_iterations, double tol,
-                            bool want_history, bool zero_guess = false);
+                            bool want_history, bool zero_guess = false, const double** deferred_mean = nullptr);
 void launch_poisson_apply(Context& c, const of_grid& g, const of_pressure_bc_set& b, bool homogeneous,
                           const double* in, double* out);
 

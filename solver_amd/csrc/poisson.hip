@@ -3052,7 +3052,8 @@ class PoissonSolver {
   template <typename T>
   void vcycle();
   template <typename T>
-  PoissonResult solve_t(const double* rhs, double* p, int maxit, double tol, bool want_history, bool zero_guess);
+  PoissonResult solve_t(const double* rhs, double* p, int maxit, double tol, bool want_history, bool zero_guess,
+                        const double** deferred_mean);
   // the array holding a level's smoothed correction after the up leg: the fused up kernel
   // writes it beside x2 (its halo is still being read by neighbouring tiles)
   void* x_final(int l) const { return l < l0 && fused[l] ? lxb[l] : lxa[l]; }
@@ -3636,7 +3637,7 @@ void PoissonSolver::vcycle() {
 
 template <typename T>
 PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, double tol, bool want_history,
-                                     bool zero_guess) {
+                                    bool zero_guess, const double** deferred_mean) {
   if(hist_cap < maxit + 1) {
     if(history && !arena) OFH_CHECK(hipFree(history));
     if(coef && !arena) OFH_CHECK(hipFree(coef));
@@ -3750,7 +3751,8 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
   OFH_LAUNCH(k_pfinal, pg, dim3(256), 0, c.stream, p, pc, dr, folded, folded == 0 && zero_guess ? 1 : 0, F, Lc,
              c.partials, ps);
   launch_finalize(c.stream, c.partials, pg, EpiPfinal{ps, n0, zmi});
-  if(zm) OFH_LAUNCH(k_psub, ll.grid, ll.block, 0, c.stream, p, F, Lc, ps, ll.kz);
+  if(deferred_mean != nullptr) *deferred_mean = zm ? ps + P_PFINAL : nullptr;
+  if(zm && deferred_mean == nullptr) OFH_LAUNCH(k_psub, ll.grid, ll.block, 0, c.stream, p, F, Lc, ps, ll.kz);
   OFH_CHECK(hipGetLastError());
   std::vector<double> hps(P_COUNT);
   OFH_CHECK(hipMemcpyAsync(hps.data(), ps, P_COUNT * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -3804,11 +3806,12 @@ PoissonSolver* poisson_create_in(Context& c, const of_grid& g, const of_pressure
 void poisson_destroy(PoissonSolver* s) { delete s; }
 
 PoissonResult poisson_solve(PoissonSolver* s, const double* rhs, double* p, int maxit, double tol, bool want_history,
-                            bool zero_guess) {
+                            bool zero_guess, const double** deferred_mean) {
   if(maxit <= 0) throw InvalidArgument("solve_pressure_poisson expects iterations > 0");
   if(!(tol > 0.0)) throw InvalidArgument("solve_pressure_poisson expects tolerance > 0");
-  if(s->precision == OF_PRECISION_MIXED) return s->solve_t<float>(rhs, p, maxit, tol, want_history, zero_guess);
-  return s->solve_t<double>(rhs, p, maxit, tol, want_history, zero_guess);
+  if(s->precision == OF_PRECISION_MIXED)
+    return s->solve_t<float>(rhs, p, maxit, tol, want_history, zero_guess, deferred_mean);
+  return s->solve_t<double>(rhs, p, maxit, tol, want_history, zero_guess, deferred_mean);
 }
 
 void launch_poisson_apply(Context& c, const of_grid& g, const of_pressure_bc_set& b, bool homogeneous,

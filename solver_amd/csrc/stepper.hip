@@ -70,7 +70,7 @@ constexpr unsigned kStepDiagMaxMask = 0x7eu;  // values 1..6 are maxima
 // against 19 for the three kernels.
 __global__ void k_finish_periodic(const double* __restrict__ pu, const double* __restrict__ pv,
                                   const double* __restrict__ pw, const double* __restrict__ phi,
-                                  const double* __restrict__ prhs, const double* __restrict__ uo,
+                                  const double* __restrict__ phi_mean, const double* __restrict__ prhs, const double* __restrict__ uo,
                                   const double* __restrict__ vo, const double* __restrict__ wo,
                                   double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
                                   double* __restrict__ pt, Lay Lu, Lay Lv, Lay Lw, Lay Lc, of_grid g, double scale,
@@ -78,6 +78,9 @@ __global__ void k_finish_periodic(const double* __restrict__ pu, const double* _
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   double vals[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // the solve's final mean of phi, subtracted on every read of phi (its ghosts are copies of
+  // active values taken before the subtraction): the values of the subtracted field, bitwise
+  const double m = phi_mean != nullptr ? *phi_mean : 0.0;
   if(i <= g.nx && j <= g.ny) {
     const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + g.nz);
     // per-thread column offsets plus uniform plane strides (no 64-bit index products per point)
@@ -88,8 +91,8 @@ __global__ void k_finish_periodic(const double* __restrict__ pu, const double* _
     const bool xe = i == g.nx, ye = j == g.ny;
     // phi of plane k and the lower w face of plane k carried along the column: wb of plane k is
     // wf of plane k - 1 (the same operands, so the same value)
-    const double phm = phi[oc + sc * (size_t)(kb - 1)];
-    double ph0 = phi[oc + sc * (size_t)kb];
+    const double phm = phi[oc + sc * (size_t)(kb - 1)] - m;
+    double ph0 = phi[oc + sc * (size_t)kb] - m;
     double wb = pw[ow + sw * (size_t)kb] + scale * ((ph0 - phm) / g.dz);
     double wob = wo[ow + sw * (size_t)kb];
     for(int k = kb; k < ke; ++k) {
@@ -97,8 +100,8 @@ __global__ void k_finish_periodic(const double* __restrict__ pu, const double* _
       const size_t w1 = ow + sw * (size_t)(k + 1);
       // every load is unconditional (the upper-face values of the line ends are read by every
       // thread and used where they apply), so no branch join waits for a load
-      const double php = phi[c0 + sc], pxm = phi[c0 - 1], pxp = phi[c0 + 1], pym = phi[c0 - cy],
-                   pyp = phi[c0 + cy];
+      const double php = phi[c0 + sc] - m, pxm = phi[c0 - 1] - m, pxp = phi[c0 + 1] - m,
+                   pym = phi[c0 - cy] - m, pyp = phi[c0 + cy] - m;
       const double pu0 = pu[u0], pu1 = pu[u0 + 1], pv0 = pv[v0], pv1 = pv[v0 + vy], pw1 = pw[w1];
       const double uo0 = uo[u0], uo1 = uo[u0 + 1], vo0 = vo[v0], vo1 = vo[v0 + vy], wo1 = wo[w1];
       const double ptc = pt[c0], rc = prhs[c0];
@@ -436,8 +439,10 @@ void Stepper::step() {
   // phi starts from zero (solver/taylor_green.cpp:596): the zero-warm-start solve neither reads
   // it nor needs it cleared; it writes the active range and the BC fill the face ghosts (the
   // edge ghosts keep their zeros from allocation)
-  const PoissonResult pr =
-      poisson_solve(poisson, prhs, phi, cfg.poisson_max_iterations, cfg.poisson_tolerance, false, true);
+  // on a periodic grid the final mean of phi is subtracted where the step end reads phi
+  const double* phi_mean = nullptr;
+  const PoissonResult pr = poisson_solve(poisson, prhs, phi, cfg.poisson_max_iterations, cfg.poisson_tolerance, false,
+                                         true, all_periodic ? &phi_mean : nullptr);
   if(!pr.diag.converged) {  // solver/projection.cpp:1133-1147
     char buf[512];
     std::snprintf(buf, sizeof buf,
@@ -452,7 +457,8 @@ void Stepper::step() {
     const ColumnLaunch cl = column_launch(g.nx, g.ny, g.nz, 16);
     {
       ProfScope prof_(c, "k_finish_periodic", true);
-      OFH_LAUNCH(k_finish_periodic, cl.grid, cl.block, 0, c.stream, pred[0], pred[1], pred[2], phi, prhs, vel[0],
+      OFH_LAUNCH(k_finish_periodic, cl.grid, cl.block, 0, c.stream, pred[0], pred[1], pred[2], phi, phi_mean, prhs,
+                 vel[0],
                  vel[1], vel[2], corr[0], corr[1], corr[2], p_total, make_lay(g, OF_FACE_X), make_lay(g, OF_FACE_Y),
                  make_lay(g, OF_FACE_Z), make_lay(g, OF_CELL), g, -dt / 1.0, -0.5 * nu * dt, dt, c.partials, cl.kz);
     }
