@@ -491,15 +491,15 @@ __global__ void k_setup1(const double* __restrict__ rhs_g, const double* __restr
 
 // k_setup1 for a zero warm start: p is neither read nor copied (the pressure accumulation
 // starts from zero).
-__global__ void k_setup1_zero(const double* __restrict__ rhs_g, Lay Lc, PLevel L, double* __restrict__ sys,
-                              double* partials, int kz) {
+// Sum of the system rhs sys = -rhs - src for its mean (a zero warm start, whose mean is 0). sys
+// is not stored: k_setup2_zero forms it again from rhs by the same expression (one read of rhs
+// instead of a write of sys and its read back).
+__global__ void k_setup1_zero(const double* __restrict__ rhs_g, Lay Lc, PLevel L, double* partials, int kz) {
   const Tile t = tile(L, kz);
   double v[2] = {0.0, 0.0};
   if(t.i < L.nx && t.j < L.ny) {
     for(int k = t.k0; k < t.k1; ++k) {
-      const size_t id = (size_t)t.i + (size_t)L.nx * ((size_t)t.j + (size_t)L.ny * k);
       const double s = -rhs_g[Lc.at(t.i + 1, t.j + 1, k + 1)] + -1.0 * boundary_source(L, t.i, t.j, k);
-      sys[id] = s;
       v[0] += s;
     }
   }
@@ -508,8 +508,11 @@ __global__ void k_setup1_zero(const double* __restrict__ rhs_g, Lay Lc, PLevel L
 }
 
 // k_setup2 for a zero warm start: A(p - m_p) = A 0 = 0 exactly, so r = sys - m_sys pointwise
-// (the same values as the stencil path).
-__global__ void k_setup2_zero(double* __restrict__ r, PLevel L, double* partials, const double* ps, int kz) {
+// (the same values as the stencil path), sys formed again from rhs as k_setup1_zero forms it.
+// lo (the mixed mode) also receives r rounded to T: the rhs of the finest FP32 level.
+template <typename T>
+__global__ void k_setup2_zero(const double* __restrict__ rhs_g, Lay Lc, double* __restrict__ r, T* __restrict__ lo,
+                              PLevel L, double* partials, const double* ps, int kz) {
   const double msys = ps[P_MSYS];
   const Tile t = tile(L, kz);
   double v[3] = {0.0, 0.0, 0.0};
@@ -517,8 +520,10 @@ __global__ void k_setup2_zero(double* __restrict__ r, PLevel L, double* partials
     const int cxy = bclass(t.i, L.nx) + 4 * bclass(t.j, L.ny);
     for(int k = t.k0; k < t.k1; ++k) {
       const size_t id = (size_t)t.i + (size_t)L.nx * ((size_t)t.j + (size_t)L.ny * k);
-      const double rv = r[id] - msys;
+      const double s = -rhs_g[Lc.at(t.i + 1, t.j + 1, k + 1)] + -1.0 * boundary_source(L, t.i, t.j, k);
+      const double rv = s - msys;
       r[id] = rv;
+      if(lo != nullptr) lo[id] = T(rv);
       v[0] += rv;
       v[1] += rv * rv;
       v[2] += rv * L.winv[cxy + 16 * bclass(k, L.nz)];
@@ -3667,17 +3672,19 @@ PoissonResult PoissonSolver::solve_t(const double* rhs, double* p, int maxit, do
     for(int q = 0; q < 2 * kHostMirror; ++q) hm[q] = 0.0;
   }
   if(zero_guess)
-    OFH_LAUNCH(k_setup1_zero, ll.grid, ll.block, 0, c.stream, rhs, Lc, F, r, c.partials, ll.kz);
+    OFH_LAUNCH(k_setup1_zero, ll.grid, ll.block, 0, c.stream, rhs, Lc, F, c.partials, ll.kz);
   else
     OFH_LAUNCH(k_setup1, ll.grid, ll.block, 0, c.stream, rhs, p, Lc, F, r, pc, c.partials, ll.kz);
   launch_finalize(c.stream, c.partials, ll.grid, EpiSetup1{ps, n0, zmi});
+  // the FP32 copy of the initial residual (the rhs of the finest FP32 level) comes with it for a
+  // zero warm start, else from k_cast
+  T* lo0 = mixed ? static_cast<T*>(lb[0]) : nullptr;
   if(zero_guess)
-    OFH_LAUNCH(k_setup2_zero, ll.grid, ll.block, 0, c.stream, r, F, c.partials, ps, ll.kz);
+    OFH_LAUNCH(k_setup2_zero<T>, ll.grid, ll.block, 0, c.stream, rhs, Lc, r, lo0, F, c.partials, ps, ll.kz);
   else
     OFH_LAUNCH(k_setup2, ll.grid, ll.block, 0, c.stream, r, pc, F, c.partials, ps, ll.kz);
   launch_finalize(c.stream, c.partials, ll.grid, EpiSetup2{ps, history, n0, tol, zmi});
-  if(mixed)  // FP32 copy of the initial residual: the rhs of the finest FP32 level
-    OFH_LAUNCH(k_cast<T>, 2048, 256, 0, c.stream, r, static_cast<T*>(lb[0]), F.n, ps);
+  if(mixed && !zero_guess) OFH_LAUNCH(k_cast<T>, 2048, 256, 0, c.stream, r, lo0, F.n, ps);
   vcycle<T>();
   const dim3 pg(2048);  // launch shape of the pressure passes (fixes the order of p's sum)
   int folded = 0;       // directions [0, folded) have been added to pc
