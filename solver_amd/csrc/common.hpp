@@ -284,4 +284,18 @@ __device__ inline unsigned xcd_remap(unsigned bid, unsigned nblocks) {
   return (bid % kXcd) * per + bid / kXcd;
 }
 
+// The block of a 3D grid after the XCD-aware remap of its linear id (x fastest): blocks of
+// neighbouring rows and planes, which read each other's boundary rows, then mostly run on one XCD
+// and share its L2.
+__device__ inline uint3 remapped_block3() {
+  unsigned r = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                         gridDim.x * gridDim.y * gridDim.z);
+  uint3 b;
+  b.x = r % gridDim.x;
+  r /= gridDim.x;
+  b.y = r % gridDim.y;
+  b.z = r / gridDim.y;
+  return b;
+}
+
 }  // namespace ofh

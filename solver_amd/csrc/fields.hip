@@ -171,10 +171,11 @@ __global__ void k_gradient(const double* __restrict__ p, Lay Lp, double* __restr
 __global__ void k_divergence(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
                              Lay Lu, Lay Lv, Lay Lw, Lay Lc, of_grid g, double scale, int apply_scale,
                              double* __restrict__ d, int kz) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const uint3 bk = remapped_block3();
+  const int i = 1 + (int)bk.x * blockDim.x + threadIdx.x;
+  const int j = 1 + (int)bk.y * blockDim.y + threadIdx.y;
   if(i > Lc.ax || j > Lc.ay) return;
-  const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + Lc.az);
+  const int kb = 1 + (int)bk.z * kz, ke = min(kb + kz, 1 + Lc.az);
   for(int k = kb; k < ke; ++k) {
     const double a = (u[Lu.at(i + 1, j, k)] - u[Lu.at(i, j, k)]) / g.dx;
     const double b = (v[Lv.at(i, j + 1, k)] - v[Lv.at(i, j, k)]) / g.dy;

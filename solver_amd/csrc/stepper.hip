@@ -75,14 +75,15 @@ __global__ void k_finish_periodic(const double* __restrict__ pu, const double* _
                                   double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
                                   double* __restrict__ pt, Lay Lu, Lay Lv, Lay Lw, Lay Lc, of_grid g, double scale,
                                   double sp, double dt, double* partials, int kz) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const uint3 bk = remapped_block3();
+  const int i = 1 + (int)bk.x * blockDim.x + threadIdx.x;
+  const int j = 1 + (int)bk.y * blockDim.y + threadIdx.y;
   double vals[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // the solve's final mean of phi, subtracted on every read of phi (its ghosts are copies of
   // active values taken before the subtraction): the values of the subtracted field, bitwise
   const double m = phi_mean != nullptr ? *phi_mean : 0.0;
   if(i <= g.nx && j <= g.ny) {
-    const int kb = 1 + blockIdx.z * kz, ke = min(kb + kz, 1 + g.nz);
+    const int kb = 1 + (int)bk.z * kz, ke = min(kb + kz, 1 + g.nz);
     // per-thread column offsets plus uniform plane strides (no 64-bit index products per point)
     const size_t sc = (size_t)Lc.sx * Lc.sy, su = (size_t)Lu.sx * Lu.sy, sv = (size_t)Lv.sx * Lv.sy,
                  sw = (size_t)Lw.sx * Lw.sy;
