@@ -1862,15 +1862,18 @@ __device__ inline bool pow2i(int v) { return v > 0 && (v & (v - 1)) == 0; }
 __device__ inline int log2i(int v) { return 31 - __clz(v); }
 
 // Cells per thread: cells / nt on a level of at least nt cells, else 2 (on cells / 2 threads).
-__device__ inline int col_km(const PLevel& L, int nt) { return L.n >= nt ? L.n / nt : 2; }
+// Every count here is a power of two (nt is 64 or the workgroup size, a level's sizes are checked
+// first), so quotients are shifts: a division by a run-time value is some forty dependent
+// instructions, and the small levels run on one wave, where nothing hides them.
+__device__ inline int col_km(const PLevel& L, int nt) { return L.n >= nt ? L.n >> log2i(nt) : 2; }
 __device__ inline bool col_ok(const PLevel& L, const PLevel& Cl, int nt) {
   if(!(pow2i(L.nx) && pow2i(L.ny) && pow2i(L.nz))) return false;
   if(L.nx < 2 || L.ny < 2 || L.nz < 2 || L.nx > 32) return false;
   if(L.rx != 2 || L.ry != 2 || L.rz != 2) return false;
   if(2 * Cl.nx != L.nx || 2 * Cl.ny != L.ny || 2 * Cl.nz != L.nz) return false;
-  const int km = col_km(L, nt), na = L.n / km, plane = L.nx * L.ny;
-  if(na % 64 != 0 || na > nt || km > L.nz) return false;  // whole waves own cells
-  return km >= 2 ? km % 2 == 0 : 2 * plane <= 64;
+  const int km = col_km(L, nt), na = L.n >> log2i(km), plane = L.nx * L.ny;
+  if((na & 63) != 0 || na > nt || km > L.nz) return false;  // whole waves own cells
+  return km >= 2 ? (km & 1) == 0 : 2 * plane <= 64;
 }
 
 struct ColMap {
@@ -1889,7 +1892,7 @@ __device__ inline ColMap col_map(const PLevel& L, int t, int nt) {
   c.j = (t >> lx) & (L.ny - 1);
   const int kq = t >> (lx + ly);
   c.km = col_km(L, nt);
-  c.act = t < L.n / c.km;
+  c.act = t < (L.n >> log2i(c.km));
   c.plane = L.nx * L.ny;
   c.nz = L.nz;
   c.k0 = kq * c.km;
@@ -2178,10 +2181,13 @@ __global__ void __launch_bounds__(kTailThreads)
     const T mb = T(mbc);
     for(int i = G.tid; i < n; i += G.nt) work[i] = (double)(b[i] - mb);
     G.sync();
-    if(cstage) {  // x = C (b - mb), row i on lane i
+    if(cstage) {  // x = C (b - mb), row i on lane i; unrolled, so the reads go out together
       double xi = 0.0;
-      if(G.tid < n)
-        for(int j = 0; j < n; ++j) xi += cop[G.tid * n + j] * work[j];
+      if(G.tid < n) {
+#pragma unroll
+        for(int j = 0; j < kCoarseStage; ++j)
+          if(j < n) xi += cop[G.tid * n + j] * work[j];
+      }
       G.sync();
       if(G.tid < n) work[G.tid] = xi;
       if(G.tid == 0) lm[lc][3] = 0.0;
