@@ -1959,7 +1959,43 @@ __global__ void __launch_bounds__(kTailThreads)
   __shared__ double rz_sh;
   const int s0 = P_LEVEL + 8 * l0;
   double mb0, sbw0;
-  down_fold<kTailThreads>(fold, ps, s0, (double)n0, zm, mb0, sbw0);
+  // The fold of the finer level's partials (its EpiDown) on wave 0 alone: the few dozen partials
+  // of a small level need no workgroup reduction and its barriers, and the barrier after the
+  // copy-in publishes the result. Partials are read four a lane at a time at clamped indices.
+  if(tid < 64) {
+    if(fold.src == nullptr) {
+      mb0 = ps[s0 + L_MB];
+      sbw0 = ps[s0 + L_SBW];
+    } else {
+      double r[3] = {0.0, 0.0, 0.0};
+      const unsigned last = fold.nb > 0 ? fold.nb - 1 : 0;
+      for(unsigned base = (unsigned)tid; base < fold.nb; base += 256) {
+        double x[4][3];
+#pragma unroll
+        for(int u = 0; u < 4; ++u) {
+          const unsigned b = min(base + 64u * u, last);
+#pragma unroll
+          for(int k = 0; k < 3; ++k) x[u][k] = fold.src[(size_t)k * kMaxPartials + b];
+        }
+#pragma unroll
+        for(int u = 0; u < 4; ++u) {
+          const bool in = base + 64u * u < fold.nb;
+#pragma unroll
+          for(int k = 0; k < 3; ++k) r[k] = in ? r[k] + x[u][k] : r[k];
+        }
+      }
+#pragma unroll
+      for(int k = 0; k < 3; ++k) r[k] = wave_sum_all(r[k]);
+      mb0 = zm ? r[1] / (double)n0 : 0.0;
+      sbw0 = r[2];
+      if(tid == 0) {
+        ps[fold.pslot + L_MX2] = zm ? r[0] / fold.pn : 0.0;
+        ps[s0 + L_MB] = mb0;
+        ps[s0 + L_SBW] = sbw0;
+        ps[s0 + L_SUMB] = r[1];
+      }
+    }
+  }
   if(LDS) {
     T* dst = Bp(l0);
 #pragma unroll
