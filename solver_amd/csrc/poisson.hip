@@ -830,20 +830,50 @@ __global__ void __launch_bounds__(256) k_smooth_last(const T* __restrict__ b, co
 // use different partial buffers.
 // NT: the calling block's thread count. Producers folded here have at most a few blocks per
 // thread, so two partials per value are requested at a time.
+// Folds a producer's nb partials per value into sums valid in every thread of the block (NT
+// threads, whole waves, blocks of 64 x ROWS / 2 threads indexed by threadIdx.x + 64 threadIdx.y):
+// thread t adds partials t, t + NT, ... (read two at a time at clamped indices), each wave sums
+// its threads by DPP moves, and every thread adds the wave sums in wave order after one barrier
+// (a block reduction and a broadcast took three).
 template <int N, int NT>
 __device__ inline void fold_all(const double* __restrict__ src, unsigned nb, double (&out)[N]) {
-  __shared__ double fsh[N];
-  int ops[N];
+  static_assert(NT % 64 == 0 && NT <= 1024, "whole waves");
+  constexpr int NW = NT / 64;
+  __shared__ double fred[NW][N];
+  const unsigned tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const unsigned last = nb > 0 ? nb - 1 : 0;
+  double acc[N];
 #pragma unroll
-  for(int k = 0; k < N; ++k) ops[k] = kSum;
-  fold_partials<N, 2, NT>(src, nb, ops, out);
-  if(threadIdx.x == 0 && threadIdx.y == 0) {
+  for(int k = 0; k < N; ++k) acc[k] = 0.0;
+  for(unsigned base = tid; base < nb; base += 2 * NT) {
+    double x[2][N];
 #pragma unroll
-    for(int k = 0; k < N; ++k) fsh[k] = out[k];
+    for(int u = 0; u < 2; ++u) {
+      const unsigned b = min(base + u * NT, last);
+#pragma unroll
+      for(int k = 0; k < N; ++k) x[u][k] = src[(size_t)k * kMaxPartials + b];
+    }
+#pragma unroll
+    for(int u = 0; u < 2; ++u) {
+      const bool in = base + u * NT < nb;
+#pragma unroll
+      for(int k = 0; k < N; ++k) acc[k] = in ? acc[k] + x[u][k] : acc[k];
+    }
+  }
+#pragma unroll
+  for(int k = 0; k < N; ++k) acc[k] = wave_sum_all(acc[k]);
+  if((tid & 63) == 0) {
+#pragma unroll
+    for(int k = 0; k < N; ++k) fred[tid >> 6][k] = acc[k];
   }
   __syncthreads();
 #pragma unroll
-  for(int k = 0; k < N; ++k) out[k] = fsh[k];
+  for(int k = 0; k < N; ++k) {
+    double v = fred[0][k];
+#pragma unroll
+    for(int w = 1; w < NW; ++w) v = v + fred[w][k];
+    out[k] = v;
+  }
 }
 
 // Folded EpiDown of the finer level pl into level l (the coarse one): returns mb and the sum of
