@@ -94,7 +94,8 @@ def cavity_steps_until(ctx, p, precision=sa.FP64):
 
 
 def test_c2_cavity_128_to_steady_state(ctx):
-    """BASELINE's C2 setting (max|du| <= 1e-8, Poisson tolerance 1e-10): the device follows the
+    """The 128^2 lid-driven cavity at Re 100 run to steady state at max|du| <= 1e-8 with Poisson
+    tolerance 1e-10 (the C2 configuration): the device follows the
     reference step for step and ends where the reference ends -- the reference's own run stops
     before reaching 1e-8, when the pressure solve meets its 1e-30 breakdown guard above the
     tolerance, with its non-convergence error; the device raises the same error at the same
